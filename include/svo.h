@@ -1,0 +1,194 @@
+/*
+ * svo.h -- C ABI of the MI355X (gfx950) sparse-voxel-octree hot path.
+ *
+ * This is the boundary the reference's hot-path classes would call instead of recording Vulkan
+ * command buffers.  Every entry point names the reference interface it replaces
+ * (AdamYuan/SparseVoxelOctree, paths relative to that repository).  Plain C types only: pointers,
+ * sizes and opaque handles.  A context owns one HIP stream on one device; calls on one context are
+ * ordered on that stream and must not be made from two threads at once; distinct contexts are
+ * independent.  "Enqueue" functions return before the GPU finishes; functions that return a size
+ * decided on the device (fragment count, octree words) synchronise the context's stream.
+ *
+ * Data formats (byte-identical to the reference):
+ *   fragment   uint32[2]  .x = x | y << 12 | (z & 0xff) << 24,  .y = (z >> 8) << 28 | rgb24
+ *                         (shader/voxelizer.frag:38-42)
+ *   node word  uint32     0 = empty; bit31 = non-empty; bit30 = leaf.  Leaf: bits29..24 = fragment
+ *                         count (saturates at 63), bits23..0 = RGB (R in the low byte).  Internal:
+ *                         bits29..0 = word index of the first of its 8 children.  The root is
+ *                         implicit; its children are words 0..7; child slot = x | y << 1 | z << 2
+ *                         (shader/octree_tag_node.comp:10-60, shader/octree_alloc_node.comp:21)
+ *   octree space [1,2]^3 (shader/octree.glsl:52-53); voxel v of level L spans [1+v/2^L, 1+(v+1)/2^L].
+ *   Node order: canonical breadth-first order (child blocks in ascending parent word index); leaf
+ *   colours folded in ascending (fragment.x, fragment.y) order.
+ */
+#ifndef SVO_H
+#define SVO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------------------------------
+ * Status codes.  The reference has none (Vulkan results are mostly unchecked, Scene::Create
+ * returns nullptr, src/Scene.cpp:398-401); every entry point here returns one of these.
+ * ------------------------------------------------------------------------------------------- */
+#define SVO_OK 0
+#define SVO_ERR_INVALID_ARGUMENT 1 /* null pointer, bad variant, size mismatch */
+#define SVO_ERR_BAD_LEVEL 2        /* level outside [1, 12] (src/Config.hpp:18-19) */
+#define SVO_ERR_OUT_OF_MEMORY 3    /* device allocation failed */
+#define SVO_ERR_HIP 4              /* HIP runtime error (text in svo_last_error) */
+#define SVO_ERR_OVERFLOW 5         /* more than 2^32-1 fragments, or an octree above 2^30 words */
+#define SVO_ERR_NO_DEVICE 6        /* no HIP device with that index */
+#define SVO_ERR_MALFORMED 7        /* octree violates the node format during marching */
+
+/* Opaque handles. */
+typedef struct svo_ctx svo_ctx;
+typedef struct svo_scene svo_scene;       /* triangles resident in HBM (Scene) */
+typedef struct svo_fraglist svo_fraglist; /* voxel fragment list in HBM (Voxelizer) */
+typedef struct svo_octree svo_octree;     /* octree words in HBM (OctreeBuilder / Octree) */
+
+/* Camera uniform block {vec4 pos, look, side, up} (src/Camera.hpp:28-30, src/Camera.cpp:77-95). */
+typedef struct svo_camera {
+	float pos[4];
+	float look[4];
+	float side[4];
+	float up[4];
+} svo_camera;
+
+/* Per-ray result, 32 bytes.  The reference only shades it into RGBA8
+ * (shader/octree_tracer.frag:37-47); this record keeps the raw outputs of Octree_RayMarchLeaf. */
+typedef struct svo_hit {
+	float pos[3];      /* o_pos */
+	uint32_t iter;     /* o_iter (0 for the 5-argument marcher) */
+	uint32_t vxy;      /* hit cube x | y << 16 (units of the hit cube size); 0xFFFFFFFF on a miss */
+	uint32_t vz_flags; /* z | depth << 16 | normal << 24 | hit << 31; normal = 2*axis + (sign > 0).
+	                      A miss stores 0xFFFF | normal << 24 */
+	uint32_t leaf;     /* node word at the hit (o_color = its low 24 bits) */
+	uint32_t nfetch;   /* octree words loaded by this ray */
+} svo_hit;
+
+#define SVO_MARCH_LEAF_ITER 0 /* Octree_RayMarchLeaf, 6 arguments (shader/octree.glsl:179-340) */
+#define SVO_MARCH_LEAF 1      /* Octree_RayMarchLeaf, 5 arguments (shader/octree.glsl:342-502) */
+#define SVO_MARCH_OCCLUDE 2   /* Octree_RayMarchOcclude (shader/octree.glsl:504-629): hit bit only */
+
+/* Device time of the last voxelize / build / march on a context, in milliseconds. */
+typedef struct svo_timing {
+	float normalize_ms;  /* A1 bounding box + transform */
+	float vox_count_ms;  /* count pass + scan (Voxelizer::count_and_create_fragment_list) */
+	float vox_fill_ms;   /* fill pass (Voxelizer::CmdVoxelize) */
+	float build_ms;      /* all level passes incl. the leaf fold (OctreeBuilder::CmdBuild) */
+	float beam_ms;       /* beam pre-pass (OctreeTracer::CmdBeamRenderPass) */
+	float march_ms;      /* primary march (OctreeTracer::CmdDrawPipeline) */
+	float rays_ms;       /* last svo_march_rays */
+	float bounce_ms;     /* last svo_bounce_rays */
+} svo_timing;
+
+/* ---- context (replaces myvk::Device / Queue / CommandPool set-up, src/Application.cpp:161-293) */
+int svo_ctx_create(int device, svo_ctx **out);
+int svo_ctx_destroy(svo_ctx *ctx);
+const char *svo_last_error(const svo_ctx *ctx);
+int svo_ctx_synchronize(svo_ctx *ctx);
+/* Returns the context's hipStream_t, for callers that order their own work against it. */
+void *svo_ctx_stream(svo_ctx *ctx);
+/* Device-side error bits raised by kernels since the last call (and cleared): 1 = march
+ * iteration cap, 2 = octree read out of range, 4 = traversal deeper than the stack. */
+int svo_ctx_take_device_errors(svo_ctx *ctx, uint32_t *bits);
+/* Release cached device memory held by the context's allocator. */
+int svo_ctx_trim(svo_ctx *ctx);
+/* Bytes currently held by the context's allocator: in use and cached. */
+int svo_ctx_memory(svo_ctx *ctx, uint64_t *in_use, uint64_t *cached, uint64_t *peak_in_use);
+
+/* ---- device memory helpers, for callers that keep ray batches and hit buffers in HBM */
+int svo_malloc(svo_ctx *ctx, uint64_t bytes, void **dptr);
+int svo_free(svo_ctx *ctx, void *dptr);
+int svo_upload(svo_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes);   /* synchronous */
+int svo_download(svo_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes); /* synchronous */
+
+/* ---- per-kernel profiler: when on, every kernel launch on the context is bracketed by HIP events
+ * on its stream.  svo_prof_get fills name / launches / total device ms of entry i. */
+int svo_prof_enable(svo_ctx *ctx, int on);
+int svo_prof_reset(svo_ctx *ctx);
+int svo_prof_count(svo_ctx *ctx, int *n);
+int svo_prof_get(svo_ctx *ctx, int i, const char **name, uint64_t *launches, double *total_ms);
+
+int svo_last_timing(svo_ctx *ctx, svo_timing *out); /* synchronises */
+
+/* ---- Scene (Scene::Create / load_meshes, src/Scene.cpp:36-147,387-417).
+ * tri_xyz: 9 floats per triangle (triangle soup); tri_rgb: RGB8 colour per triangle (R in the low
+ * byte; stands in for packUnorm4x8(Kd) of the triangle's material, src/Scene.cpp:164).
+ * normalize != 0 applies the reference's normalisation to [-1,1]^3 (src/Scene.cpp:90-99) on the
+ * device.  The _device variant copies from device arrays. */
+int svo_scene_create(svo_ctx *ctx, const float *tri_xyz, const uint32_t *tri_rgb, uint64_t T, int normalize,
+                     svo_scene **out);
+int svo_scene_create_device(svo_ctx *ctx, const float *tri_xyz_dev, const uint32_t *tri_rgb_dev, uint64_t T,
+                            int normalize, svo_scene **out);
+int svo_scene_info(const svo_scene *scene, uint64_t *T, const float **tri_xyz_dev, const uint32_t **tri_rgb_dev);
+int svo_scene_download(svo_ctx *ctx, const svo_scene *scene, float *tri_xyz);
+int svo_scene_destroy(svo_scene *scene);
+
+/* ---- Voxelizer (Voxelizer::Create + CmdVoxelize, src/Voxelizer.cpp:5-26,134-179).
+ * svo_voxelize_count = the count-only pass, a read-back of F and the allocation of the list
+ * (Voxelizer::count_and_create_fragment_list); svo_voxelize_fill enqueues the fill pass
+ * (Voxelizer::CmdVoxelize); svo_voxelize does both.  Fragments are stored in triangle order and,
+ * inside a triangle, row by row of the projection plane. */
+int svo_voxelize_count(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out);
+int svo_voxelize_fill(svo_ctx *ctx, svo_fraglist *frags);
+int svo_voxelize(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out);
+/* A fragment list from the host (e.g. one read back from the reference) for svo_build. */
+int svo_fraglist_create(svo_ctx *ctx, const uint32_t *frags, uint64_t F, uint32_t level, svo_fraglist **out);
+/* Voxelizer::GetLevel / GetVoxelFragmentCount / GetVoxelFragmentList (src/Voxelizer.hpp:45-52). */
+int svo_fraglist_info(const svo_fraglist *frags, uint32_t *level, uint64_t *F, const uint32_t **frags_dev);
+int svo_fraglist_download(svo_ctx *ctx, const svo_fraglist *frags, uint32_t *out, uint64_t cap_fragments);
+int svo_fraglist_destroy(svo_fraglist *frags);
+
+/* ---- OctreeBuilder (OctreeBuilder::Create / CmdBuild / GetOctreeRange, src/OctreeBuilder.cpp:8-214).
+ * svo_build_enqueue records every level pass on the stream without a host round trip
+ * (CmdBuild); svo_build_finish waits, grows the node buffer if it overflowed and returns the used
+ * range (GetOctreeRange: W = (counter + 1) * 8 words); svo_build does both. */
+int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
+int svo_build_finish(svo_ctx *ctx, svo_octree *oct, uint64_t *W);
+int svo_build(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
+/* Initial node-buffer size in words for later builds on this context (kOctreeNodeNumMin /
+ * kOctreeNodeNumMax estimate of src/OctreeBuilder.cpp:41-47 when 0, the default).  A buffer that
+ * turns out too small is grown by svo_build_finish, so this only trades memory against re-runs. */
+int svo_build_set_initial_capacity(svo_ctx *ctx, uint64_t words);
+/* An octree from host words (any node order), e.g. one read back from the reference. */
+int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t level, svo_octree **out);
+/* Octree::GetLevel / GetRange / GetBuffer (src/Octree.hpp:18-33). */
+int svo_octree_info(const svo_octree *oct, uint32_t *level, uint64_t *W, const uint32_t **words_dev);
+int svo_octree_download(svo_ctx *ctx, const svo_octree *oct, uint32_t *out, uint64_t cap_words);
+int svo_octree_destroy(svo_octree *oct);
+
+/* ---- OctreeTracer (CmdBeamRenderPass / CmdDrawPipeline, src/OctreeTracer.cpp:183-246).
+ * Primary rays through every pixel of a width x height frame from the camera; beam != 0 runs the
+ * 8x8 beam pre-pass first (m_beam_enable).  hits_dev: width*height svo_hit, row-major (y = 0 is
+ * the first row, as gl_FragCoord in Vulkan).  beam_dev (optional, may be NULL): receives the
+ * (ceil(W/8)+1) x (ceil(H/8)+1) beam image. */
+int svo_march(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
+              int beam, svo_hit *hits_dev, float *beam_dev);
+/* Explicit rays (path-tracer bounces, shader/path_tracer.comp:32-60): rays_dev = R x {o.xyz, d.xyz}
+ * floats in octree space; variant = SVO_MARCH_*. */
+int svo_march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays_dev, uint64_t R, int variant,
+                   svo_hit *hits_dev);
+/* One cosine-weighted diffuse bounce per hit (Diffuse_Sample + AlignDirection,
+ * shader/diffuse.glsl:7-14, shader/util.glsl:5-9) with a seeded, XOR-scrambled 2-D Sobol sample
+ * per ray index.  rays_dev = R x 6 floats; a ray whose hit bit is clear gets o = 0, d = -1. */
+int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t seed, float *rays_dev);
+
+/* ---- camera helper (Camera::fetch_uniform_data, src/Camera.cpp:77-95). */
+int svo_camera_from_yaw_pitch(const float position[3], float yaw, float pitch, float fov, float aspect,
+                              svo_camera *out);
+
+/* ---- synthetic scenes used by the tests and the benchmark (seeded, deterministic).
+ * kind: 1 = UV spheres (64x32, 4032 triangles each), 2 = box room + rotated boxes + spheres,
+ * 3 = spheres + thin slivers.  With tri_xyz == NULL only *T is returned (the triangle count). */
+int svo_gen_scene(int kind, uint64_t seed, uint64_t param, float *tri_xyz, uint32_t *tri_rgb, uint64_t *T);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SVO_H */

@@ -1,0 +1,171 @@
+"""CPU oracle of the SVO hot path -- TEST INFRASTRUCTURE ONLY.
+
+A C++ restatement of the reference's algorithm (oracle.cpp, built into _build/liboracle.so by
+build_oracle.py).  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package; the product (sparsevoxeloctree_amd, libsvo.so) never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+HIT_DTYPE = np.dtype(
+    [
+        ("pos", "<f4", (3,)),
+        ("iter", "<u4"),
+        ("vxy", "<u4"),
+        ("vz_flags", "<u4"),
+        ("leaf", "<u4"),
+        ("nfetch", "<u4"),
+    ]
+)
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            from . import build_oracle
+
+            build_oracle.build()
+        lib = C.CDLL(LIB_PATH)
+        P, u64, u32, i32, f32 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int, C.c_float
+        sigs = {
+            "orc_normalize": (None, [P, u64]),
+            "orc_voxelize": (u64, [P, P, u64, u32, P, u64, i32]),
+            "orc_build": (u64, [P, u64, u32, P, u64]),
+            "orc_canonicalize": (u64, [P, u64, u32, P, u64]),
+            "orc_gen_ray": (None, [P, f32, f32, P]),
+            "orc_beam": (None, [P, u64, u32, P, u32, u32, P, P, i32]),
+            "orc_march_frame": (None, [P, u64, u32, P, u32, u32, P, P, i32]),
+            "orc_march_rays": (None, [P, u64, u32, P, u64, i32, P, i32]),
+            "orc_bounce_rays": (None, [P, u64, u64, P, P]),
+            "orc_sincos": (None, [f32, P, P]),
+            "orc_stack_uninit_reads": (u64, []),
+            "orc_reset_counters": (None, []),
+        }
+        for n, (r, a) in sigs.items():
+            fn = getattr(lib, n)
+            fn.restype = r
+            fn.argtypes = a
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)
+
+
+def nthreads():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def normalize(xyz):
+    """src/Scene.cpp:62-99 normalisation; returns a new (T, 9) float32 array."""
+    out = np.ascontiguousarray(xyz, dtype=np.float32).copy()
+    load().orc_normalize(_p(out), out.size // 3)
+    return out
+
+
+def voxelize(xyz, rgb, level, threads=None):
+    """Fragment list (F, 2) uint32 in triangle order, row-major inside a triangle."""
+    lib = load()
+    xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1)
+    rgb = np.ascontiguousarray(rgb, dtype=np.uint32).reshape(-1)
+    T = rgb.shape[0]
+    th = threads or nthreads()
+    F = lib.orc_voxelize(_p(xyz), _p(rgb), T, level, None, 0, th)
+    out = np.empty(2 * max(F, 1), dtype=np.uint32)
+    lib.orc_voxelize(_p(xyz), _p(rgb), T, level, _p(out), F, th)
+    return out[: 2 * F].reshape(-1, 2)
+
+
+def build(frags, level):
+    """Canonical octree words (uint32)."""
+    lib = load()
+    frags = np.ascontiguousarray(frags, dtype=np.uint32).reshape(-1)
+    F = frags.shape[0] // 2
+    W = lib.orc_build(_p(frags), F, level, None, 0)
+    out = np.empty(W, dtype=np.uint32)
+    lib.orc_build(_p(frags), F, level, _p(out), W)
+    return out
+
+
+def canonicalize(words, level):
+    lib = load()
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    n = lib.orc_canonicalize(_p(words), words.shape[0], level, None, 0)
+    if n == 0:
+        raise ValueError("malformed octree")
+    out = np.empty(n, dtype=np.uint32)
+    lib.orc_canonicalize(_p(words), words.shape[0], level, _p(out), n)
+    return out
+
+
+def gen_ray(cam, cx, cy):
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    d = np.empty(3, dtype=np.float32)
+    load().orc_gen_ray(_p(cam), cx, cy, _p(d))
+    return d
+
+
+def beam(words, level, cam, width, height, threads=None):
+    lib = load()
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    bw, bh = (width + 7) // 8 + 1, (height + 7) // 8 + 1
+    out = np.empty(bw * bh, dtype=np.float32)
+    nf = np.empty(bw * bh, dtype=np.uint32)
+    lib.orc_beam(_p(words), words.shape[0], level, _p(cam), width, height, _p(out), _p(nf), threads or nthreads())
+    return out.reshape(bh, bw), nf.reshape(bh, bw)
+
+
+def march_frame(words, level, cam, width, height, beam_img=None, threads=None):
+    lib = load()
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    hits = np.empty(width * height, dtype=HIT_DTYPE)
+    b = None if beam_img is None else np.ascontiguousarray(beam_img, dtype=np.float32)
+    lib.orc_march_frame(_p(words), words.shape[0], level, _p(cam), width, height, _p(b), _p(hits), threads or nthreads())
+    return hits.reshape(height, width)
+
+
+def march_rays(words, level, rays, variant=0, threads=None):
+    lib = load()
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+    hits = np.empty(rays.shape[0], dtype=HIT_DTYPE)
+    lib.orc_march_rays(_p(words), words.shape[0], level, _p(rays), rays.shape[0], variant, _p(hits), threads or nthreads())
+    return hits
+
+
+def bounce_rays(hits, seed):
+    lib = load()
+    hits = np.ascontiguousarray(hits.reshape(-1))
+    R = hits.shape[0]
+    rays = np.empty((R, 6), dtype=np.float32)
+    mask = np.empty(R, dtype=np.uint8)
+    lib.orc_bounce_rays(_p(hits), R, seed, _p(rays), _p(mask))
+    return rays, mask.astype(bool)
+
+
+def sincos(phi):
+    s, c = C.c_float(), C.c_float()
+    load().orc_sincos(phi, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def stack_uninit_reads():
+    return load().orc_stack_uninit_reads()
+
+
+def reset_counters():
+    load().orc_reset_counters()

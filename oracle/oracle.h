@@ -1,0 +1,93 @@
+/*
+ * CPU oracle for the sparse-voxel-octree hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker the HIP path is compared against.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product library (libsvo.so) never links, loads or calls it.
+ *
+ * Every function restates the reference's algorithm (AdamYuan/SparseVoxelOctree,
+ * GLSL shaders and host C++) in plain C++ with the fixed floating-point rules:
+ *   - IEEE-754 binary32, round to nearest, no contraction (-ffp-contract=off);
+ *   - correctly rounded '/' and sqrtf;
+ *   - normalize(v) = v * (1.0f / sqrtf((v.x*v.x + v.y*v.y) + v.z*v.z));
+ *   - GLSL min(x,y) = (y < x) ? y : x, max(x,y) = (x < y) ? y : x;
+ *   - float -> uint conversion saturates (negative / NaN -> 0);
+ *   - findMSB(0) = 0xFFFFFFFF;
+ *   - the ESVO traversal stack starts zeroed, and reads of never-written
+ *     entries are counted.
+ * Canonical orders (the reference's atomics make both nondeterministic):
+ *   - leaf colours are folded in ascending (fragment.x, fragment.y) order;
+ *   - octree blocks are laid out by a serial allocator in word-index order.
+ */
+#ifndef SVO_ORACLE_H
+#define SVO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same 32-byte layout as svo_hit in include/svo.h. */
+typedef struct orc_hit {
+	float pos[3];      /* o_pos of Octree_RayMarchLeaf */
+	uint32_t iter;     /* o_iter (0 for the 5-argument variant) */
+	uint32_t vxy;      /* hit cube coordinate x | y << 16, 0xFFFFFFFF on a miss */
+	uint32_t vz_flags; /* z | depth << 16 | normal code << 24 | hit << 31 */
+	uint32_t leaf;     /* node word at the hit (colour + count for leaves) */
+	uint32_t nfetch;   /* executed octree loads */
+} orc_hit;
+
+/* Scene::load_meshes normalisation to [-1,1]^3 (src/Scene.cpp:62-99). In place on xyz[3*nverts]. */
+void orc_normalize(float *xyz, uint64_t nverts);
+
+/* Voxelizer (shader/voxelizer.geom:15-54, shader/voxelizer.frag:17-44, conservative rule).
+ * tri: 9 floats per triangle (already normalised); rgb: colour per triangle (R in the low byte).
+ * Returns the fragment count F.  When frags != NULL, writes min(F, cap) fragments (2 uint32 each)
+ * in triangle order, row-major (j outer, i inner) inside a triangle. */
+uint64_t orc_voxelize(const float *tri, const uint32_t *rgb, uint64_t T, uint32_t level, uint32_t *frags,
+                      uint64_t cap, int nthreads);
+
+/* OctreeBuilder (shader/octree_{init,tag,alloc}_node.comp, octree_modify_arg.comp,
+ * src/OctreeBuilder.cpp:142-214).  Returns W = (counter + 1) * 8 words; writes min(W, cap) words. */
+uint64_t orc_build(const uint32_t *frags, uint64_t F, uint32_t level, uint32_t *words, uint64_t cap);
+
+/* Breadth-first relayout of any octree in the reference node format into the canonical block order.
+ * Returns the canonical word count (0 on a malformed octree); writes min(result, cap) words. */
+uint64_t orc_canonicalize(const uint32_t *words, uint64_t W, uint32_t level, uint32_t *out, uint64_t cap);
+
+/* Camera_GenRay (shader/camera.glsl:11-14).  cam = {pos, look, side, up} as 4 x vec4. */
+void orc_gen_ray(const float *cam, float cx, float cy, float *d_out);
+
+/* Beam pre-pass (shader/octree_tracer_beam.frag:12-29): bw x bh = (ceil(W/8)+1) x (ceil(H/8)+1) floats. */
+void orc_beam(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+              uint32_t height, float *beam, uint32_t *beam_nfetch, int nthreads);
+
+/* Primary-ray frame (shader/octree_tracer.frag:23-48 + Octree_RayMarchLeaf 6-arg).
+ * beam may be NULL (beam disabled).  hits[width*height], row-major. */
+void orc_march_frame(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                     uint32_t height, const float *beam, orc_hit *hits, int nthreads);
+
+/* Explicit rays: rays[6*R] = (o.xyz, d.xyz).  variant 0 = Octree_RayMarchLeaf 6-arg
+ * (shader/octree.glsl:179-340), 1 = 5-arg (shader/octree.glsl:342-502), 2 = Octree_RayMarchOcclude
+ * (shader/octree.glsl:504-629; hit flag only, other fields as documented in include/svo.h). */
+void orc_march_rays(const uint32_t *words, uint64_t W, uint32_t level, const float *rays, uint64_t R, int variant,
+                    orc_hit *hits, int nthreads);
+
+/* One cosine-weighted diffuse bounce from primary hits (shader/path_tracer.comp:38-44,
+ * shader/diffuse.glsl:7-14, shader/util.glsl:5-9) with the seeded Sobol sample of pixel i.
+ * rays[6*R]; a miss gets o = d = 0 and is flagged in mask[i] = 0 (mask may be NULL). */
+void orc_bounce_rays(const orc_hit *hits, uint64_t R, uint64_t seed, float *rays, uint8_t *mask);
+
+/* sin / cos of phi used by the bounce sampler (own polynomial; identical on the device). */
+void orc_sincos(float phi, float *s, float *c);
+
+/* Number of stack reads of never-written entries seen since the last reset (must stay 0). */
+uint64_t orc_stack_uninit_reads(void);
+void orc_reset_counters(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,23 @@
+"""MI355X-native sparse-voxel-octree hot path: voxelize -> level-by-level octree build -> ESVO
+ray marching, as hand-written HIP kernels for gfx950 behind a C ABI (include/svo.h).
+
+The compute lives in lib/libsvo.so; this package is the ctypes binding plus a Python mirror of the
+reference's Voxelizer / OctreeBuilder / Octree / OctreeTracer class surface.  Importing the
+package fails if the library has not been built: there is no CPU fallback.
+"""
+from .api import (  # noqa: F401
+    HIT_DTYPE,
+    Camera,
+    Context,
+    DeviceArray,
+    Octree,
+    OctreeBuilder,
+    OctreeTracer,
+    Scene,
+    SvoError,
+    Voxelizer,
+    camera_look_at,
+    gen_scene,
+    march_rays,
+)
+from ._capi import LIB_PATH, exported_symbols  # noqa: F401

@@ -1,0 +1,135 @@
+"""ctypes binding of libsvo.so (the C ABI declared in include/svo.h).
+
+The library is loaded from sparsevoxeloctree_amd/lib/libsvo.so (built by build.py).  There is no
+fallback: when the library is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libsvo.so")
+
+SVO_OK = 0
+ERRORS = {
+    1: "SVO_ERR_INVALID_ARGUMENT",
+    2: "SVO_ERR_BAD_LEVEL",
+    3: "SVO_ERR_OUT_OF_MEMORY",
+    4: "SVO_ERR_HIP",
+    5: "SVO_ERR_OVERFLOW",
+    6: "SVO_ERR_NO_DEVICE",
+    7: "SVO_ERR_MALFORMED",
+}
+SVO_MARCH_LEAF_ITER, SVO_MARCH_LEAF, SVO_MARCH_OCCLUDE = 0, 1, 2
+
+HIT_DTYPE = np.dtype(
+    [
+        ("pos", "<f4", (3,)),
+        ("iter", "<u4"),
+        ("vxy", "<u4"),
+        ("vz_flags", "<u4"),
+        ("leaf", "<u4"),
+        ("nfetch", "<u4"),
+    ]
+)
+assert HIT_DTYPE.itemsize == 32
+
+
+class svo_camera(C.Structure):
+    _fields_ = [("pos", C.c_float * 4), ("look", C.c_float * 4), ("side", C.c_float * 4), ("up", C.c_float * 4)]
+
+
+class svo_timing(C.Structure):
+    _fields_ = [
+        (n, C.c_float)
+        for n in (
+            "normalize_ms",
+            "vox_count_ms",
+            "vox_fill_ms",
+            "build_ms",
+            "beam_ms",
+            "march_ms",
+            "rays_ms",
+            "bounce_ms",
+        )
+    ]
+
+
+class SvoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        "libsvo.so not found at %s: build it with `python -m sparsevoxeloctree_amd.build` "
+        "(there is no CPU fallback)" % LIB_PATH
+    )
+
+lib = C.CDLL(LIB_PATH)
+
+P = C.c_void_p
+u32, u64, i32, f32 = C.c_uint32, C.c_uint64, C.c_int, C.c_float
+PP = C.POINTER(C.c_void_p)
+
+_SIGS = {
+    "svo_ctx_create": (i32, [i32, PP]),
+    "svo_ctx_destroy": (i32, [P]),
+    "svo_last_error": (C.c_char_p, [P]),
+    "svo_ctx_synchronize": (i32, [P]),
+    "svo_ctx_stream": (P, [P]),
+    "svo_ctx_take_device_errors": (i32, [P, C.POINTER(u32)]),
+    "svo_ctx_trim": (i32, [P]),
+    "svo_ctx_memory": (i32, [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
+    "svo_malloc": (i32, [P, u64, PP]),
+    "svo_free": (i32, [P, P]),
+    "svo_upload": (i32, [P, P, P, u64]),
+    "svo_download": (i32, [P, P, P, u64]),
+    "svo_prof_enable": (i32, [P, i32]),
+    "svo_prof_reset": (i32, [P]),
+    "svo_prof_count": (i32, [P, C.POINTER(i32)]),
+    "svo_prof_get": (i32, [P, i32, C.POINTER(C.c_char_p), C.POINTER(u64), C.POINTER(C.c_double)]),
+    "svo_last_timing": (i32, [P, C.POINTER(svo_timing)]),
+    "svo_scene_create": (i32, [P, P, P, u64, i32, PP]),
+    "svo_scene_create_device": (i32, [P, P, P, u64, i32, PP]),
+    "svo_scene_info": (i32, [P, C.POINTER(u64), PP, PP]),
+    "svo_scene_download": (i32, [P, P, P]),
+    "svo_scene_destroy": (i32, [P]),
+    "svo_voxelize_count": (i32, [P, P, u32, PP]),
+    "svo_voxelize_fill": (i32, [P, P]),
+    "svo_voxelize": (i32, [P, P, u32, PP]),
+    "svo_fraglist_create": (i32, [P, P, u64, u32, PP]),
+    "svo_fraglist_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
+    "svo_fraglist_download": (i32, [P, P, P, u64]),
+    "svo_fraglist_destroy": (i32, [P]),
+    "svo_build_enqueue": (i32, [P, P, PP]),
+    "svo_build_finish": (i32, [P, P, C.POINTER(u64)]),
+    "svo_build": (i32, [P, P, PP]),
+    "svo_build_set_initial_capacity": (i32, [P, u64]),
+    "svo_octree_create": (i32, [P, P, u64, u32, PP]),
+    "svo_octree_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
+    "svo_octree_download": (i32, [P, P, P, u64]),
+    "svo_octree_destroy": (i32, [P]),
+    "svo_march": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, P, P]),
+    "svo_march_rays": (i32, [P, P, P, u64, i32, P]),
+    "svo_bounce_rays": (i32, [P, P, u64, u64, P]),
+    "svo_camera_from_yaw_pitch": (i32, [C.POINTER(C.c_float), f32, f32, f32, f32, C.POINTER(svo_camera)]),
+    "svo_gen_scene": (i32, [i32, u64, u64, P, P, C.POINTER(u64)]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _fn = getattr(lib, _name)
+    _fn.restype = _res
+    _fn.argtypes = _args
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def ptr(a):
+    """Host pointer of a contiguous numpy array."""
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)

@@ -1,0 +1,413 @@
+"""Python mirror of the reference's hot-path class surface over the C ABI (include/svo.h).
+
+Same class names and method roles as AdamYuan/SparseVoxelOctree's C++ classes, re-based on a
+context (one HIP stream) instead of Vulkan command pools; "Cmd*" methods enqueue on the stream:
+
+    Scene.Create                 src/Scene.cpp:387-417 (+ normalisation src/Scene.cpp:90-99)
+    Voxelizer.Create / CmdVoxelize / GetLevel / GetVoxelResolution / GetVoxelFragmentCount /
+        GetVoxelFragmentList     src/Voxelizer.hpp:43-52
+    OctreeBuilder.Create / CmdBuild / GetOctreeRange / GetOctree / GetLevel
+                                 src/OctreeBuilder.hpp:36-48
+    Octree.Update / GetLevel / GetRange / Empty
+                                 src/Octree.hpp:18-33
+    Camera (m_position, m_yaw, m_pitch, m_fov, m_aspect_ratio)
+                                 src/Camera.hpp:14-17
+    OctreeTracer.Create / Resize / CmdBeamRenderPass / CmdDrawPipeline, m_beam_enable
+                                 src/OctreeTracer.hpp:15-16,57-70
+
+Errors raise SvoError carrying the library's status code.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _capi
+from ._capi import HIT_DTYPE, SvoError, lib, ptr, svo_camera, svo_timing
+
+__all__ = [
+    "Context",
+    "DeviceArray",
+    "Scene",
+    "Voxelizer",
+    "OctreeBuilder",
+    "Octree",
+    "Camera",
+    "OctreeTracer",
+    "gen_scene",
+    "camera_look_at",
+    "HIT_DTYPE",
+    "SvoError",
+]
+
+
+class Context:
+    """One HIP stream on one device (the reference's device + queue + command pool)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        rc = lib.svo_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise SvoError(rc, "svo_ctx_create(%d) failed" % device)
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc != 0:
+            msg = lib.svo_last_error(self.h)
+            raise SvoError(rc, (what + ": " if what else "") + (msg.decode() if msg else ""))
+
+    def close(self):
+        if self.h:
+            lib.svo_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        self.check(lib.svo_ctx_synchronize(self.h), "synchronize")
+
+    def alloc(self, nbytes):
+        return DeviceArray(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        d = DeviceArray(self, arr.nbytes)
+        self.check(lib.svo_upload(self.h, d.ptr, ptr(arr), arr.nbytes), "upload")
+        return d
+
+    def timing(self):
+        t = svo_timing()
+        self.check(lib.svo_last_timing(self.h, C.byref(t)), "timing")
+        return {n: getattr(t, n) for n, _ in svo_timing._fields_}
+
+    def device_errors(self):
+        b = C.c_uint32()
+        self.check(lib.svo_ctx_take_device_errors(self.h, C.byref(b)), "device errors")
+        return b.value
+
+    def memory(self):
+        a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        self.check(lib.svo_ctx_memory(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"in_use": a.value, "cached": b.value, "peak_in_use": c.value}
+
+    def prof_enable(self, on=True):
+        self.check(lib.svo_prof_enable(self.h, 1 if on else 0))
+
+    def prof_reset(self):
+        self.check(lib.svo_prof_reset(self.h))
+
+    def prof_stats(self):
+        n = C.c_int()
+        self.check(lib.svo_prof_count(self.h, C.byref(n)))
+        out = {}
+        for i in range(n.value):
+            name, cnt, ms = C.c_char_p(), C.c_uint64(), C.c_double()
+            self.check(lib.svo_prof_get(self.h, i, C.byref(name), C.byref(cnt), C.byref(ms)))
+            out[name.value.decode()] = (cnt.value, ms.value)
+        return out
+
+
+class DeviceArray:
+    """Device memory from the context's allocator."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx.check(lib.svo_malloc(ctx.h, max(self.nbytes, 1), C.byref(p)), "svo_malloc")
+        self.ptr = p
+
+    def download(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            self.ctx.check(lib.svo_download(self.ctx.h, ptr(out), self.ptr, out.nbytes), "download")
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ctx.h:
+            lib.svo_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Scene:
+    """Triangles resident in HBM.  xyz: (T, 9) float32, rgb: (T,) uint32 (R in the low byte)."""
+
+    def __init__(self, ctx, handle, T):
+        self.ctx, self.h, self.T = ctx, handle, T
+
+    @staticmethod
+    def Create(ctx, xyz, rgb, normalize=True):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1)
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint32).reshape(-1)
+        T = rgb.shape[0]
+        assert xyz.shape[0] == 9 * T
+        h = C.c_void_p()
+        ctx.check(lib.svo_scene_create(ctx.h, ptr(xyz), ptr(rgb), T, 1 if normalize else 0, C.byref(h)), "scene")
+        return Scene(ctx, h, T)
+
+    def GetTriangleCount(self):
+        return self.T
+
+    def download(self):
+        out = np.empty(9 * self.T, dtype=np.float32)
+        self.ctx.check(lib.svo_scene_download(self.ctx.h, self.h, ptr(out)), "scene download")
+        return out.reshape(-1, 9)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib.svo_scene_destroy(self.h)
+            self.h = None
+
+
+class Voxelizer:
+    """Voxelizer::Create runs the count pass and allocates the list; CmdVoxelize fills it."""
+
+    def __init__(self, ctx, handle, level, scene=None):
+        self.ctx, self.h, self.level, self._scene = ctx, handle, level, scene
+
+    @staticmethod
+    def Create(scene, octree_level):
+        ctx = scene.ctx
+        h = C.c_void_p()
+        ctx.check(lib.svo_voxelize_count(ctx.h, scene.h, octree_level, C.byref(h)), "voxelize count")
+        return Voxelizer(ctx, h, octree_level, scene)
+
+    @staticmethod
+    def FromFragments(ctx, frags, octree_level):
+        """A fragment list from the host (uint32 pairs), e.g. read back from the reference."""
+        frags = np.ascontiguousarray(frags, dtype=np.uint32).reshape(-1)
+        h = C.c_void_p()
+        ctx.check(lib.svo_fraglist_create(ctx.h, ptr(frags), frags.shape[0] // 2, octree_level, C.byref(h)), "frags")
+        return Voxelizer(ctx, h, octree_level)
+
+    def CmdVoxelize(self):
+        self.ctx.check(lib.svo_voxelize_fill(self.ctx.h, self.h), "voxelize fill")
+
+    def GetLevel(self):
+        return self.level
+
+    def GetVoxelResolution(self):
+        return 1 << self.level
+
+    def GetVoxelFragmentCount(self):
+        F = C.c_uint64()
+        self.ctx.check(lib.svo_fraglist_info(self.h, None, C.byref(F), None))
+        return F.value
+
+    def GetVoxelFragmentList(self):
+        """Downloads the fragment list: (F, 2) uint32."""
+        F = self.GetVoxelFragmentCount()
+        out = np.empty(2 * F, dtype=np.uint32)
+        if F:
+            self.ctx.check(lib.svo_fraglist_download(self.ctx.h, self.h, ptr(out), F), "fragment download")
+        return out.reshape(-1, 2)
+
+    def device_ptr(self):
+        p = C.c_void_p()
+        self.ctx.check(lib.svo_fraglist_info(self.h, None, None, C.byref(p)))
+        return p
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib.svo_fraglist_destroy(self.h)
+            self.h = None
+
+
+class Octree:
+    """Octree::Update takes the builder's buffer and its used range."""
+
+    def __init__(self, ctx=None, handle=None, level=0, W=0):
+        self.ctx, self.h, self.level, self.W = ctx, handle, level, W
+        self._owner = None
+
+    @staticmethod
+    def FromWords(ctx, words, level):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        h = C.c_void_p()
+        ctx.check(lib.svo_octree_create(ctx.h, ptr(words), words.shape[0], level, C.byref(h)), "octree create")
+        o = Octree(ctx, h, level, words.shape[0])
+        o._own = True
+        return o
+
+    def Update(self, builder):
+        W = builder.GetOctreeRange() // 4
+        self.ctx, self.h, self.level, self.W = builder.ctx, builder.h, builder.level, W
+        self._owner = builder
+
+    def Empty(self):
+        return self.h is None
+
+    def GetLevel(self):
+        return self.level
+
+    def GetRange(self):
+        return self.W * 4
+
+    def words(self):
+        out = np.empty(self.W, dtype=np.uint32)
+        self.ctx.check(lib.svo_octree_download(self.ctx.h, self.h, ptr(out), self.W), "octree download")
+        return out
+
+    def __del__(self):
+        if getattr(self, "_own", False) and self.h and self.ctx.h:
+            lib.svo_octree_destroy(self.h)
+            self.h = None
+
+
+class OctreeBuilder:
+    """CmdBuild enqueues every level pass; GetOctreeRange waits and returns the used bytes."""
+
+    def __init__(self, voxelizer):
+        self.ctx, self.voxelizer, self.level = voxelizer.ctx, voxelizer, voxelizer.level
+        self.h = None
+        self.W = None
+
+    @staticmethod
+    def Create(voxelizer):
+        return OctreeBuilder(voxelizer)
+
+    def GetLevel(self):
+        return self.level
+
+    def CmdBuild(self):
+        if self.h is not None:
+            lib.svo_octree_destroy(self.h)
+        h = C.c_void_p()
+        self.ctx.check(lib.svo_build_enqueue(self.ctx.h, self.voxelizer.h, C.byref(h)), "build")
+        self.h, self.W = h, None
+
+    def GetOctreeRange(self):
+        if self.W is None:
+            W = C.c_uint64()
+            self.ctx.check(lib.svo_build_finish(self.ctx.h, self.h, C.byref(W)), "build finish")
+            self.W = W.value
+        return self.W * 4
+
+    def GetOctree(self):
+        o = Octree()
+        o.Update(self)
+        return o
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib.svo_octree_destroy(self.h)
+            self.h = None
+
+
+class Camera:
+    """Camera::fetch_uniform_data (src/Camera.cpp:77-95)."""
+
+    def __init__(self, position=(1.5, 1.5, 1.5), yaw=0.0, pitch=0.0, fov=math.pi / 3.0, aspect=1280.0 / 720.0):
+        self.m_position = tuple(float(x) for x in position)
+        self.m_yaw, self.m_pitch, self.m_fov, self.m_aspect_ratio = yaw, pitch, fov, aspect
+
+    def uniform(self):
+        cam = svo_camera()
+        pos = (C.c_float * 3)(*self.m_position)
+        rc = lib.svo_camera_from_yaw_pitch(pos, self.m_yaw, self.m_pitch, self.m_fov, self.m_aspect_ratio, C.byref(cam))
+        if rc:
+            raise SvoError(rc, "camera")
+        return cam
+
+    def uniform_array(self):
+        c = self.uniform()
+        return np.array(list(c.pos) + list(c.look) + list(c.side) + list(c.up), dtype=np.float32)
+
+
+def camera_look_at(eye, target, fov=math.pi / 3.0, aspect=1.0):
+    """Camera at `eye` (octree space) looking at `target`, via the reference's yaw/pitch form."""
+    d = np.asarray(target, dtype=np.float64) - np.asarray(eye, dtype=np.float64)
+    d /= np.linalg.norm(d)
+    pitch = math.asin(max(-1.0, min(1.0, d[1])))
+    yaw = math.atan2(d[0], d[2])
+    return Camera(eye, yaw, pitch, fov, aspect)
+
+
+class OctreeTracer:
+    """Primary-ray marching of a whole frame into a device hit buffer (svo_hit records)."""
+
+    def __init__(self, octree, camera):
+        self.octree, self.camera = octree, camera
+        self.ctx = octree.ctx
+        self.m_beam_enable = True
+        self.width, self.height = 1280, 720
+        self.hits = None
+        self.beam = None
+
+    @staticmethod
+    def Create(octree, camera):
+        return OctreeTracer(octree, camera)
+
+    def Resize(self, width, height):
+        self.width, self.height = int(width), int(height)
+        self.hits = None
+
+    def _ensure(self):
+        if self.hits is None or self.hits.nbytes != self.width * self.height * 32:
+            self.hits = self.ctx.alloc(self.width * self.height * 32)
+            bw, bh = (self.width + 7) // 8 + 1, (self.height + 7) // 8 + 1
+            self.beam = self.ctx.alloc(bw * bh * 4)
+
+    def CmdBeamRenderPass(self):
+        """The beam pre-pass runs inside CmdDrawPipeline when m_beam_enable is set."""
+
+    def CmdDrawPipeline(self):
+        self._ensure()
+        cam = self.camera.uniform() if isinstance(self.camera, Camera) else self.camera
+        self.ctx.check(
+            lib.svo_march(
+                self.ctx.h,
+                self.octree.h,
+                C.byref(cam),
+                self.width,
+                self.height,
+                1 if self.m_beam_enable else 0,
+                self.hits.ptr,
+                self.beam.ptr if self.m_beam_enable else None,
+            ),
+            "march",
+        )
+
+    def GetHits(self):
+        return self.hits.download(HIT_DTYPE, self.width * self.height).reshape(self.height, self.width)
+
+    def GetBeam(self):
+        bw, bh = (self.width + 7) // 8 + 1, (self.height + 7) // 8 + 1
+        return self.beam.download(np.float32, bw * bh).reshape(bh, bw)
+
+
+def march_rays(ctx, octree, rays, variant=_capi.SVO_MARCH_LEAF_ITER):
+    """Marches explicit rays (R, 6) float32 in octree space; returns hit records."""
+    rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)
+    R = rays.shape[0]
+    d_rays = ctx.upload(rays)
+    d_hits = ctx.alloc(R * 32)
+    ctx.check(lib.svo_march_rays(ctx.h, octree.h, d_rays.ptr, R, variant, d_hits.ptr), "march rays")
+    return d_hits.download(HIT_DTYPE, R)
+
+
+def gen_scene(kind, seed, param=0):
+    """Seeded synthetic scene: (xyz (T, 9) float32, rgb (T,) uint32)."""
+    T = C.c_uint64()
+    rc = lib.svo_gen_scene(kind, seed, param, None, None, C.byref(T))
+    if rc:
+        raise SvoError(rc, "gen_scene")
+    xyz = np.empty((T.value, 9), dtype=np.float32)
+    rgb = np.empty(T.value, dtype=np.uint32)
+    rc = lib.svo_gen_scene(kind, seed, param, ptr(xyz), ptr(rgb), C.byref(T))
+    if rc:
+        raise SvoError(rc, "gen_scene")
+    return xyz, rgb

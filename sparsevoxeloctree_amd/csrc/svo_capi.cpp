@@ -1,0 +1,868 @@
+// C ABI of the SVO library (include/svo.h) and the host runtime pieces (allocator, profiler,
+// stage timers).  Compiled by hipcc with the kernels into libsvo.so.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "svo_runtime.hpp"
+
+namespace svo {
+
+int set_error(svo_ctx *ctx, int code, const std::string &msg) {
+	if (ctx)
+		ctx->last_error = msg;
+	return code;
+}
+
+int hip_check(svo_ctx *ctx, hipError_t e, const char *what) {
+	if (e == hipSuccess)
+		return SVO_OK;
+	std::string m = std::string(what) + ": " + hipGetErrorString(e);
+	return set_error(ctx, e == hipErrorOutOfMemory ? SVO_ERR_OUT_OF_MEMORY : SVO_ERR_HIP, m);
+}
+
+// ---- Pool: size-class caching allocator.  Blocks are reused only by later work on the owning
+// context's stream, so reuse is ordered after every earlier use without extra synchronisation.
+Pool::~Pool() {
+	for (auto &kv : m_free)
+		(void)hipFree(kv.second);
+	for (auto &kv : m_live)
+		(void)hipFree(kv.first);
+}
+
+uint64_t Pool::round_size(uint64_t bytes) {
+	if (bytes < 256)
+		bytes = 256;
+	if (bytes <= (64ull << 20)) {
+		uint64_t p = 256;
+		while (p < bytes)
+			p <<= 1;
+		return p;
+	}
+	const uint64_t g = 64ull << 20;
+	return (bytes + g - 1) / g * g;
+}
+
+hipError_t Pool::alloc(void **p, uint64_t bytes) {
+	const uint64_t sz = round_size(bytes);
+	auto it = m_free.find(sz);
+	if (it != m_free.end()) {
+		*p = it->second;
+		m_free.erase(it);
+		m_cached -= sz;
+	} else {
+		hipError_t e = hipMalloc(p, sz);
+		if (e != hipSuccess) {
+			(void)hipGetLastError();
+			trim(); // give cached blocks back and retry once
+			e = hipMalloc(p, sz);
+			if (e != hipSuccess) {
+				(void)hipGetLastError();
+				*p = nullptr;
+				return e;
+			}
+		}
+	}
+	m_live[*p] = sz;
+	m_in_use += sz;
+	m_peak = std::max(m_peak, m_in_use);
+	return hipSuccess;
+}
+
+void Pool::free(void *p) {
+	if (!p)
+		return;
+	auto it = m_live.find(p);
+	if (it == m_live.end())
+		return;
+	m_in_use -= it->second;
+	m_cached += it->second;
+	m_free.emplace(it->second, p);
+	m_live.erase(it);
+}
+
+void Pool::trim() {
+	for (auto &kv : m_free)
+		(void)hipFree(kv.second);
+	m_free.clear();
+	m_cached = 0;
+}
+
+// ---- Profiler
+hipEvent_t Profiler::get_event() {
+	if (!m_free_events.empty()) {
+		hipEvent_t e = m_free_events.back();
+		m_free_events.pop_back();
+		return e;
+	}
+	hipEvent_t e = nullptr;
+	(void)hipEventCreate(&e);
+	return e;
+}
+
+void Profiler::begin(hipStream_t s, const char *name) {
+	int idx = -1;
+	for (size_t i = 0; i < entries.size(); ++i)
+		if (entries[i].name == name) {
+			idx = (int)i;
+			break;
+		}
+	if (idx < 0) {
+		entries.push_back(ProfEntry{name, 0, 0.0});
+		idx = (int)entries.size() - 1;
+	}
+	m_open = idx;
+	m_open_ev = get_event();
+	(void)hipEventRecord(m_open_ev, s);
+}
+
+void Profiler::end(hipStream_t s) {
+	if (m_open < 0)
+		return;
+	hipEvent_t b = get_event();
+	(void)hipEventRecord(b, s);
+	m_pending.push_back(Pending{m_open, m_open_ev, b});
+	m_open = -1;
+	m_open_ev = nullptr;
+	if (m_pending.size() > 4096)
+		resolve();
+}
+
+void Profiler::resolve() {
+	for (auto &p : m_pending) {
+		(void)hipEventSynchronize(p.b);
+		float ms = 0.0f;
+		if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+			entries[p.entry].launches += 1;
+			entries[p.entry].total_ms += ms;
+		}
+		m_free_events.push_back(p.a);
+		m_free_events.push_back(p.b);
+	}
+	m_pending.clear();
+}
+
+void Profiler::reset() {
+	resolve();
+	entries.clear();
+}
+
+void StageTimer::start(hipStream_t s) {
+	if (!a) {
+		(void)hipEventCreate(&a);
+		(void)hipEventCreate(&b);
+	}
+	(void)hipEventRecord(a, s);
+	armed = false;
+}
+void StageTimer::stop(hipStream_t s) {
+	if (!b)
+		return;
+	(void)hipEventRecord(b, s);
+	armed = true;
+}
+float StageTimer::read() {
+	if (!armed)
+		return 0.0f;
+	(void)hipEventSynchronize(b);
+	float ms = 0.0f;
+	if (hipEventElapsedTime(&ms, a, b) != hipSuccess)
+		return 0.0f;
+	return ms;
+}
+void StageTimer::destroy() {
+	if (a)
+		(void)hipEventDestroy(a);
+	if (b)
+		(void)hipEventDestroy(b);
+	a = b = nullptr;
+}
+
+} // namespace svo
+
+using namespace svo;
+
+namespace {
+
+struct DeviceGuard { // makes the context's device current for the call
+	int prev = -1;
+	explicit DeviceGuard(const svo_ctx *ctx) {
+		if (hipGetDevice(&prev) != hipSuccess)
+			prev = -1;
+		if (ctx && prev != ctx->device)
+			(void)hipSetDevice(ctx->device);
+	}
+	~DeviceGuard() {
+		int cur = -1;
+		if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev)
+			(void)hipSetDevice(prev);
+	}
+};
+
+// Initial node-buffer size: the reference's estimate (src/OctreeBuilder.cpp:41-47,
+// clamp(F * (L / 3), 1e6, 5e8) words) bounded by the most words the level structure allows
+// (8 + 8 * sum_{d<L} min(F, 8^d)); grown on overflow by svo_build_finish.
+uint64_t initial_capacity(uint64_t F, uint32_t L) {
+	uint64_t bound = 8, p = 1;
+	for (uint32_t d = 1; d < L; ++d) {
+		p *= 8;
+		bound += 8 * std::min<uint64_t>(F, p);
+	}
+	uint64_t est = std::max<uint64_t>(1000000ull, F * std::max<uint32_t>(1u, L / 3u));
+	uint64_t cap = std::min(bound, est);
+	cap = std::max<uint64_t>(cap, 8);
+	return std::min<uint64_t>(cap, 1ull << 30);
+}
+
+void release_build_scratch(svo_octree *o) {
+	svo_ctx *ctx = o->ctx;
+	ctx->pool->free(o->aux);
+	ctx->pool->free(o->node);
+	ctx->pool->free(o->bucket);
+	ctx->pool->free(o->big);
+	o->aux = o->node = o->bucket = o->big = nullptr;
+}
+
+} // namespace
+
+extern "C" {
+
+int svo_ctx_create(int device, svo_ctx **out) {
+	if (!out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+		return SVO_ERR_NO_DEVICE;
+	svo_ctx *ctx = new (std::nothrow) svo_ctx();
+	if (!ctx)
+		return SVO_ERR_OUT_OF_MEMORY;
+	ctx->device = device;
+	DeviceGuard g(ctx);
+	hipError_t e;
+	if ((e = hipSetDevice(device)) != hipSuccess || (e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking)) != hipSuccess) {
+		delete ctx;
+		return SVO_ERR_HIP;
+	}
+	hipDeviceProp_t prop;
+	if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+		ctx->cu_count = prop.multiProcessorCount;
+	ctx->pool = new Pool(device);
+	if (hipMalloc((void **)&ctx->d_scan_parts, sizeof(uint64_t) * (2 * kScanBlocks + 8)) != hipSuccess ||
+	    hipMalloc((void **)&ctx->d_state, sizeof(BuildState)) != hipSuccess ||
+	    hipMalloc((void **)&ctx->d_err, sizeof(uint32_t)) != hipSuccess ||
+	    hipHostMalloc((void **)&ctx->h_state, sizeof(BuildState) + 64, hipHostMallocDefault) != hipSuccess) {
+		svo_ctx_destroy(ctx);
+		return SVO_ERR_OUT_OF_MEMORY;
+	}
+	(void)hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream);
+	(void)hipMemsetAsync(ctx->d_state, 0, sizeof(BuildState), ctx->stream);
+	if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+		svo_ctx_destroy(ctx);
+		return SVO_ERR_HIP;
+	}
+	*out = ctx;
+	return SVO_OK;
+}
+
+int svo_ctx_destroy(svo_ctx *ctx) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	if (ctx->stream)
+		(void)hipStreamSynchronize(ctx->stream);
+	ctx->prof.reset();
+	for (StageTimer *t : {&ctx->t_normalize, &ctx->t_count, &ctx->t_fill, &ctx->t_build, &ctx->t_beam, &ctx->t_march,
+	                      &ctx->t_rays, &ctx->t_bounce})
+		t->destroy();
+	if (ctx->d_scan_parts)
+		(void)hipFree(ctx->d_scan_parts);
+	if (ctx->d_state)
+		(void)hipFree(ctx->d_state);
+	if (ctx->d_err)
+		(void)hipFree(ctx->d_err);
+	if (ctx->h_state)
+		(void)hipHostFree(ctx->h_state);
+	delete ctx->pool;
+	if (ctx->stream)
+		(void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+	return SVO_OK;
+}
+
+const char *svo_last_error(const svo_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int svo_ctx_synchronize(svo_ctx *ctx) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	return hip_check(ctx, hipStreamSynchronize(ctx->stream), "synchronize");
+}
+
+void *svo_ctx_stream(svo_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int svo_ctx_take_device_errors(svo_ctx *ctx, uint32_t *bits) {
+	if (!ctx || !bits)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	hipError_t e;
+	if ((e = hipMemcpyAsync(&ctx->h_state->pad, ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream)) !=
+	        hipSuccess ||
+	    (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+		return hip_check(ctx, e, "read error word");
+	*bits = ctx->h_state->pad;
+	if ((e = hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream)) != hipSuccess)
+		return hip_check(ctx, e, "clear error word");
+	return hip_check(ctx, hipStreamSynchronize(ctx->stream), "clear error word");
+}
+
+int svo_ctx_trim(svo_ctx *ctx) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	(void)hipStreamSynchronize(ctx->stream);
+	ctx->pool->trim();
+	return SVO_OK;
+}
+
+int svo_ctx_memory(svo_ctx *ctx, uint64_t *in_use, uint64_t *cached, uint64_t *peak) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (in_use)
+		*in_use = ctx->pool->in_use();
+	if (cached)
+		*cached = ctx->pool->cached();
+	if (peak)
+		*peak = ctx->pool->peak();
+	return SVO_OK;
+}
+
+int svo_malloc(svo_ctx *ctx, uint64_t bytes, void **dptr) {
+	if (!ctx || !dptr)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	if (ctx->pool->alloc(dptr, bytes) != hipSuccess)
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "svo_malloc failed");
+	return SVO_OK;
+}
+
+int svo_free(svo_ctx *ctx, void *dptr) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	ctx->pool->free(dptr);
+	return SVO_OK;
+}
+
+int svo_upload(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+	if (!ctx || (bytes && (!dst || !src)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	return hip_check(ctx, e, "upload");
+}
+
+int svo_download(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+	if (!ctx || (bytes && (!dst || !src)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	return hip_check(ctx, e, "download");
+}
+
+int svo_prof_enable(svo_ctx *ctx, int on) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	ctx->prof.enabled = on != 0;
+	return SVO_OK;
+}
+int svo_prof_reset(svo_ctx *ctx) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	ctx->prof.reset();
+	return SVO_OK;
+}
+int svo_prof_count(svo_ctx *ctx, int *n) {
+	if (!ctx || !n)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	ctx->prof.resolve();
+	*n = (int)ctx->prof.entries.size();
+	return SVO_OK;
+}
+int svo_prof_get(svo_ctx *ctx, int i, const char **name, uint64_t *launches, double *total_ms) {
+	if (!ctx || i < 0 || i >= (int)ctx->prof.entries.size())
+		return SVO_ERR_INVALID_ARGUMENT;
+	const ProfEntry &p = ctx->prof.entries[i];
+	if (name)
+		*name = p.name.c_str();
+	if (launches)
+		*launches = p.launches;
+	if (total_ms)
+		*total_ms = p.total_ms;
+	return SVO_OK;
+}
+
+int svo_last_timing(svo_ctx *ctx, svo_timing *out) {
+	if (!ctx || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	out->normalize_ms = ctx->t_normalize.read();
+	out->vox_count_ms = ctx->t_count.read();
+	out->vox_fill_ms = ctx->t_fill.read();
+	out->build_ms = ctx->t_build.read();
+	out->beam_ms = ctx->t_beam.read();
+	out->march_ms = ctx->t_march.read();
+	out->rays_ms = ctx->t_rays.read();
+	out->bounce_ms = ctx->t_bounce.read();
+	return SVO_OK;
+}
+
+// ---- Scene
+static int scene_make(svo_ctx *ctx, uint64_t T, svo_scene **out, svo_scene *&s) {
+	s = new (std::nothrow) svo_scene();
+	if (!s)
+		return SVO_ERR_OUT_OF_MEMORY;
+	s->ctx = ctx;
+	s->T = T;
+	if (ctx->pool->alloc((void **)&s->xyz, sizeof(float) * 9 * (T ? T : 1)) != hipSuccess ||
+	    ctx->pool->alloc((void **)&s->rgb, sizeof(uint32_t) * (T ? T : 1)) != hipSuccess) {
+		svo_scene_destroy(s);
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "scene allocation failed");
+	}
+	(void)out;
+	return SVO_OK;
+}
+
+int svo_scene_create(svo_ctx *ctx, const float *tri_xyz, const uint32_t *tri_rgb, uint64_t T, int normalize,
+                     svo_scene **out) {
+	if (!ctx || !out || (T && (!tri_xyz || !tri_rgb)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	*out = nullptr;
+	svo_scene *s;
+	int rc = scene_make(ctx, T, out, s);
+	if (rc)
+		return rc;
+	hipError_t e = hipSuccess;
+	if (T) {
+		if ((e = hipMemcpyAsync(s->xyz, tri_xyz, sizeof(float) * 9 * T, hipMemcpyHostToDevice, ctx->stream)) ==
+		    hipSuccess)
+			e = hipMemcpyAsync(s->rgb, tri_rgb, sizeof(uint32_t) * T, hipMemcpyHostToDevice, ctx->stream);
+	}
+	if (e == hipSuccess && normalize && T) {
+		rc = normalize_scene(ctx, s);
+		if (rc) {
+			svo_scene_destroy(s);
+			return rc;
+		}
+	}
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	if (e != hipSuccess) {
+		svo_scene_destroy(s);
+		return hip_check(ctx, e, "scene upload");
+	}
+	*out = s;
+	return SVO_OK;
+}
+
+int svo_scene_create_device(svo_ctx *ctx, const float *tri_xyz_dev, const uint32_t *tri_rgb_dev, uint64_t T,
+                            int normalize, svo_scene **out) {
+	if (!ctx || !out || (T && (!tri_xyz_dev || !tri_rgb_dev)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	*out = nullptr;
+	svo_scene *s;
+	int rc = scene_make(ctx, T, out, s);
+	if (rc)
+		return rc;
+	hipError_t e = hipSuccess;
+	if (T) {
+		if ((e = hipMemcpyAsync(s->xyz, tri_xyz_dev, sizeof(float) * 9 * T, hipMemcpyDeviceToDevice, ctx->stream)) ==
+		    hipSuccess)
+			e = hipMemcpyAsync(s->rgb, tri_rgb_dev, sizeof(uint32_t) * T, hipMemcpyDeviceToDevice, ctx->stream);
+	}
+	if (e != hipSuccess) {
+		svo_scene_destroy(s);
+		return hip_check(ctx, e, "scene copy");
+	}
+	if (normalize && T) {
+		rc = normalize_scene(ctx, s);
+		if (rc) {
+			svo_scene_destroy(s);
+			return rc;
+		}
+	}
+	*out = s;
+	return SVO_OK;
+}
+
+int svo_scene_info(const svo_scene *s, uint64_t *T, const float **xyz, const uint32_t **rgb) {
+	if (!s)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (T)
+		*T = s->T;
+	if (xyz)
+		*xyz = s->xyz;
+	if (rgb)
+		*rgb = s->rgb;
+	return SVO_OK;
+}
+
+int svo_scene_download(svo_ctx *ctx, const svo_scene *s, float *tri_xyz) {
+	if (!ctx || !s || (s->T && !tri_xyz))
+		return SVO_ERR_INVALID_ARGUMENT;
+	return svo_download(ctx, tri_xyz, s->xyz, sizeof(float) * 9 * s->T);
+}
+
+int svo_scene_destroy(svo_scene *s) {
+	if (!s)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(s->ctx);
+	s->ctx->pool->free(s->xyz);
+	s->ctx->pool->free(s->rgb);
+	delete s;
+	return SVO_OK;
+}
+
+// ---- Voxelizer
+int svo_voxelize_count(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out) {
+	if (!ctx || !scene || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	if (level < 1 || level > 12)
+		return set_error(ctx, SVO_ERR_BAD_LEVEL, "level must be in [1, 12]");
+	DeviceGuard g(ctx);
+	svo_fraglist *fl = new (std::nothrow) svo_fraglist();
+	if (!fl)
+		return SVO_ERR_OUT_OF_MEMORY;
+	fl->ctx = ctx;
+	fl->scene = scene;
+	fl->level = level;
+	int rc = voxelize_count(ctx, fl);
+	if (rc) {
+		svo_fraglist_destroy(fl);
+		return rc;
+	}
+	*out = fl;
+	return SVO_OK;
+}
+
+int svo_voxelize_fill(svo_ctx *ctx, svo_fraglist *fl) {
+	if (!ctx || !fl || !fl->scene || !fl->offsets)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	return voxelize_fill(ctx, fl);
+}
+
+int svo_voxelize(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out) {
+	int rc = svo_voxelize_count(ctx, scene, level, out);
+	if (rc)
+		return rc;
+	rc = svo_voxelize_fill(ctx, *out);
+	if (rc) {
+		svo_fraglist_destroy(*out);
+		*out = nullptr;
+	}
+	return rc;
+}
+
+int svo_fraglist_create(svo_ctx *ctx, const uint32_t *frags, uint64_t F, uint32_t level, svo_fraglist **out) {
+	if (!ctx || !out || (F && !frags))
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	if (level < 1 || level > 12)
+		return set_error(ctx, SVO_ERR_BAD_LEVEL, "level must be in [1, 12]");
+	if (F > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 fragments");
+	DeviceGuard g(ctx);
+	svo_fraglist *fl = new (std::nothrow) svo_fraglist();
+	if (!fl)
+		return SVO_ERR_OUT_OF_MEMORY;
+	fl->ctx = ctx;
+	fl->level = level;
+	fl->F = F;
+	if (ctx->pool->alloc((void **)&fl->frags, sizeof(uint32_t) * 2 * (F ? F : 1)) != hipSuccess) {
+		svo_fraglist_destroy(fl);
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "fragment list allocation failed");
+	}
+	int rc = svo_upload(ctx, fl->frags, frags, sizeof(uint32_t) * 2 * F);
+	if (rc) {
+		svo_fraglist_destroy(fl);
+		return rc;
+	}
+	fl->filled = true;
+	*out = fl;
+	return SVO_OK;
+}
+
+int svo_fraglist_info(const svo_fraglist *fl, uint32_t *level, uint64_t *F, const uint32_t **dev) {
+	if (!fl)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (level)
+		*level = fl->level;
+	if (F)
+		*F = fl->F;
+	if (dev)
+		*dev = fl->frags;
+	return SVO_OK;
+}
+
+int svo_fraglist_download(svo_ctx *ctx, const svo_fraglist *fl, uint32_t *out, uint64_t cap) {
+	if (!ctx || !fl || (fl->F && !out))
+		return SVO_ERR_INVALID_ARGUMENT;
+	return svo_download(ctx, out, fl->frags, sizeof(uint32_t) * 2 * std::min(cap, fl->F));
+}
+
+int svo_fraglist_destroy(svo_fraglist *fl) {
+	if (!fl)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(fl->ctx);
+	fl->ctx->pool->free(fl->frags);
+	fl->ctx->pool->free(fl->offsets);
+	delete fl;
+	return SVO_OK;
+}
+
+// ---- OctreeBuilder
+int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *fl, svo_octree **out) {
+	if (!ctx || !fl || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	if (!fl->filled)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "fragment list not filled (call svo_voxelize_fill)");
+	DeviceGuard g(ctx);
+	svo_octree *o = new (std::nothrow) svo_octree();
+	if (!o)
+		return SVO_ERR_OUT_OF_MEMORY;
+	o->ctx = ctx;
+	o->level = fl->level;
+	o->src = fl;
+	o->cap = ctx->build_cap_hint ? std::max<uint64_t>(8, std::min<uint64_t>(ctx->build_cap_hint, 1ull << 30))
+	                             : initial_capacity(fl->F, fl->level);
+	const uint64_t F = fl->F ? fl->F : 1;
+	const uint64_t big_words = 2 + F / 33u;
+	if (ctx->pool->alloc((void **)&o->words, sizeof(uint32_t) * o->cap) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->aux, sizeof(uint32_t) * o->cap) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->node, sizeof(uint32_t) * F) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->bucket, sizeof(uint32_t) * 2 * F) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->big, sizeof(uint32_t) * big_words) != hipSuccess) {
+		svo_octree_destroy(o);
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
+	}
+	o->pending = true;
+	ctx->t_build.start(ctx->stream);
+	int rc = build_enqueue(ctx, o, 1);
+	ctx->t_build.stop(ctx->stream);
+	if (rc) {
+		svo_octree_destroy(o);
+		return rc;
+	}
+	*out = o;
+	return SVO_OK;
+}
+
+int svo_build_finish(svo_ctx *ctx, svo_octree *o, uint64_t *W) {
+	if (!ctx || !o)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	while (o->pending) {
+		hipError_t e;
+		if ((e = hipMemcpyAsync(ctx->h_state, ctx->d_state, sizeof(BuildState), hipMemcpyDeviceToHost, ctx->stream)) !=
+		        hipSuccess ||
+		    (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+			return hip_check(ctx, e, "octree build");
+		const BuildState st = *ctx->h_state;
+		if (st.overflow == 0) {
+			const uint32_t slot = o->level & 1u;
+			o->W = ((uint64_t)st.counter[slot] + 1u) * 8u; // src/OctreeBuilder.cpp:212-214
+			o->pending = false;
+			release_build_scratch(o);
+			break;
+		}
+		// The allocation of level st.overflow did not fit: grow both cap-sized buffers, keep the
+		// words written so far (everything below the next level's range), redo that allocation.
+		const uint64_t keep = st.need_words - 0; // upper bound of the written range
+		uint64_t ncap = std::max<uint64_t>(st.need_words + st.need_words / 2, o->cap * 2);
+		if (st.need_words > (1ull << 30))
+			return set_error(ctx, SVO_ERR_OVERFLOW, "octree needs more than 2^30 words (30-bit child pointers)");
+		ncap = std::min<uint64_t>(ncap, 1ull << 30);
+		uint32_t *nw = nullptr, *na = nullptr;
+		if (ctx->pool->alloc((void **)&nw, sizeof(uint32_t) * ncap) != hipSuccess ||
+		    ctx->pool->alloc((void **)&na, sizeof(uint32_t) * ncap) != hipSuccess) {
+			ctx->pool->free(nw);
+			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree buffer growth failed");
+		}
+		if ((e = hipMemcpyAsync(nw, o->words, sizeof(uint32_t) * std::min(keep, o->cap), hipMemcpyDeviceToDevice,
+		                        ctx->stream)) != hipSuccess)
+			return hip_check(ctx, e, "octree buffer growth");
+		ctx->pool->free(o->words);
+		ctx->pool->free(o->aux);
+		o->words = nw;
+		o->aux = na;
+		o->cap = ncap;
+		ctx->t_build.start(ctx->stream);
+		int rc = build_enqueue(ctx, o, st.overflow);
+		ctx->t_build.stop(ctx->stream);
+		if (rc)
+			return rc;
+	}
+	if (W)
+		*W = o->W;
+	return SVO_OK;
+}
+
+int svo_build_set_initial_capacity(svo_ctx *ctx, uint64_t words) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	ctx->build_cap_hint = words;
+	return SVO_OK;
+}
+
+int svo_build(svo_ctx *ctx, const svo_fraglist *fl, svo_octree **out) {
+	int rc = svo_build_enqueue(ctx, fl, out);
+	if (rc)
+		return rc;
+	rc = svo_build_finish(ctx, *out, nullptr);
+	if (rc) {
+		svo_octree_destroy(*out);
+		*out = nullptr;
+	}
+	return rc;
+}
+
+int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t level, svo_octree **out) {
+	if (!ctx || !out || !words || W < 8)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	if (level < 1 || level > 12)
+		return set_error(ctx, SVO_ERR_BAD_LEVEL, "level must be in [1, 12]");
+	if (W > (1ull << 30))
+		return set_error(ctx, SVO_ERR_OVERFLOW, "octree above 2^30 words");
+	DeviceGuard g(ctx);
+	svo_octree *o = new (std::nothrow) svo_octree();
+	if (!o)
+		return SVO_ERR_OUT_OF_MEMORY;
+	o->ctx = ctx;
+	o->level = level;
+	o->W = o->cap = W;
+	if (ctx->pool->alloc((void **)&o->words, sizeof(uint32_t) * W) != hipSuccess) {
+		svo_octree_destroy(o);
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree allocation failed");
+	}
+	int rc = svo_upload(ctx, o->words, words, sizeof(uint32_t) * W);
+	if (rc) {
+		svo_octree_destroy(o);
+		return rc;
+	}
+	*out = o;
+	return SVO_OK;
+}
+
+int svo_octree_info(const svo_octree *o, uint32_t *level, uint64_t *W, const uint32_t **dev) {
+	if (!o)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (level)
+		*level = o->level;
+	if (W)
+		*W = o->W;
+	if (dev)
+		*dev = o->words;
+	return o->pending ? SVO_ERR_INVALID_ARGUMENT : SVO_OK;
+}
+
+int svo_octree_download(svo_ctx *ctx, const svo_octree *o, uint32_t *out, uint64_t cap) {
+	if (!ctx || !o || !out || o->pending)
+		return SVO_ERR_INVALID_ARGUMENT;
+	return svo_download(ctx, out, o->words, sizeof(uint32_t) * std::min(cap, o->W));
+}
+
+int svo_octree_destroy(svo_octree *o) {
+	if (!o)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(o->ctx);
+	if (o->pending)
+		(void)hipStreamSynchronize(o->ctx->stream);
+	release_build_scratch(o);
+	o->ctx->pool->free(o->words);
+	delete o;
+	return SVO_OK;
+}
+
+// ---- OctreeTracer
+int svo_march(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t width, uint32_t height, int beam,
+              svo_hit *hits_dev, float *beam_dev) {
+	if (!ctx || !o || !cam || !hits_dev || o->pending)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (width == 0 || height == 0)
+		return SVO_OK;
+	if ((uint64_t)width * height > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame too large");
+	DeviceGuard g(ctx);
+	return march_frame(ctx, o, cam, width, height, beam, hits_dev, beam_dev);
+}
+
+int svo_march_rays(svo_ctx *ctx, const svo_octree *o, const float *rays_dev, uint64_t R, int variant,
+                   svo_hit *hits_dev) {
+	if (!ctx || !o || o->pending || (R && (!rays_dev || !hits_dev)) || variant < 0 || variant > 2)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (R > (uint64_t)0xFFFFFFFFull * 256u)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
+	DeviceGuard g(ctx);
+	return march_rays(ctx, o, rays_dev, R, variant, hits_dev);
+}
+
+int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t seed, float *rays_dev) {
+	if (!ctx || (R && (!hits_dev || !rays_dev)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (R > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
+	DeviceGuard g(ctx);
+	return bounce_rays(ctx, hits_dev, R, seed, rays_dev);
+}
+
+// Camera::fetch_uniform_data (src/Camera.cpp:77-95): yaw about +y, then pitch about -x;
+// look = R * (0,0,1), side = normalize(R * (1,0,0)) * tan(fov/2) * aspect,
+// up = normalize(cross(look, side)) * tan(fov/2).
+int svo_camera_from_yaw_pitch(const float position[3], float yaw, float pitch, float fov, float aspect,
+                              svo_camera *out) {
+	if (!position || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	const float cy = std::cos(yaw), sy = std::sin(yaw), cp = std::cos(pitch), sp = std::sin(pitch);
+	// rotate(yaw, +y) * rotate(pitch, -x) applied to the basis vectors
+	float look[3] = {sy * cp, sp, cy * cp};
+	float side[3] = {cy, 0.0f, -sy};
+	auto nrm = [](float *v) {
+		float s = 1.0f / std::sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+		v[0] *= s, v[1] *= s, v[2] *= s;
+	};
+	nrm(look);
+	nrm(side);
+	const float tg = std::tan(fov * 0.5f);
+	for (int c = 0; c < 3; ++c)
+		side[c] = side[c] * tg * aspect;
+	float up[3] = {look[1] * side[2] - side[1] * look[2], look[2] * side[0] - side[2] * look[0],
+	               look[0] * side[1] - side[0] * look[1]};
+	nrm(up);
+	for (int c = 0; c < 3; ++c)
+		up[c] *= tg;
+	for (int c = 0; c < 3; ++c) {
+		out->pos[c] = position[c];
+		out->look[c] = look[c];
+		out->side[c] = side[c];
+		out->up[c] = up[c];
+	}
+	out->pos[3] = out->look[3] = out->side[3] = out->up[3] = 1.0f;
+	return SVO_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,182 @@
+// Host runtime of the SVO library: context (one HIP stream on one device), caching device
+// allocator, per-kernel profiler and stage timers.  Replaces the Vulkan device / queue / command
+// pool plumbing of the reference (src/Application.cpp:161-293, dep/MyVK).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/svo.h"
+
+namespace svo {
+
+// Device-side build state of one octree build (the reference's build-info + counter buffers,
+// src/OctreeBuilder.cpp:24-39): the current level range, the allocated-block counter, and the
+// overflow report.
+// Level i reads slot (i & 1) and its allocation writes slot ((i + 1) & 1), so no kernel has to
+// advance the range in place (the reference's octree_modify_arg.comp does it with one thread).
+struct BuildState {
+	uint32_t begin[2];   // uAllocBegin: first word of the level's range
+	uint32_t num[2];     // uAllocNum: words in the level's range
+	uint32_t counter[2]; // uCounter: 8-word blocks allocated below the root block
+	uint32_t overflow;   // level whose allocation did not fit the node buffer, 0 = none
+	uint32_t pad;
+	uint64_t need_words; // words that allocation needed
+};
+
+// Error bits raised by kernels into the context's error word.
+enum DeviceError : uint32_t {
+	kErrIterCap = 1u,
+	kErrOutOfRange = 2u,
+	kErrTooDeep = 4u,
+};
+
+class Pool {
+public:
+	explicit Pool(int device) : m_device(device) {}
+	~Pool();
+	hipError_t alloc(void **p, uint64_t bytes);
+	void free(void *p);
+	void trim();
+	uint64_t in_use() const { return m_in_use; }
+	uint64_t cached() const { return m_cached; }
+	uint64_t peak() const { return m_peak; }
+
+private:
+	static uint64_t round_size(uint64_t bytes);
+	int m_device;
+	std::multimap<uint64_t, void *> m_free;
+	std::unordered_map<void *, uint64_t> m_live;
+	uint64_t m_in_use = 0, m_cached = 0, m_peak = 0;
+};
+
+struct ProfEntry {
+	std::string name;
+	uint64_t launches = 0;
+	double total_ms = 0.0;
+};
+
+class Profiler {
+public:
+	bool enabled = false;
+	void begin(hipStream_t s, const char *name);
+	void end(hipStream_t s);
+	void resolve(); // waits for pending events
+	void reset();
+	std::vector<ProfEntry> entries;
+
+private:
+	struct Pending {
+		int entry;
+		hipEvent_t a, b;
+	};
+	std::vector<Pending> m_pending;
+	std::vector<hipEvent_t> m_free_events;
+	int m_open = -1;
+	hipEvent_t m_open_ev = nullptr;
+	hipEvent_t get_event();
+};
+
+// Stage timer: a pair of events around a stage, read lazily.
+struct StageTimer {
+	hipEvent_t a = nullptr, b = nullptr;
+	bool armed = false;
+	void start(hipStream_t s);
+	void stop(hipStream_t s);
+	float read(); // ms, 0 when never armed
+	void destroy();
+};
+
+} // namespace svo
+
+struct svo_ctx {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	std::string last_error;
+	svo::Pool *pool = nullptr;
+	svo::Profiler prof;
+	// small device scratch: scan partials, build state, error word
+	uint64_t *d_scan_parts = nullptr; // 2 * kScanBlocks uint64
+	svo::BuildState *d_state = nullptr;
+	uint32_t *d_err = nullptr;
+	svo::BuildState *h_state = nullptr; // pinned mirror for read-back
+	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_beam, t_march, t_rays, t_bounce;
+	int cu_count = 256;
+	uint64_t build_cap_hint = 0; // words; 0 = automatic
+};
+
+struct svo_scene {
+	svo_ctx *ctx;
+	uint64_t T;
+	float *xyz;
+	uint32_t *rgb;
+};
+
+struct svo_fraglist {
+	svo_ctx *ctx;
+	const svo_scene *scene; // null when created from host fragments
+	uint32_t level;
+	uint64_t F;
+	uint32_t *frags;   // 2 * F words
+	uint32_t *offsets; // per-triangle exclusive offsets from the count pass (T + 1), or null
+	bool filled;
+};
+
+struct svo_octree {
+	svo_ctx *ctx;
+	uint32_t level;
+	uint64_t W;        // used words, valid after finish
+	uint64_t cap;      // words allocated
+	uint32_t *words;
+	// build-in-flight resources
+	const svo_fraglist *src;
+	uint32_t *aux;     // cap words: leaf offsets
+	uint32_t *node;    // F words: per-fragment node index
+	uint32_t *bucket;  // 2 * F words: fragments grouped by leaf
+	uint32_t *big;     // big-bucket work list (1 + F words, capped)
+	bool pending;
+};
+
+namespace svo {
+
+// Shared helpers for the entry points.
+int set_error(svo_ctx *ctx, int code, const std::string &msg);
+int hip_check(svo_ctx *ctx, hipError_t e, const char *what);
+
+constexpr int kScanBlocks = 1024;  // partial sums per device-wide scan
+constexpr int kScanThreads = 256;
+
+// Launch helper: brackets the launch with profiler events when profiling is on.
+template <class Kernel, class... Args>
+inline hipError_t launch(svo_ctx *ctx, const char *name, Kernel k, dim3 grid, dim3 block, size_t lds,
+                         Args... args) {
+	if (ctx->prof.enabled)
+		ctx->prof.begin(ctx->stream, name);
+	hipLaunchKernelGGL(k, grid, block, lds, ctx->stream, args...);
+	hipError_t e = hipGetLastError();
+	if (ctx->prof.enabled)
+		ctx->prof.end(ctx->stream);
+	return e;
+}
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Entry points implemented per stage.
+int normalize_scene(svo_ctx *ctx, svo_scene *scene);
+int voxelize_count(svo_ctx *ctx, svo_fraglist *fl);
+int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl);
+int build_enqueue(svo_ctx *ctx, svo_octree *oct, uint32_t resume_level);
+int build_finish(svo_ctx *ctx, svo_octree *oct);
+int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
+                svo_hit *hits, float *beam_out);
+int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits);
+int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays);
+
+// Device-wide exclusive scan of n (read from the device) uint32 counts; see svo_scan.hpp.
+} // namespace svo

@@ -1,0 +1,377 @@
+// Mesh voxelization into the reference's fragment list (Voxelizer, src/Voxelizer.cpp:5-179), as
+// compute kernels instead of the geometry-shader + conservative-raster pipeline.
+//
+//   per triangle (shader/voxelizer.geom:21-54): dominant axis of the face normal, swizzled
+//     projection, pixel AABB and voxel depth range, exactly as the geometry shader computes them;
+//   coverage (conservative rasterization, src/Voxelizer.cpp:112-127): pixel (i,j) is covered when
+//     the closed triangle, with vertices snapped to 1/256 pixel, meets the half-open pixel square
+//     [i,i+1) x [j,j+1).  Written as three linear constraints lo_k <= Sx_k*i + Sy_k*j <= hi_k
+//     (one per edge normal, exact int64), so every row's covered pixels form one span found with
+//     two divisions per edge instead of testing pixels one by one;
+//   per pixel (shader/voxelizer.frag:17-44): depth of the triangle's plane at the pixel centre,
+//     clamps to [0,res-1] and to the depth range, swizzle back, fragment packing.
+// Two passes like the reference's count-only + fill draws: vox_count writes the per-triangle count,
+// a device scan turns counts into offsets, vox_fill writes each triangle's fragments at its offset
+// (triangle order, row-major inside a triangle).
+//
+// Scene normalisation (src/Scene.cpp:62-99) runs here too: bounding-box reduction + transform.
+#include "svo_device.hpp"
+#include "svo_runtime.hpp"
+#include "svo_scan.hpp"
+
+namespace svo {
+namespace {
+
+constexpr int kThreads = 256;
+
+struct Edge {
+	int64_t sx, sy; // 256 * edge normal
+	int64_t lo, hi; // covered iff lo <= sx*i + sy*j <= hi
+};
+
+struct Setup {
+	bool empty;
+	uint32_t axis;
+	uint32_t dr0, dr1;
+	int32_t ilo, ihi, jlo, jhi; // pixel rectangle: AABB ∩ snapped bbox ∩ [0, res)
+	float px0, py0, pz0;
+	float nx, ny, nz; // plane of the projected triangle (pixel units, depth in [0,1])
+	Edge e[3];
+};
+
+__device__ __forceinline__ f3 project(f3 v, uint32_t axis) { // voxelizer.geom:15-19
+	f3 r = axis == 0 ? f3{v.y, v.z, v.x} : (axis == 1 ? f3{v.z, v.x, v.y} : v);
+	r.z = (r.z + 1.0f) * 0.5f;
+	return r;
+}
+
+__device__ __forceinline__ int64_t imin64(int64_t a, int64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ int64_t imax64(int64_t a, int64_t b) { return a < b ? b : a; }
+
+__device__ void tri_setup(const float *__restrict__ tri, uint64_t t, uint32_t res, Setup &s) {
+	const float *p = tri + 9 * t;
+	f3 p0{p[0], p[1], p[2]}, p1{p[3], p[4], p[5]}, p2{p[6], p[7], p[8]};
+	f3 e1{p1.x - p0.x, p1.y - p0.y, p1.z - p0.z}, e2{p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+	f3 n = cross3(e1, e2); // voxelizer.geom:29
+	float wx = fabsf(n.x), wy = fabsf(n.y), wz = fabsf(n.z);
+	s.axis = (wx > wy && wx > wz) ? 0u : (wy > wz ? 1u : 2u);
+	bool degenerate = (n.x == 0.0f && n.y == 0.0f && n.z == 0.0f);
+	f3 q0 = project(p0, s.axis), q1 = project(p1, s.axis), q2 = project(p2, s.axis);
+	const float fres = (float)res;
+	// voxelizer.geom:40-43
+	float mnx = gmin(q0.x, gmin(q1.x, q2.x)), mny = gmin(q0.y, gmin(q1.y, q2.y));
+	float mxx = gmax(q0.x, gmax(q1.x, q2.x)), mxy = gmax(q0.y, gmax(q1.y, q2.y));
+	uint32_t a0 = f2u_sat((mnx + 1.0f) * 0.5f * fres), a1 = f2u_sat((mny + 1.0f) * 0.5f * fres);
+	uint32_t a2 = f2u_sat((mxx + 1.0f) * 0.5f * fres), a3 = f2u_sat((mxy + 1.0f) * 0.5f * fres);
+	s.dr0 = f2u_sat(gmin(q0.z, gmin(q1.z, q2.z)) * fres);
+	s.dr1 = f2u_sat(gmax(q0.z, gmax(q1.z, q2.z)) * fres);
+
+	float px[3] = {(q0.x + 1.0f) * 0.5f * fres, (q1.x + 1.0f) * 0.5f * fres, (q2.x + 1.0f) * 0.5f * fres};
+	float py[3] = {(q0.y + 1.0f) * 0.5f * fres, (q1.y + 1.0f) * 0.5f * fres, (q2.y + 1.0f) * 0.5f * fres};
+	float pz[3] = {q0.z, q1.z, q2.z};
+	int64_t X[3], Y[3];
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		X[k] = (int64_t)__builtin_rintf(px[k] * 256.0f);
+		Y[k] = (int64_t)__builtin_rintf(py[k] * 256.0f);
+	}
+	s.px0 = px[0];
+	s.py0 = py[0];
+	s.pz0 = pz[0];
+	{
+		float ax = px[1] - px[0], ay = py[1] - py[0], az = pz[1] - pz[0];
+		float bx = px[2] - px[0], by = py[2] - py[0], bz = pz[2] - pz[0];
+		s.nx = ay * bz - az * by;
+		s.ny = az * bx - ax * bz;
+		s.nz = ax * by - ay * bx;
+	}
+	int64_t xmn = imin64(X[0], imin64(X[1], X[2])), xmx = imax64(X[0], imax64(X[1], X[2]));
+	int64_t ymn = imin64(Y[0], imin64(Y[1], Y[2])), ymx = imax64(Y[0], imax64(Y[1], Y[2]));
+	int64_t ilo = imax64((int64_t)a0, xmn >> 8), ihi = imin64(imin64((int64_t)a2, (int64_t)res - 1), xmx >> 8);
+	int64_t jlo = imax64((int64_t)a1, ymn >> 8), jhi = imin64(imin64((int64_t)a3, (int64_t)res - 1), ymx >> 8);
+	s.ilo = (int32_t)imax64(ilo, 0);
+	s.ihi = (int32_t)imax64(ihi, -1);
+	s.jlo = (int32_t)imax64(jlo, 0);
+	s.jhi = (int32_t)imax64(jhi, -1);
+	s.empty = degenerate || s.ilo > s.ihi || s.jlo > s.jhi;
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		const int a = k, b = (k + 1) % 3, c = (k + 2) % 3;
+		int64_t A = Y[b] - Y[a], B = X[a] - X[b];
+		int64_t ce = A * X[a] + B * Y[a], cc = A * X[c] + B * Y[c];
+		int64_t tlo = imin64(ce, cc), thi = imax64(ce, cc);
+		bool lo_att = A >= 0 && B >= 0, hi_att = A <= 0 && B <= 0;
+		int64_t thi2 = lo_att ? thi : thi - 1, tlo2 = hi_att ? tlo : tlo + 1;
+		int64_t sx = A * 256, sy = B * 256;
+		int64_t off_lo = (A < 0 ? sx : 0) + (B < 0 ? sy : 0);
+		int64_t width = (sx < 0 ? -sx : sx) + (sy < 0 ? -sy : sy);
+		s.e[k] = Edge{sx, sy, tlo2 - off_lo - width, thi2 - off_lo};
+	}
+}
+
+// Covered pixels of row j: [i0, i1] (empty when i0 > i1).  Exact: the double quotient of two
+// integers below 2^53 never crosses an integer for |quotient| < 2^25 (|sx| <= 2^29), and bounds
+// further out are clamped away by the pixel rectangle.
+__device__ __forceinline__ void row_span(const Setup &s, int32_t j, int32_t &i0, int32_t &i1) {
+	double lo = (double)s.ilo, hi = (double)s.ihi;
+#pragma unroll
+	for (int k = 0; k < 3; ++k) {
+		const Edge &e = s.e[k];
+		int64_t syj = e.sy * (int64_t)j;
+		int64_t nlo = e.lo - syj, nhi = e.hi - syj;
+		if (e.sx > 0) {
+			lo = fmax(lo, ceil((double)nlo / (double)e.sx));
+			hi = fmin(hi, floor((double)nhi / (double)e.sx));
+		} else if (e.sx < 0) {
+			lo = fmax(lo, ceil((double)nhi / (double)e.sx));
+			hi = fmin(hi, floor((double)nlo / (double)e.sx));
+		} else if (nlo > 0 || nhi < 0) {
+			hi = -1.0;
+		}
+	}
+	if (lo > hi) {
+		i0 = 1;
+		i1 = 0;
+	} else {
+		i0 = (int32_t)lo;
+		i1 = (int32_t)hi;
+	}
+}
+
+__global__ void __launch_bounds__(kThreads)
+    vox_count(const float *__restrict__ tri, uint64_t T, uint32_t res, uint32_t *__restrict__ counts) {
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x; t < T; t += stride) {
+		Setup s;
+		tri_setup(tri, t, res, s);
+		uint32_t c = 0;
+		if (!s.empty)
+			for (int32_t j = s.jlo; j <= s.jhi; ++j) {
+				int32_t i0, i1;
+				row_span(s, j, i0, i1);
+				if (i0 <= i1)
+					c += (uint32_t)(i1 - i0 + 1);
+			}
+		counts[t] = c;
+	}
+}
+
+__global__ void __launch_bounds__(kThreads)
+    vox_fill(const float *__restrict__ tri, const uint32_t *__restrict__ rgb, uint64_t T, uint32_t res,
+             const uint32_t *__restrict__ offsets, uint2 *__restrict__ frags) {
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x; t < T; t += stride) {
+		Setup s;
+		tri_setup(tri, t, res, s);
+		if (s.empty)
+			continue;
+		const uint32_t colour = rgb[t] & 0x00ffffffu;
+		uint64_t k = offsets[t];
+		const float fres = (float)res;
+		for (int32_t j = s.jlo; j <= s.jhi; ++j) {
+			int32_t i0, i1;
+			row_span(s, j, i0, i1);
+			const float cy = (float)j + 0.5f;
+			const float dy = s.ny * (cy - s.py0);
+			for (int32_t i = i0; i <= i1; ++i) {
+				// voxelizer.frag:17-25: depth at the pixel centre, clamps, swizzle back
+				float zc = s.pz0;
+				if (s.nz != 0.0f) {
+					float cx = (float)i + 0.5f;
+					zc = s.pz0 - (s.nx * (cx - s.px0) + dy) / s.nz;
+				}
+				uint32_t uz = umin32(f2u_sat(zc * fres), res - 1u);
+				uz = umin32(umax32(uz, s.dr0), s.dr1);
+				uint32_t ui = (uint32_t)i, uj = (uint32_t)j, vx, vy, vz;
+				if (s.axis == 0u)
+					vx = uz, vy = ui, vz = uj;
+				else if (s.axis == 1u)
+					vx = uj, vy = uz, vz = ui;
+				else
+					vx = ui, vy = uj, vz = uz;
+				// voxelizer.frag:38-42
+				frags[k++] = make_uint2(vx | (vy << 12u) | ((vz & 0xffu) << 24u), ((vz >> 8u) << 28u) | colour);
+			}
+		}
+	}
+}
+
+struct CountSrc {
+	const uint32_t *counts;
+	uint64_t n;
+	__device__ uint64_t size() const { return n; }
+	__device__ uint32_t operator()(uint64_t i) const { return counts[i]; }
+};
+struct TotalCommit {
+	uint64_t *total;
+	__device__ void operator()(uint64_t t) const { *total = t; }
+};
+struct OffsetDst {
+	uint32_t *offsets;
+	__device__ bool active() const { return true; }
+	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { offsets[i] = (uint32_t)k; }
+};
+
+// ---- normalisation (src/Scene.cpp:62-99) -------------------------------------------------------
+// Per-block bounding box of the vertices -> parts[6 * b].
+__global__ void __launch_bounds__(kThreads) norm_bbox(const float *__restrict__ xyz, uint64_t nverts, float *parts) {
+	float mn[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+	float mx[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t v = (uint64_t)blockIdx.x * kThreads + threadIdx.x; v < nverts; v += stride) {
+#pragma unroll
+		for (int c = 0; c < 3; ++c) {
+			float p = xyz[3 * v + c];
+			mn[c] = mn[c] < p ? mn[c] : p;
+			mx[c] = p < mx[c] ? mx[c] : p;
+		}
+	}
+	__shared__ float s[6][kThreads];
+#pragma unroll
+	for (int c = 0; c < 3; ++c) {
+		s[c][threadIdx.x] = mn[c];
+		s[3 + c][threadIdx.x] = mx[c];
+	}
+	__syncthreads();
+	for (int w = kThreads / 2; w > 0; w >>= 1) {
+		if ((int)threadIdx.x < w) {
+#pragma unroll
+			for (int c = 0; c < 3; ++c) {
+				float a = s[c][threadIdx.x], b = s[c][threadIdx.x + w];
+				s[c][threadIdx.x] = a < b ? a : b;
+				float x = s[3 + c][threadIdx.x], y = s[3 + c][threadIdx.x + w];
+				s[3 + c][threadIdx.x] = y < x ? x : y;
+			}
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x < 6)
+		parts[6 * blockIdx.x + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// Combines the per-block boxes and computes {center.xyz, inv_extent} into out[4].
+__global__ void norm_params(const float *parts, uint32_t nparts, float *out) {
+	if (threadIdx.x != 0)
+		return;
+	float mn[3] = {3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f};
+	float mx[3] = {-3.402823466e+38f, -3.402823466e+38f, -3.402823466e+38f};
+	for (uint32_t b = 0; b < nparts; ++b)
+		for (int c = 0; c < 3; ++c) {
+			float a = parts[6 * b + c], x = parts[6 * b + 3 + c];
+			mn[c] = mn[c] < a ? mn[c] : a;
+			mx[c] = x < mx[c] ? mx[c] : x;
+		}
+	float ex = mx[0] - mn[0], ey = mx[1] - mn[1], ez = mx[2] - mn[2];
+	float extent = gmax(ex, gmax(ey, ez)) * 0.5f;
+	out[0] = (mx[0] + mn[0]) * 0.5f;
+	out[1] = (mx[1] + mn[1]) * 0.5f;
+	out[2] = (mx[2] + mn[2]) * 0.5f;
+	out[3] = 1.0f / extent;
+}
+
+__global__ void __launch_bounds__(kThreads) norm_apply(float *xyz, uint64_t nverts, const float *params) {
+	const float cx = params[0], cy = params[1], cz = params[2], inv = params[3];
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t v = (uint64_t)blockIdx.x * kThreads + threadIdx.x; v < nverts; v += stride) {
+		xyz[3 * v + 0] = (xyz[3 * v + 0] - cx) * inv;
+		xyz[3 * v + 1] = (xyz[3 * v + 1] - cy) * inv;
+		xyz[3 * v + 2] = (xyz[3 * v + 2] - cz) * inv;
+	}
+}
+
+uint32_t grid_for(uint64_t items, const svo_ctx *ctx) {
+	uint64_t g = (items + kThreads - 1) / kThreads;
+	uint64_t cap = (uint64_t)ctx->cu_count * 16u;
+	if (g > cap)
+		g = cap;
+	return (uint32_t)(g ? g : 1);
+}
+
+} // namespace
+
+int normalize_scene(svo_ctx *ctx, svo_scene *scene) {
+	const uint64_t nverts = scene->T * 3u;
+	if (nverts == 0)
+		return SVO_OK;
+	// parts: 6 floats per block + 4 result floats, carved out of the scan scratch (2 * kScanBlocks u64)
+	float *parts = reinterpret_cast<float *>(ctx->d_scan_parts);
+	const uint32_t blocks = 256; // 256 * 6 + 4 floats < 2 * kScanBlocks * 2 floats
+	float *params = parts + 6 * blocks;
+	hipError_t e;
+	ctx->t_normalize.start(ctx->stream);
+	if ((e = launch(ctx, "norm_bbox", norm_bbox, dim3(blocks), dim3(kThreads), 0, (const float *)scene->xyz, nverts,
+	                parts)) != hipSuccess)
+		return hip_check(ctx, e, "norm_bbox");
+	if ((e = launch(ctx, "norm_params", norm_params, dim3(1), dim3(64), 0, (const float *)parts, blocks, params)) !=
+	    hipSuccess)
+		return hip_check(ctx, e, "norm_params");
+	if ((e = launch(ctx, "norm_apply", norm_apply, dim3(grid_for(nverts, ctx)), dim3(kThreads), 0, scene->xyz, nverts,
+	                (const float *)params)) != hipSuccess)
+		return hip_check(ctx, e, "norm_apply");
+	ctx->t_normalize.stop(ctx->stream);
+	return SVO_OK;
+}
+
+// Count pass + scan + read-back of F + allocation of the list (Voxelizer::count_and_create_fragment_list).
+int voxelize_count(svo_ctx *ctx, svo_fraglist *fl) {
+	const svo_scene *sc = fl->scene;
+	const uint64_t T = sc->T;
+	const uint32_t res = 1u << fl->level;
+	hipError_t e;
+	if (T > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 triangles");
+	if ((e = ctx->pool->alloc((void **)&fl->offsets, sizeof(uint32_t) * (T ? T : 1))) != hipSuccess)
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "offsets allocation failed");
+	ctx->t_count.start(ctx->stream);
+	if (T > 0) {
+		if ((e = launch(ctx, "vox_count", vox_count, dim3(grid_for(T, ctx)), dim3(kThreads), 0,
+		                (const float *)sc->xyz, T, res, fl->offsets)) != hipSuccess)
+			return hip_check(ctx, e, "vox_count");
+		CountSrc src{fl->offsets, T};
+		uint64_t *total = ctx->d_scan_parts + 2 * kScanBlocks;
+		if ((e = launch(ctx, "vox_scan_reduce", scan_reduce<CountSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
+		                ctx->d_scan_parts)) != hipSuccess)
+			return hip_check(ctx, e, "vox_scan_reduce");
+		if ((e = launch(ctx, "vox_scan_parts", scan_parts<TotalCommit>, dim3(1), dim3(1024), 0, ctx->d_scan_parts,
+		                TotalCommit{total})) != hipSuccess)
+			return hip_check(ctx, e, "vox_scan_parts");
+		if ((e = launch(ctx, "vox_scan_down", scan_down<CountSrc, OffsetDst>, dim3(kScanBlocks), dim3(kScanThreads),
+		                0, src, OffsetDst{fl->offsets}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
+			return hip_check(ctx, e, "vox_scan_down");
+		uint64_t F = 0;
+		if ((e = hipMemcpyAsync(&ctx->h_state->need_words, total, sizeof(uint64_t), hipMemcpyDeviceToHost,
+		                        ctx->stream)) != hipSuccess)
+			return hip_check(ctx, e, "read fragment count");
+		ctx->t_count.stop(ctx->stream);
+		if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+			return hip_check(ctx, e, "count pass");
+		F = ctx->h_state->need_words;
+		if (F > 0xFFFFFFFFull)
+			return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 fragments");
+		fl->F = F;
+	} else {
+		ctx->t_count.stop(ctx->stream);
+		fl->F = 0;
+	}
+	if ((e = ctx->pool->alloc((void **)&fl->frags, sizeof(uint32_t) * 2 * (fl->F ? fl->F : 1))) != hipSuccess)
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "fragment list allocation failed");
+	return SVO_OK;
+}
+
+int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl) {
+	const svo_scene *sc = fl->scene;
+	const uint64_t T = sc->T;
+	hipError_t e;
+	ctx->t_fill.start(ctx->stream);
+	if (T > 0 && fl->F > 0) {
+		if ((e = launch(ctx, "vox_fill", vox_fill, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const float *)sc->xyz,
+		                (const uint32_t *)sc->rgb, T, 1u << fl->level, (const uint32_t *)fl->offsets,
+		                reinterpret_cast<uint2 *>(fl->frags))) != hipSuccess)
+			return hip_check(ctx, e, "vox_fill");
+	}
+	ctx->t_fill.stop(ctx->stream);
+	fl->filled = true;
+	return SVO_OK;
+}
+
+} // namespace svo

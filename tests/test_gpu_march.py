@@ -1,0 +1,171 @@
+"""ESVO marching on the GPU against the oracle: every field of every hit record bit-exact.
+
+Reference: shader/octree.glsl (Octree_RayMarchCoarse / RayMarchLeaf 6-arg and 5-arg /
+RayMarchOcclude), shader/octree_tracer_beam.frag, shader/octree_tracer.frag, shader/camera.glsl,
+shader/diffuse.glsl, shader/util.glsl, src/OctreeTracer.cpp:183-246.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from svo_testutil import hit_mask, hits_equal
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def scene_octree(ctx):
+    import sparsevoxeloctree_amd as S
+
+    xyz, rgb = S.gen_scene(1, 0x5EED0001, 24)
+    level = 8
+    sc = S.Scene.Create(ctx, xyz, rgb)
+    vox = S.Voxelizer.Create(sc, level)
+    vox.CmdVoxelize()
+    b = S.OctreeBuilder.Create(vox)
+    b.CmdBuild()
+    octree = b.GetOctree()
+    words = octree.words()
+    return octree, words, level, (b, vox, sc)
+
+
+def cameras():
+    import sparsevoxeloctree_amd as S
+
+    out = []
+    for k, (eye, fov) in enumerate(
+        [
+            ((2.9, 2.1, 0.6), math.pi / 3),
+            ((1.5, 1.5, 3.2), math.pi / 3),
+            ((0.2, 1.9, 1.1), math.pi / 4),
+            ((1.45, 1.55, 1.5), math.pi / 2),  # inside the octree
+        ]
+    ):
+        out.append(S.camera_look_at(eye, (1.5, 1.5, 1.5), fov=fov, aspect=1.25).uniform_array())
+    return out
+
+
+@pytest.mark.parametrize("cam_idx", [0, 1, 2, 3])
+@pytest.mark.parametrize("beam", [False, True])
+def test_march_frame_matches_oracle(ctx, no_device_errors, scene_octree, cam_idx, beam):
+    import sparsevoxeloctree_amd as S
+
+    octree, words, level, _ = scene_octree
+    cam = cameras()[cam_idx]
+    W, H = 160, 128
+    print("march frame cam=%d beam=%d" % (cam_idx, beam), flush=True)
+    tr = S.OctreeTracer.Create(octree, _camera_struct(cam))
+    tr.Resize(W, H)
+    tr.m_beam_enable = beam
+    tr.CmdDrawPipeline()
+    h_gpu = tr.GetHits()
+    b_orc = None
+    if beam:
+        b_orc, _ = oracle.beam(words, level, cam, W, H)
+        assert np.array_equal(tr.GetBeam().view(np.uint32), b_orc.view(np.uint32))
+    oracle.reset_counters()
+    h_orc = oracle.march_frame(words, level, cam, W, H, b_orc)
+    assert oracle.stack_uninit_reads() == 0
+    ok, msg = hits_equal(h_gpu, h_orc)
+    assert ok, msg
+    if cam_idx != 3:
+        assert hit_mask(h_gpu).sum() > 100
+
+
+def _camera_struct(cam):
+    from sparsevoxeloctree_amd._capi import svo_camera
+
+    c = svo_camera()
+    for i, name in enumerate(("pos", "look", "side", "up")):
+        for k in range(4):
+            getattr(c, name)[k] = float(cam[4 * i + k])
+    return c
+
+
+def _random_rays(seed, n):
+    rng = np.random.default_rng(seed)
+    o = np.concatenate(
+        [rng.uniform(0.5, 2.5, (n // 2, 3)), rng.uniform(1.0, 2.0, (n - n // 2, 3))]
+    ).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d.astype(np.float32)
+    # axis-aligned and tiny components exercise the EPS clamp (>= vs > in the two variants)
+    d[: n // 16, 1:] = 0.0
+    d[n // 16 : n // 8, 0] = np.float32(3.552713678800501e-15)
+    d[n // 8 : n // 8 + n // 32, 2] = -0.0
+    return np.concatenate([o, d], axis=1).astype(np.float32)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_march_rays_match_oracle(ctx, no_device_errors, scene_octree, variant):
+    import sparsevoxeloctree_amd as S
+
+    octree, words, level, _ = scene_octree
+    rays = _random_rays(77 + variant, 20000)
+    h_gpu = S.march_rays(ctx, octree, rays, variant)
+    h_orc = oracle.march_rays(words, level, rays, variant)
+    ok, msg = hits_equal(h_gpu, h_orc)
+    assert ok, msg
+
+
+def test_bounce_rays_and_march_match_oracle(ctx, no_device_errors, scene_octree):
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    octree, words, level, _ = scene_octree
+    cam = cameras()[0]
+    W, H = 128, 96
+    tr = S.OctreeTracer.Create(octree, _camera_struct(cam))
+    tr.Resize(W, H)
+    tr.m_beam_enable = True
+    tr.CmdDrawPipeline()
+    R = W * H
+    d_rays = ctx.alloc(R * 24)
+    seed = 0x5EED0002
+    ctx.check(lib.svo_bounce_rays(ctx.h, tr.hits.ptr, R, seed, d_rays.ptr))
+    rays_gpu = d_rays.download(np.float32, R * 6).reshape(R, 6)
+    rays_orc, mask = oracle.bounce_rays(tr.GetHits(), seed)
+    assert np.array_equal(rays_gpu.view(np.uint32), rays_orc.view(np.uint32))
+    assert mask.sum() > 100
+    d_hits = ctx.alloc(R * 32)
+    ctx.check(lib.svo_march_rays(ctx.h, octree.h, d_rays.ptr, R, 1, d_hits.ptr))
+    h_gpu = d_hits.download(S.HIT_DTYPE, R)
+    ok, msg = hits_equal(h_gpu, oracle.march_rays(words, level, rays_orc, 1))
+    assert ok, msg
+
+
+def test_march_uploaded_noncanonical_octree(ctx, no_device_errors, scene_octree):
+    # an octree in a different block order (as the reference's atomics produce) marches to the
+    # same hits once canonicalised, and the marcher itself does not care about the order
+    import sparsevoxeloctree_amd as S
+
+    octree, words, level, _ = scene_octree
+    shuffled = _shuffle_blocks(words, level, seed=5)
+    assert not np.array_equal(shuffled, words)
+    assert np.array_equal(oracle.canonicalize(shuffled, level), words)
+    o2 = S.Octree.FromWords(ctx, shuffled, level)
+    rays = _random_rays(3, 4000)
+    h_a = S.march_rays(ctx, o2, rays, 0)
+    h_b = oracle.march_rays(shuffled, level, rays, 0)
+    ok, msg = hits_equal(h_a, h_b)
+    assert ok, msg
+    h_c = S.march_rays(ctx, octree, rays, 0)
+    assert np.array_equal(h_a["vxy"], h_c["vxy"]) and np.array_equal(h_a["iter"], h_c["iter"])
+
+
+def _shuffle_blocks(words, level, seed):
+    """Relayout the non-root blocks in a random order, rewriting child pointers."""
+    rng = np.random.default_rng(seed)
+    nblk = words.shape[0] // 8
+    perm = np.concatenate([[0], 1 + rng.permutation(nblk - 1)])  # new position of old block b
+    out = np.zeros_like(words)
+    for b in range(nblk):
+        blk = words[8 * b : 8 * b + 8].copy()
+        internal = ((blk & 0x80000000) != 0) & ((blk & 0x40000000) == 0)
+        ptrs = (blk & 0x3FFFFFFF) // 8
+        blk[internal] = 0x80000000 | (perm[ptrs[internal]] * 8).astype(np.uint32)
+        out[8 * perm[b] : 8 * perm[b] + 8] = blk
+    return out
