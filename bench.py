@@ -1,0 +1,290 @@
+"""Benchmark of the MI355X SVO hot path (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--no-cpu]
+
+A step is one pass of the hot path over the resident scene: voxelize (count pass, allocation,
+fill), level-by-level octree build, beam pre-pass + primary-ray march of the whole frame, and for
+the room scene one cosine-weighted diffuse bounce (bounce generation + 5-argument march).
+The triangles are uploaded and normalised before timing (the reference loads and normalises on
+the host once, src/Scene.cpp:36-147).
+
+Prints one JSON line.  `value` is the primary-ray throughput (Mrays/s, beam pre-pass included)
+of the timed steps; the SVO build time is reported in the reference's scope (fill voxelize + all
+level passes, device time, src/LoaderThread.cpp:57-75) and end to end (count + allocation + fill +
+build, wall clock).  Per-kernel device times come from HIP events on the library's stream
+(svo_prof_*); `roofline` prices the dominant kernel's algorithmic bytes against 8 TB/s.
+`cpu_baseline` times the CPU oracle (a C++ port of the reference's algorithm) on the host cores.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SVO build ms @2^10/2^12; primary Mrays/s; batched/served p50/p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+CONFIGS = {
+    1: dict(
+        workload="c1: 64 UV spheres (258,048 tris) voxelized at 2^8, SVO build, 512x512 primary rays",
+        kind=1, seed=0x5EED0001, param=64, level=8, width=512, height=512, bounce=False, camera="orbit",
+    ),
+    2: dict(
+        workload="c2: box room (walls 0.5% tessellation) + 400 rotated boxes + 200 spheres (990,800 tris) "
+        "at 2^10, full build, 1920x1080 primary rays + 1 cosine diffuse bounce",
+        kind=2, seed=0x5EED0002, param=1, level=10, width=1920, height=1080, bounce=True, camera="room",
+    ),
+    3: dict(
+        workload="c3: 992 UV spheres + 3,999,744 slivers (aspect log-U[1,1000]) at 2^12, full build, "
+        "3840x2160 primary rays",
+        kind=3, seed=0x5EED0003, param=992, level=12, width=3840, height=2160, bounce=False, camera="orbit",
+    ),
+}
+
+
+def make_camera(S, cfg, xyz):
+    aspect = cfg["width"] / cfg["height"]
+    if cfg["camera"] == "room":
+        # inside the room near the -z wall, looking along +z (the reference's default yaw/pitch = 0)
+        return S.Camera((1.5, 1.45, 1.04), 0.0, 0.0, math.pi / 3.0, aspect)
+    # eye on a sphere of radius 1.5 about the world point (0.5,0.5,0.5), mapped to octree space by
+    # the scene normalisation (src/Scene.cpp:90-99) and v -> 1.5 + 0.5 v
+    p = xyz.reshape(-1, 3).astype(np.float64)
+    mn, mx = p.min(axis=0), p.max(axis=0)
+    centre, ext = (mn + mx) * 0.5, (mx - mn).max() * 0.5
+    rng = np.random.default_rng(cfg["seed"])
+    v = rng.normal(size=3)
+    v /= np.linalg.norm(v)
+    eye_w = np.array([0.5, 0.5, 0.5]) + 1.5 * v
+    to_oct = lambda w: 1.5 + 0.5 * (w - centre) / ext  # noqa: E731
+    return S.camera_look_at(tuple(to_oct(eye_w)), tuple(to_oct(np.array([0.5, 0.5, 0.5]))), math.pi / 3.0, aspect)
+
+
+def run_step(S, ctx, scene, cfg, cam, bufs):
+    from sparsevoxeloctree_amd._capi import lib
+
+    t0 = time.perf_counter()
+    vox = S.Voxelizer.Create(scene, cfg["level"])  # count pass + read-back of F + allocation
+    vox.CmdVoxelize()  # fill
+    builder = S.OctreeBuilder.Create(vox)
+    builder.CmdBuild()
+    W = builder.GetOctreeRange() // 4  # waits for the build
+    t_build_e2e = time.perf_counter() - t0
+    octree = builder.GetOctree()
+    w, h = cfg["width"], cfg["height"]
+    R = w * h
+    ctx.check(lib.svo_march(ctx.h, octree.h, C_cam(cam), w, h, 1, bufs["hits"].ptr, bufs["beam"].ptr), "march")
+    if cfg["bounce"]:
+        ctx.check(lib.svo_bounce_rays(ctx.h, bufs["hits"].ptr, R, cfg["seed"], bufs["rays"].ptr), "bounce")
+        ctx.check(lib.svo_march_rays(ctx.h, octree.h, bufs["rays"].ptr, R, 1, bufs["hits2"].ptr), "bounce march")
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    stage = ctx.timing()
+    F = vox.GetVoxelFragmentCount()
+    return dict(wall=wall, build_e2e=t_build_e2e, stage=stage, F=F, W=W), (vox, builder, octree)
+
+
+_CAM_CACHE = {}
+
+
+def C_cam(cam):
+    import ctypes
+
+    key = id(cam)
+    if key not in _CAM_CACHE:
+        _CAM_CACHE[key] = cam.uniform()
+    return ctypes.byref(_CAM_CACHE[key])
+
+
+def cpu_baseline(S, cfg, cam, words, budget_s=20.0):
+    """The CPU oracle (port of the reference's traversal) on the host cores: beam pre-pass + primary
+    march of a band of rows of the same frame, sized to a bounded amount of work."""
+    import oracle
+
+    cores = max(1, min(16, os.cpu_count() or 1))
+    level, w, h = cfg["level"], cfg["width"], cfg["height"]
+    cam_arr = cam.uniform_array()
+    t0 = time.perf_counter()
+    beam, _ = oracle.beam(words, level, cam_arr, w, h, threads=cores)
+    t_beam = time.perf_counter() - t0
+    # probe a small band to size the sample
+    y0 = h // 2
+    t0 = time.perf_counter()
+    oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + 8, beam, threads=cores)
+    per_row = (time.perf_counter() - t0) / 8
+    rows = int(max(8, min(h, budget_s / max(per_row, 1e-6))))
+    y0 = max(0, h // 2 - rows // 2)
+    t0 = time.perf_counter()
+    oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + rows, beam, threads=cores)
+    t_rows = time.perf_counter() - t0
+    t = t_rows + t_beam * rows / h  # the beam pass's share for those rows
+    return {
+        "value": round(rows * w / t / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": "CPU oracle, beam pre-pass + Octree_RayMarchLeaf for rows %d..%d of the %dx%d frame "
+        "(%d rays, %.1f s)" % (y0, y0 + rows, w, h, rows * w, t),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(
+            "bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import sparsevoxeloctree_amd as S
+
+    cfg = CONFIGS[args.config]
+    xyz, rgb = S.gen_scene(cfg["kind"], cfg["seed"], cfg["param"])
+    T = rgb.shape[0]
+    cam = make_camera(S, cfg, xyz)
+    ctx = S.Context(0)
+    scene = S.Scene.Create(ctx, xyz, rgb, normalize=True)
+    R = cfg["width"] * cfg["height"]
+    bw, bh = (cfg["width"] + 7) // 8 + 1, (cfg["height"] + 7) // 8 + 1
+    bufs = {"hits": ctx.alloc(R * 32), "beam": ctx.alloc(bw * bh * 4)}
+    if cfg["bounce"]:
+        bufs["rays"] = ctx.alloc(R * 24)
+        bufs["hits2"] = ctx.alloc(R * 32)
+
+    for _ in range(args.warmup):
+        run_step(S, ctx, scene, cfg, cam, bufs)
+    ctx.synchronize()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    results = []
+    keep = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r, keep = run_step(S, ctx, scene, cfg, cam, bufs)
+        results.append(r)
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    ctx.prof_enable(False)
+    prof = ctx.prof_stats()
+    assert ctx.device_errors() == 0
+
+    st = {k: float(np.mean([r["stage"][k] for r in results])) for k in results[0]["stage"]}
+    F, W = results[-1]["F"], results[-1]["W"]
+    march_ms = st["beam_ms"] + st["march_ms"]
+    value = R / (march_ms * 1e-3) / 1e6
+    build_ref = st["vox_fill_ms"] + st["build_ms"]
+    build_e2e = float(np.mean([r["build_e2e"] for r in results])) * 1e3
+
+    # algorithmic bytes of the kernels (fixed formulas, implementation independent)
+    _vox, _builder, octree = keep
+    hits = bufs["hits"].download(S.HIT_DTYPE, R)
+    nfetch_primary = int(hits["nfetch"].astype(np.uint64).sum())
+    from sparsevoxeloctree_amd._capi import lib
+
+    nf_beam_buf = ctx.alloc(bw * bh * 4)
+    beam_tmp = ctx.alloc(bw * bh * 4)
+    ctx.check(lib.svo_march_beam(ctx.h, octree.h, C_cam(cam), cfg["width"], cfg["height"], beam_tmp.ptr, nf_beam_buf.ptr))
+    nfetch_beam = int(nf_beam_buf.download(np.uint32, bw * bh).astype(np.uint64).sum())
+    L = cfg["level"]
+    alg = {
+        "march_frame": 4 * nfetch_primary + R * 32,
+        "march_beam": 4 * nfetch_beam + bw * bh * 4,
+        "vox_fill": 40 * T + 4 * T + 8 * F,
+        "vox_count": 40 * T + 4 * T,
+    }
+    nb = None
+    if cfg["bounce"]:
+        hits2 = bufs["hits2"].download(S.HIT_DTYPE, R)
+        nb = int(((hits["vz_flags"] >> 31) & 1).sum())
+        alg["march_rays5"] = 4 * int(hits2["nfetch"].astype(np.uint64).sum()) + R * (24 + 32)
+        alg["bounce"] = R * (32 + 24)
+    kernel_ms = {k: v[1] / max(v[0], 1) for k, v in prof.items()}
+    kernel_total = {k: v[1] for k, v in prof.items()}
+    dominant = max(kernel_total, key=kernel_total.get)
+    roofline = None
+    if dominant in alg:
+        achieved = alg[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic_%s_c%d.json" % (dominant, args.config))
+        if os.path.exists(tpath):
+            with open(tpath) as f:
+                traffic = json.load(f).get("bytes_per_launch")
+        roofline = {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": traffic,
+            "alg_bytes_per_launch": alg[dominant],
+            "avg_launch_ms": round(kernel_ms[dominant], 5),
+        }
+    build_bytes = 8 * F * L + 12 * W  # B_build
+    vox_bytes = 40 * T + 8 * F  # B_vox
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {
+            "workload": cfg["workload"],
+            "triangles": T,
+            "level": L,
+            "width": cfg["width"],
+            "height": cfg["height"],
+            "fragments": F,
+            "octree_words": W,
+            "beam": True,
+            "bounce": cfg["bounce"],
+        },
+        "primary_mrays_per_s": round(value, 3),
+        "primary_ms": round(march_ms, 4),
+        "build_ms_ref_scope": round(build_ref, 4),
+        "build_ms_end_to_end": round(build_e2e, 4),
+        "build_gbs_ref_scope": round((build_bytes + vox_bytes) / (build_ref * 1e-3) / 1e9, 2),
+        "stage_ms": {k: round(v, 4) for k, v in st.items()},
+        "kernel_ms": {k: round(v, 5) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernel_total[kv[0]])},
+        "roofline": roofline,
+    }
+    if cfg["bounce"]:
+        line["bounce_rays"] = nb
+        line["bounce_mrays_per_s"] = round(nb / (st["rays_ms"] * 1e-3) / 1e6, 3)
+    mem = ctx.memory()
+    line["peak_pool_bytes"] = mem["peak_in_use"]
+    if not args.no_cpu:
+        words = octree.words()
+        line["cpu_baseline"] = cpu_baseline(S, cfg, cam, words, args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    del keep, octree, _vox, _builder
+    print(json.dumps(line))
+    sys.stdout.flush()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -169,6 +169,11 @@ int svo_octree_destroy(svo_octree *oct);
  * (ceil(W/8)+1) x (ceil(H/8)+1) beam image. */
 int svo_march(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
               int beam, svo_hit *hits_dev, float *beam_dev);
+/* The beam pre-pass alone (OctreeTracer::CmdBeamRenderPass): beam_dev receives the
+ * (ceil(W/8)+1) x (ceil(H/8)+1) image of max(0, t - size) or 1e10; nfetch_dev (optional) the octree
+ * words each beam cell loaded. */
+int svo_march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
+                   float *beam_dev, uint32_t *nfetch_dev);
 /* Explicit rays (path-tracer bounces, shader/path_tracer.comp:32-60): rays_dev = R x {o.xyz, d.xyz}
  * floats in octree space; variant = SVO_MARCH_*. */
 int svo_march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays_dev, uint64_t R, int variant,

@@ -43,6 +43,7 @@ def load():
             "orc_gen_ray": (None, [P, f32, f32, P]),
             "orc_beam": (None, [P, u64, u32, P, u32, u32, P, P, i32]),
             "orc_march_frame": (None, [P, u64, u32, P, u32, u32, P, P, i32]),
+            "orc_march_frame_rows": (None, [P, u64, u32, P, u32, u32, u32, u32, P, P, i32]),
             "orc_march_rays": (None, [P, u64, u32, P, u64, i32, P, i32]),
             "orc_bounce_rays": (None, [P, u64, u64, P, P]),
             "orc_sincos": (None, [f32, P, P]),
@@ -136,6 +137,18 @@ def march_frame(words, level, cam, width, height, beam_img=None, threads=None):
     b = None if beam_img is None else np.ascontiguousarray(beam_img, dtype=np.float32)
     lib.orc_march_frame(_p(words), words.shape[0], level, _p(cam), width, height, _p(b), _p(hits), threads or nthreads())
     return hits.reshape(height, width)
+
+
+def march_frame_rows(words, level, cam, width, height, y0, y1, beam_img=None, threads=None):
+    lib = load()
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    hits = np.empty(width * (y1 - y0), dtype=HIT_DTYPE)
+    b = None if beam_img is None else np.ascontiguousarray(beam_img, dtype=np.float32)
+    lib.orc_march_frame_rows(
+        _p(words), words.shape[0], level, _p(cam), width, height, y0, y1, _p(b), _p(hits), threads or nthreads()
+    )
+    return hits.reshape(y1 - y0, width)
 
 
 def march_rays(words, level, rays, variant=0, threads=None):

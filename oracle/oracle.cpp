@@ -716,13 +716,19 @@ void orc_beam(const uint32_t *words, uint64_t W, uint32_t level, const float *ca
 
 void orc_march_frame(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
                      uint32_t height, const float *beam, orc_hit *hits, int nthreads) {
+	orc_march_frame_rows(words, W, level, cam, width, height, 0, height, beam, hits, nthreads);
+}
+
+void orc_march_frame_rows(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                          uint32_t height, uint32_t y0, uint32_t y1, const float *beam, orc_hit *hits, int nthreads) {
 	(void)level;
 	Octree oc{words, W};
 	uint32_t bw = beam_dim(width);
 	float fw = (float)width, fh = (float)height;
-	parallel_for((uint64_t)width * height, nthreads, [&](uint64_t b, uint64_t e) {
-		for (uint64_t k = b; k < e; ++k) {
-			uint32_t px = (uint32_t)(k % width), py = (uint32_t)(k / width);
+	parallel_for((uint64_t)width * (y1 - y0), nthreads, [&](uint64_t b, uint64_t e) {
+		for (uint64_t kk = b; kk < e; ++kk) {
+			uint64_t k = kk; // index into hits (rows y0..y1)
+			uint32_t px = (uint32_t)(kk % width), py = y0 + (uint32_t)(kk / width);
 			// octree_tracer.frag:23-32
 			V3 o = cam_vec(cam, 0);
 			V3 d = gen_ray(cam, (float)px / fw, (float)py / fh);

@@ -68,6 +68,10 @@ void orc_beam(const uint32_t *words, uint64_t W, uint32_t level, const float *ca
 void orc_march_frame(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
                      uint32_t height, const float *beam, orc_hit *hits, int nthreads);
 
+/* Rows [y0, y1) of the same frame only; hits[width*(y1-y0)]. */
+void orc_march_frame_rows(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                          uint32_t height, uint32_t y0, uint32_t y1, const float *beam, orc_hit *hits, int nthreads);
+
 /* Explicit rays: rays[6*R] = (o.xyz, d.xyz).  variant 0 = Octree_RayMarchLeaf 6-arg
  * (shader/octree.glsl:179-340), 1 = 5-arg (shader/octree.glsl:342-502), 2 = Octree_RayMarchOcclude
  * (shader/octree.glsl:504-629; hit flag only, other fields as documented in include/svo.h). */

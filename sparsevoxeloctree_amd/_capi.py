@@ -113,6 +113,7 @@ _SIGS = {
     "svo_octree_download": (i32, [P, P, P, u64]),
     "svo_octree_destroy": (i32, [P]),
     "svo_march": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, P, P]),
+    "svo_march_beam": (i32, [P, P, C.POINTER(svo_camera), u32, u32, P, P]),
     "svo_march_rays": (i32, [P, P, P, u64, i32, P]),
     "svo_bounce_rays": (i32, [P, P, u64, u64, P]),
     "svo_camera_from_yaw_pitch": (i32, [C.POINTER(C.c_float), f32, f32, f32, f32, C.POINTER(svo_camera)]),

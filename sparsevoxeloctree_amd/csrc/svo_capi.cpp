@@ -811,6 +811,18 @@ int svo_march(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t
 	return march_frame(ctx, o, cam, width, height, beam, hits_dev, beam_dev);
 }
 
+int svo_march_beam(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t width, uint32_t height,
+                   float *beam_dev, uint32_t *nfetch_dev) {
+	if (!ctx || !o || !cam || !beam_dev || o->pending)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (width == 0 || height == 0)
+		return SVO_OK;
+	if ((uint64_t)width * height > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame too large");
+	DeviceGuard g(ctx);
+	return march_beam(ctx, o, cam, width, height, beam_dev, nfetch_dev);
+}
+
 int svo_march_rays(svo_ctx *ctx, const svo_octree *o, const float *rays_dev, uint64_t R, int variant,
                    svo_hit *hits_dev) {
 	if (!ctx || !o || o->pending || (R && (!rays_dev || !hits_dev)) || variant < 0 || variant > 2)

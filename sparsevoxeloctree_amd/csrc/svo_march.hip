@@ -453,6 +453,22 @@ uint64_t splitmix64(uint64_t &s) {
 
 } // namespace
 
+int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam_img,
+               uint32_t *nfetch) {
+	const CamArgs ca = cam_args(cam);
+	const uint32_t bw = (w + 7u) / 8u + 1u, bh = (h + 7u) / 8u + 1u; // src/OctreeTracer.cpp:6
+	// exp2f(1.0f - level) (src/OctreeTracer.cpp:206): an exact power of two
+	const float orig_sz = ldexpf(1.0f, 1 - (int)oct->level);
+	ctx->t_beam.start(ctx->stream);
+	hipError_t e = launch(ctx, "march_beam", march_beam_kernel, dim3(div_up((uint64_t)bw * bh, kBlock)), dim3(kBlock),
+	                      0, (const uint32_t *)oct->words, oct->W, ca, w, h, bw, bh, orig_sz, beam_img, nfetch,
+	                      ctx->d_err);
+	ctx->t_beam.stop(ctx->stream);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "march_beam");
+	return SVO_OK;
+}
+
 int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
                 svo_hit *hits, float *beam_out) {
 	hipError_t e;
@@ -465,15 +481,9 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 			if ((e = ctx->pool->alloc((void **)&beam_img, sizeof(float) * bw * bh)) != hipSuccess)
 				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "beam image allocation failed");
 		}
-		// exp2f(1.0f - level) (src/OctreeTracer.cpp:206): an exact power of two
-		const float orig_sz = ldexpf(1.0f, 1 - (int)oct->level);
-		ctx->t_beam.start(ctx->stream);
-		e = launch(ctx, "march_beam", march_beam_kernel, dim3(div_up((uint64_t)bw * bh, kBlock)), dim3(kBlock), 0,
-		           (const uint32_t *)oct->words, oct->W, ca, w, h, bw, bh, orig_sz, beam_img, (uint32_t *)nullptr,
-		           ctx->d_err);
-		ctx->t_beam.stop(ctx->stream);
-		if (e != hipSuccess)
-			return hip_check(ctx, e, "march_beam");
+		int rc = march_beam(ctx, oct, cam, w, h, beam_img, nullptr);
+		if (rc)
+			return rc;
 	}
 	const uint32_t tiles_x = (w + 15u) / 16u, tiles_y = (h + 15u) / 16u;
 	ctx->t_march.start(ctx->stream);

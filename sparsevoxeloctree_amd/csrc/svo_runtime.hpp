@@ -175,6 +175,8 @@ int build_enqueue(svo_ctx *ctx, svo_octree *oct, uint32_t resume_level);
 int build_finish(svo_ctx *ctx, svo_octree *oct);
 int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
                 svo_hit *hits, float *beam_out);
+int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam,
+               uint32_t *nfetch);
 int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits);
 int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays);
 
