@@ -172,15 +172,19 @@ bool covers(const Tri &s, int64_t i, int64_t j) {
 	return true;
 }
 
-// Depth at the pixel centre: plane of the projected triangle (pixel units, z in [0,1]).
+// Depth at the pixel centre: plane of the projected triangle (pixel units, z in [0,1]) through
+// vertex 0 with gradients dz/dx = -nx/nz, dz/dy = -ny/nz (both 0 when the projection is degenerate).
 float depth_at(const Tri &s, uint32_t i, uint32_t j) {
 	float ax = s.px[1] - s.px[0], ay = s.py[1] - s.py[0], az = s.pz[1] - s.pz[0];
 	float bx = s.px[2] - s.px[0], by = s.py[2] - s.py[0], bz = s.pz[2] - s.pz[0];
 	float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+	float dzdx = 0.0f, dzdy = 0.0f;
+	if (nz != 0.0f) {
+		dzdx = -nx / nz;
+		dzdy = -ny / nz;
+	}
 	float cx = (float)i + 0.5f, cy = (float)j + 0.5f;
-	if (nz == 0.0f)
-		return s.pz[0];
-	return s.pz[0] - (nx * (cx - s.px[0]) + ny * (cy - s.py[0])) / nz;
+	return s.pz[0] + (dzdx * (cx - s.px[0]) + dzdy * (cy - s.py[0]));
 }
 
 template <class Emit> uint64_t raster(const Tri &s, uint32_t res, uint32_t colour, Emit &&emit) {

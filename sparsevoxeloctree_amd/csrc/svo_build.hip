@@ -42,16 +42,26 @@ __device__ __forceinline__ uint32_t child_slot(uint32_t x, uint32_t y, uint32_t 
 	return ((x >> shift) & 1u) | (((y >> shift) & 1u) << 1u) | (((z >> shift) & 1u) << 2u);
 }
 
-// Running average of the reference's CAS loop (octree_tag_node.comp:43-57), one fragment at a time.
-__device__ __forceinline__ uint32_t fold_push(uint32_t prev, uint32_t c) {
-	if (prev == 0u)
-		return 0xC1000000u | c;
-	uint32_t n = (prev >> 24u) & 0x3fu;
-	uint32_t w = n + 1u;
+// Running average of the reference's CAS loop (octree_tag_node.comp:43-57): fold colour c into a
+// leaf word that already holds n fragments (the word's 6-bit count, saturated at 63).  In a
+// sequence the a-th fragment (a >= 1) always meets n = min(a, 63), so unrolled callers pass a
+// constant n and the divisions become multiplications.
+__device__ __forceinline__ uint32_t fold_step(uint32_t prev, uint32_t c, uint32_t n) {
+	const uint32_t w = n + 1u;
 	uint32_t r = ((prev & 0xffu) * n + (c & 0xffu)) / w;
 	uint32_t g = (((prev >> 8u) & 0xffu) * n + ((c >> 8u) & 0xffu)) / w;
 	uint32_t b = (((prev >> 16u) & 0xffu) * n + ((c >> 16u) & 0xffu)) / w;
 	return (umin32(w, 0x3fu) << 24u) | (r & 0xffu) | ((g & 0xffu) << 8u) | ((b & 0xffu) << 16u) | 0xC0000000u;
+}
+
+__device__ __forceinline__ uint32_t fold_push(uint32_t prev, uint32_t c, uint32_t a) {
+	return a == 0u ? (0xC1000000u | c) : fold_step(prev, c, a < 63u ? a : 63u);
+}
+
+__device__ __forceinline__ void cmp_swap(uint64_t &x, uint64_t &y) {
+	const uint64_t lo = y < x ? y : x, hi = y < x ? x : y;
+	x = lo;
+	y = hi;
 }
 
 __device__ __forceinline__ uint64_t frag_key(uint2 f) { return ((uint64_t)f.x << 32) | f.y; }
@@ -186,19 +196,40 @@ __global__ void __launch_bounds__(kThreads)
 			continue;
 		}
 		uint2 *b = bucket + start;
-		for (uint32_t a = 1; a < n; ++a) {
-			uint2 v = b[a];
-			uint64_t kv = frag_key(v);
-			uint32_t c = a;
-			while (c > 0u && frag_key(b[c - 1]) > kv) {
-				b[c] = b[c - 1];
-				--c;
+		uint32_t word;
+		if (n <= 8u) {
+			// sort in registers: 19-comparator network for 8 keys, +inf padding
+			uint64_t k[8];
+#pragma unroll
+			for (uint32_t a = 0; a < 8u; ++a)
+				k[a] = a < n ? frag_key(b[a]) : ~0ull;
+			cmp_swap(k[0], k[1]), cmp_swap(k[2], k[3]), cmp_swap(k[4], k[5]), cmp_swap(k[6], k[7]);
+			cmp_swap(k[0], k[2]), cmp_swap(k[1], k[3]), cmp_swap(k[4], k[6]), cmp_swap(k[5], k[7]);
+			cmp_swap(k[1], k[2]), cmp_swap(k[5], k[6]), cmp_swap(k[0], k[4]), cmp_swap(k[3], k[7]);
+			cmp_swap(k[1], k[5]), cmp_swap(k[2], k[6]);
+			cmp_swap(k[1], k[4]), cmp_swap(k[3], k[6]);
+			cmp_swap(k[2], k[4]), cmp_swap(k[3], k[5]);
+			cmp_swap(k[3], k[4]);
+			word = 0xC1000000u | ((uint32_t)k[0] & 0xffffffu);
+#pragma unroll
+			for (uint32_t a = 1; a < 8u; ++a)
+				if (a < n)
+					word = fold_step(word, (uint32_t)k[a] & 0xffffffu, a);
+		} else {
+			for (uint32_t a = 1; a < n; ++a) {
+				uint2 v = b[a];
+				uint64_t kv = frag_key(v);
+				uint32_t c = a;
+				while (c > 0u && frag_key(b[c - 1]) > kv) {
+					b[c] = b[c - 1];
+					--c;
+				}
+				b[c] = v;
 			}
-			b[c] = v;
+			word = 0u;
+			for (uint32_t a = 0; a < n; ++a)
+				word = fold_push(word, b[a].y & 0xffffffu, a);
 		}
-		uint32_t word = 0u;
-		for (uint32_t a = 0; a < n; ++a)
-			word = fold_push(word, b[a].y & 0xffffffu);
 		oct[begin + j] = word;
 	}
 }
@@ -262,7 +293,7 @@ __global__ void __launch_bounds__(kThreads)
 			if (threadIdx.x == 0) {
 				uint32_t word = 0u;
 				for (uint32_t a = 0; a < n; ++a)
-					word = fold_push(word, (uint32_t)s_keys[a] & 0xffffffu);
+					word = fold_push(word, (uint32_t)s_keys[a] & 0xffffffu, a);
 				oct[begin + j] = word;
 			}
 		} else {
@@ -272,7 +303,7 @@ __global__ void __launch_bounds__(kThreads)
 			if (threadIdx.x == 0) {
 				uint32_t word = 0u;
 				for (uint32_t a = 0; a < n; ++a)
-					word = fold_push(word, b[a].y & 0xffffffu);
+					word = fold_push(word, b[a].y & 0xffffffu, a);
 				oct[begin + j] = word;
 			}
 		}

@@ -629,6 +629,7 @@ int svo_fraglist_destroy(svo_fraglist *fl) {
 	DeviceGuard g(fl->ctx);
 	fl->ctx->pool->free(fl->frags);
 	fl->ctx->pool->free(fl->offsets);
+	fl->ctx->pool->free(fl->bigtri);
 	delete fl;
 	return SVO_OK;
 }

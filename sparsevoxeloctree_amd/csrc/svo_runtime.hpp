@@ -124,7 +124,8 @@ struct svo_fraglist {
 	uint32_t level;
 	uint64_t F;
 	uint32_t *frags;   // 2 * F words
-	uint32_t *offsets; // per-triangle exclusive offsets from the count pass (T + 1), or null
+	uint32_t *offsets; // per-triangle exclusive offsets from the count pass (T), or null
+	uint32_t *bigtri;  // count + indices of the triangles with many fragments (1 + T), or null
 	bool filled;
 };
 

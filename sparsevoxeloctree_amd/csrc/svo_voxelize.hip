@@ -35,7 +35,7 @@ struct Setup {
 	uint32_t dr0, dr1;
 	int32_t ilo, ihi, jlo, jhi; // pixel rectangle: AABB ∩ snapped bbox ∩ [0, res)
 	float px0, py0, pz0;
-	float nx, ny, nz; // plane of the projected triangle (pixel units, depth in [0,1])
+	float dzdx, dzdy; // plane of the projected triangle (pixel units, depth in [0,1])
 	Edge e[3];
 };
 
@@ -81,9 +81,13 @@ __device__ void tri_setup(const float *__restrict__ tri, uint64_t t, uint32_t re
 	{
 		float ax = px[1] - px[0], ay = py[1] - py[0], az = pz[1] - pz[0];
 		float bx = px[2] - px[0], by = py[2] - py[0], bz = pz[2] - pz[0];
-		s.nx = ay * bz - az * by;
-		s.ny = az * bx - ax * bz;
-		s.nz = ax * by - ay * bx;
+		float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+		s.dzdx = 0.0f;
+		s.dzdy = 0.0f;
+		if (nz != 0.0f) {
+			s.dzdx = -nx / nz;
+			s.dzdy = -ny / nz;
+		}
 	}
 	int64_t xmn = imin64(X[0], imin64(X[1], X[2])), xmx = imax64(X[0], imax64(X[1], X[2]));
 	int64_t ymn = imin64(Y[0], imin64(Y[1], Y[2])), ymx = imax64(Y[0], imax64(Y[1], Y[2]));
@@ -138,8 +142,15 @@ __device__ __forceinline__ void row_span(const Setup &s, int32_t j, int32_t &i0,
 	}
 }
 
+__global__ void vox_zero_counter(uint32_t *c) { c[0] = 0u; }
+
+// Count pass.  Triangles with more than kBigTri fragments are also appended to `big` (big[0] is
+// the count) for vox_fill_big; the per-lane fill skips them.
+constexpr uint32_t kBigTri = 128;
+
 __global__ void __launch_bounds__(kThreads)
-    vox_count(const float *__restrict__ tri, uint64_t T, uint32_t res, uint32_t *__restrict__ counts) {
+    vox_count(const float *__restrict__ tri, uint64_t T, uint32_t res, uint32_t *__restrict__ counts,
+              uint32_t *__restrict__ big) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	for (uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x; t < T; t += stride) {
 		Setup s;
@@ -153,45 +164,139 @@ __global__ void __launch_bounds__(kThreads)
 					c += (uint32_t)(i1 - i0 + 1);
 			}
 		counts[t] = c;
+		if (c > kBigTri)
+			big[1 + atomicAdd(&big[0], 1u)] = (uint32_t)t;
 	}
 }
 
+// One fragment: depth at the pixel centre, clamps, swizzle back (voxelizer.frag:17-25) and packing
+// (voxelizer.frag:38-42).  dy = dz/dy * (j + 0.5 - py0) of the row.
+__device__ __forceinline__ uint2 make_frag(const Setup &s, int32_t i, int32_t j, float dy, uint32_t res,
+                                          uint32_t colour) {
+	const float zc = s.pz0 + (s.dzdx * (((float)i + 0.5f) - s.px0) + dy);
+	uint32_t uz = umin32(f2u_sat(zc * (float)res), res - 1u);
+	uz = umin32(umax32(uz, s.dr0), s.dr1);
+	const uint32_t ui = (uint32_t)i, uj = (uint32_t)j;
+	uint32_t vx, vy, vz;
+	if (s.axis == 0u)
+		vx = uz, vy = ui, vz = uj;
+	else if (s.axis == 1u)
+		vx = uj, vy = uz, vz = ui;
+	else
+		vx = ui, vy = uj, vz = uz;
+	return make_uint2(vx | (vy << 12u) | ((vz & 0xffu) << 24u), ((vz >> 8u) << 28u) | colour);
+}
+
+// Fill of the small triangles: each lane walks its triangle's rows and writes its fragments at the
+// triangle's offset (triangle order, row-major inside the triangle).
 __global__ void __launch_bounds__(kThreads)
     vox_fill(const float *__restrict__ tri, const uint32_t *__restrict__ rgb, uint64_t T, uint32_t res,
-             const uint32_t *__restrict__ offsets, uint2 *__restrict__ frags) {
+             const uint32_t *__restrict__ offsets, uint64_t F, uint2 *__restrict__ frags) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	for (uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x; t < T; t += stride) {
+		uint64_t k = offsets[t];
+		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : F;
+		if (k == kend || kend - k > kBigTri)
+			continue;
 		Setup s;
 		tri_setup(tri, t, res, s);
-		if (s.empty)
-			continue;
 		const uint32_t colour = rgb[t] & 0x00ffffffu;
-		uint64_t k = offsets[t];
-		const float fres = (float)res;
-		for (int32_t j = s.jlo; j <= s.jhi; ++j) {
+		for (int32_t j = s.jlo; j <= s.jhi && k < kend; ++j) {
 			int32_t i0, i1;
 			row_span(s, j, i0, i1);
-			const float cy = (float)j + 0.5f;
-			const float dy = s.ny * (cy - s.py0);
-			for (int32_t i = i0; i <= i1; ++i) {
-				// voxelizer.frag:17-25: depth at the pixel centre, clamps, swizzle back
-				float zc = s.pz0;
-				if (s.nz != 0.0f) {
-					float cx = (float)i + 0.5f;
-					zc = s.pz0 - (s.nx * (cx - s.px0) + dy) / s.nz;
-				}
-				uint32_t uz = umin32(f2u_sat(zc * fres), res - 1u);
-				uz = umin32(umax32(uz, s.dr0), s.dr1);
-				uint32_t ui = (uint32_t)i, uj = (uint32_t)j, vx, vy, vz;
-				if (s.axis == 0u)
-					vx = uz, vy = ui, vz = uj;
-				else if (s.axis == 1u)
-					vx = uj, vy = uz, vz = ui;
-				else
-					vx = ui, vy = uj, vz = uz;
-				// voxelizer.frag:38-42
-				frags[k++] = make_uint2(vx | (vy << 12u) | ((vz & 0xffu) << 24u), ((vz >> 8u) << 28u) | colour);
+			const float dy = s.dzdy * (((float)j + 0.5f) - s.py0);
+			for (int32_t i = i0; i <= i1 && k < kend; ++i)
+				frags[k++] = make_frag(s, i, j, dy, res, colour);
+		}
+	}
+}
+
+// Fill of the big triangles: one block per triangle.  Rows are taken kRowChunk at a time: one
+// thread per row computes the row's span into LDS, a block scan gives each row's first output
+// position, then every thread maps output positions p = tid, tid + 256, ... to (row, column) by a
+// binary search over the row offsets, so stores are coalesced and the work is even whatever the
+// triangle's shape.
+constexpr int kRowChunk = 1024;
+
+__global__ void __launch_bounds__(kThreads)
+    vox_fill_big(const float *__restrict__ tri, const uint32_t *__restrict__ rgb, uint64_t T, uint32_t res,
+                 const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ big, uint64_t F,
+                 uint2 *__restrict__ frags) {
+	__shared__ int32_t s_i0[kRowChunk];
+	__shared__ uint32_t s_off[kRowChunk + 1];
+	__shared__ uint32_t s_wave[kThreads / 64];
+	const uint32_t nbig = big[0];
+	for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
+		const uint64_t t = big[1 + b];
+		if (t >= T)
+			continue;
+		Setup s;
+		tri_setup(tri, t, res, s);
+		const uint32_t colour = rgb[t] & 0x00ffffffu;
+		uint64_t k = offsets[t];
+		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : F;
+		for (int32_t r0 = s.jlo; r0 <= s.jhi && k < kend; r0 += kRowChunk) {
+			const int32_t nrows = (s.jhi - r0 + 1) < kRowChunk ? (s.jhi - r0 + 1) : kRowChunk;
+			// spans and counts, kRowChunk / kThreads rows per thread
+			uint32_t local = 0;
+			int32_t cnt[kRowChunk / kThreads];
+#pragma unroll
+			for (int q = 0; q < kRowChunk / kThreads; ++q) {
+				const int32_t r = (int32_t)threadIdx.x * (kRowChunk / kThreads) + q;
+				int32_t i0 = 1, i1 = 0;
+				if (r < nrows)
+					row_span(s, r0 + r, i0, i1);
+				cnt[q] = i0 <= i1 ? i1 - i0 + 1 : 0;
+				if (r < nrows)
+					s_i0[r] = i0;
+				local += (uint32_t)cnt[q];
 			}
+			// block exclusive scan of the per-thread totals
+			const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+			uint32_t incl = local;
+#pragma unroll
+			for (int d = 1; d < 64; d <<= 1) {
+				uint32_t o = __shfl_up(incl, (unsigned)d, 64);
+				if ((int)lane >= d)
+					incl += o;
+			}
+			if (lane == 63u)
+				s_wave[wave] = incl;
+			__syncthreads();
+			uint32_t base = 0, total = 0;
+#pragma unroll
+			for (int w = 0; w < kThreads / 64; ++w) {
+				base += (w < (int)wave) ? s_wave[w] : 0u;
+				total += s_wave[w];
+			}
+			uint32_t run = base + incl - local;
+#pragma unroll
+			for (int q = 0; q < kRowChunk / kThreads; ++q) {
+				const int32_t r = (int32_t)threadIdx.x * (kRowChunk / kThreads) + q;
+				s_off[r] = run; // rows past nrows get the chunk total and are never selected
+				run += (uint32_t)cnt[q];
+			}
+			if (threadIdx.x == 0)
+				s_off[kRowChunk] = total;
+			__syncthreads();
+			const uint32_t avail = (uint32_t)((kend - k) < (uint64_t)total ? (kend - k) : total);
+			for (uint32_t p = threadIdx.x; p < avail; p += kThreads) {
+				// last row with s_off[row] <= p
+				int32_t lo = 0, hi = nrows - 1;
+				while (lo < hi) {
+					const int32_t mid = (lo + hi + 1) >> 1;
+					if (s_off[mid] <= p)
+						lo = mid;
+					else
+						hi = mid - 1;
+				}
+				const int32_t j = r0 + lo;
+				const int32_t i = s_i0[lo] + (int32_t)(p - s_off[lo]);
+				const float dy = s.dzdy * (((float)j + 0.5f) - s.py0);
+				frags[k + p] = make_frag(s, i, j, dy, res, colour);
+			}
+			k += total;
+			__syncthreads();
 		}
 	}
 }
@@ -320,12 +425,15 @@ int voxelize_count(svo_ctx *ctx, svo_fraglist *fl) {
 	hipError_t e;
 	if (T > 0xFFFFFFFFull)
 		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 triangles");
-	if ((e = ctx->pool->alloc((void **)&fl->offsets, sizeof(uint32_t) * (T ? T : 1))) != hipSuccess)
+	if ((e = ctx->pool->alloc((void **)&fl->offsets, sizeof(uint32_t) * (T ? T : 1))) != hipSuccess ||
+	    (e = ctx->pool->alloc((void **)&fl->bigtri, sizeof(uint32_t) * (T + 1))) != hipSuccess)
 		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "offsets allocation failed");
 	ctx->t_count.start(ctx->stream);
+	if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0, fl->bigtri)) != hipSuccess)
+		return hip_check(ctx, e, "vox_zero");
 	if (T > 0) {
 		if ((e = launch(ctx, "vox_count", vox_count, dim3(grid_for(T, ctx)), dim3(kThreads), 0,
-		                (const float *)sc->xyz, T, res, fl->offsets)) != hipSuccess)
+		                (const float *)sc->xyz, T, res, fl->offsets, fl->bigtri)) != hipSuccess)
 			return hip_check(ctx, e, "vox_count");
 		CountSrc src{fl->offsets, T};
 		uint64_t *total = ctx->d_scan_parts + 2 * kScanBlocks;
@@ -365,9 +473,14 @@ int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl) {
 	ctx->t_fill.start(ctx->stream);
 	if (T > 0 && fl->F > 0) {
 		if ((e = launch(ctx, "vox_fill", vox_fill, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const float *)sc->xyz,
-		                (const uint32_t *)sc->rgb, T, 1u << fl->level, (const uint32_t *)fl->offsets,
+		                (const uint32_t *)sc->rgb, T, 1u << fl->level, (const uint32_t *)fl->offsets, fl->F,
 		                reinterpret_cast<uint2 *>(fl->frags))) != hipSuccess)
 			return hip_check(ctx, e, "vox_fill");
+		if ((e = launch(ctx, "vox_fill_big", vox_fill_big, dim3((uint32_t)ctx->cu_count * 4u), dim3(kThreads), 0,
+		                (const float *)sc->xyz, (const uint32_t *)sc->rgb, T, 1u << fl->level,
+		                (const uint32_t *)fl->offsets, (const uint32_t *)fl->bigtri, fl->F,
+		                reinterpret_cast<uint2 *>(fl->frags))) != hipSuccess)
+			return hip_check(ctx, e, "vox_fill_big");
 	}
 	ctx->t_fill.stop(ctx->stream);
 	fl->filled = true;
