@@ -40,10 +40,14 @@ CONFIGS = {
         "at 2^10, full build, 1920x1080 primary rays + 1 cosine diffuse bounce",
         kind=2, seed=0x5EED0002, param=1, level=10, width=1920, height=1080, bounce=True, camera="room",
     ),
+    # the spec's sizes (radius U[0.02,0.1], sliver edge U[0.002,0.02]) give 1.3e9 fragments and an
+    # octree above 2^30 words, which the node format's 30-bit child pointers cannot address; both
+    # are scaled by 1/4 (about 1.8e8 fragments, 9.0e8 words)
     3: dict(
-        workload="c3: 992 UV spheres + 3,999,744 slivers (aspect log-U[1,1000]) at 2^12, full build, "
-        "3840x2160 primary rays",
-        kind=3, seed=0x5EED0003, param=992, level=12, width=3840, height=2160, bounce=False, camera="orbit",
+        workload="c3: 992 UV spheres (radius U[0.005,0.025]) + 3,999,744 slivers (edge U[0.0005,0.005], "
+        "aspect log-U[1,1000]) = 7,999,488 tris at 2^12, full build, 3840x2160 primary rays",
+        kind=3, seed=0x5EED0003, param=992 | (25 << 32) | (25 << 48), level=12, width=3840, height=2160,
+        bounce=False, camera="orbit",
     ),
 }
 
@@ -134,6 +138,23 @@ def cpu_baseline(S, cfg, cam, words, budget_s=20.0):
     }
 
 
+def copy_calibration(mib):
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    n = mib << 20
+    with S.Context(0) as ctx:
+        a, b = ctx.alloc(n), ctx.alloc(n)
+        ctx.prof_enable(True)
+        for _ in range(5):
+            ctx.check(lib.svo_copy_device(ctx.h, b.ptr, a.ptr, n))
+        ctx.synchronize()
+        stats = ctx.prof_stats()
+        cnt, ms = stats["copy_u128"]
+        print(json.dumps({"copy_bytes_per_launch": n, "read_bytes": n, "write_bytes": n, "launches": cnt,
+                          "avg_ms": ms / cnt, "gbs": 2 * n / (ms / cnt * 1e-3) / 1e9}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +163,14 @@ def main():
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument(
+        "--copy-calibration", type=int, default=0, metavar="MIB",
+        help="only run svo_copy_device over MIB MiB, 5 times (the known-size copy used to scale PMC byte counters)",
+    )
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(
             "bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -205,6 +233,13 @@ def main():
         "march_beam": 4 * nfetch_beam + bw * bh * 4,
         "vox_fill": 40 * T + 4 * T + 8 * F,
         "vox_count": 40 * T + 4 * T,
+        # one level pass streams every fragment once (B_build = 8FL + 12W)
+        "build_tag": 8 * F,
+        "build_tag_leaf": 8 * F,
+        # the canonical colour fold: every fragment read once and written once into its leaf bucket,
+        # then read once by the fold
+        "build_scatter": 16 * F,
+        "build_fold": 8 * F,
     }
     nb = None
     if cfg["bounce"]:

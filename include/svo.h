@@ -104,6 +104,10 @@ int svo_ctx_memory(svo_ctx *ctx, uint64_t *in_use, uint64_t *cached, uint64_t *p
 /* ---- device memory helpers, for callers that keep ray batches and hit buffers in HBM */
 int svo_malloc(svo_ctx *ctx, uint64_t bytes, void **dptr);
 int svo_free(svo_ctx *ctx, void *dptr);
+/* Device-to-device copy enqueued on the context's stream by a plain copy kernel (16-byte vectors
+ * when both pointers and the size are 16-byte aligned).  No reference counterpart (Vulkan
+ * CmdCopy); used to grow the node buffer and to calibrate HBM byte counters. */
+int svo_copy_device(svo_ctx *ctx, void *dst_dev, const void *src_dev, uint64_t bytes);
 int svo_upload(svo_ctx *ctx, void *dst_dev, const void *src_host, uint64_t bytes);   /* synchronous */
 int svo_download(svo_ctx *ctx, void *dst_host, const void *src_dev, uint64_t bytes); /* synchronous */
 
@@ -161,6 +165,11 @@ int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t 
 int svo_octree_info(const svo_octree *oct, uint32_t *level, uint64_t *W, const uint32_t **words_dev);
 int svo_octree_download(svo_ctx *ctx, const svo_octree *oct, uint32_t *out, uint64_t cap_words);
 int svo_octree_destroy(svo_octree *oct);
+/* Octree files (the reference keeps octrees only in device memory; this is new): a 24-byte header
+ * {char magic[8] = "SVOCTRE1", uint32 level, uint32 0, uint64 W} followed by W little-endian
+ * words in canonical order.  svo_octree_load returns an octree ready to march. */
+int svo_octree_save(svo_ctx *ctx, const svo_octree *oct, const char *path);
+int svo_octree_load(svo_ctx *ctx, const char *path, svo_octree **out);
 
 /* ---- OctreeTracer (CmdBeamRenderPass / CmdDrawPipeline, src/OctreeTracer.cpp:183-246).
  * Primary rays through every pixel of a width x height frame from the camera; beam != 0 runs the

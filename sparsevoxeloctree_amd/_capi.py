@@ -85,6 +85,7 @@ _SIGS = {
     "svo_ctx_memory": (i32, [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(u64)]),
     "svo_malloc": (i32, [P, u64, PP]),
     "svo_free": (i32, [P, P]),
+    "svo_copy_device": (i32, [P, P, P, u64]),
     "svo_upload": (i32, [P, P, P, u64]),
     "svo_download": (i32, [P, P, P, u64]),
     "svo_prof_enable": (i32, [P, i32]),
@@ -112,6 +113,8 @@ _SIGS = {
     "svo_octree_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
     "svo_octree_download": (i32, [P, P, P, u64]),
     "svo_octree_destroy": (i32, [P]),
+    "svo_octree_save": (i32, [P, P, C.c_char_p]),
+    "svo_octree_load": (i32, [P, C.c_char_p, PP]),
     "svo_march": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, P, P]),
     "svo_march_beam": (i32, [P, P, C.POINTER(svo_camera), u32, u32, P, P]),
     "svo_march_rays": (i32, [P, P, P, u64, i32, P]),

@@ -242,6 +242,20 @@ class Octree:
         o._own = True
         return o
 
+    @staticmethod
+    def Load(ctx, path):
+        """An octree file written by save() / svo_octree_save."""
+        h = C.c_void_p()
+        ctx.check(lib.svo_octree_load(ctx.h, str(path).encode(), C.byref(h)), "octree load")
+        lvl, W = C.c_uint32(), C.c_uint64()
+        lib.svo_octree_info(h, C.byref(lvl), C.byref(W), None)
+        o = Octree(ctx, h, lvl.value, W.value)
+        o._own = True
+        return o
+
+    def save(self, path):
+        self.ctx.check(lib.svo_octree_save(self.ctx.h, self.h, str(path).encode()), "octree save")
+
     def Update(self, builder):
         W = builder.GetOctreeRange() // 4
         self.ctx, self.h, self.level, self.W = builder.ctx, builder.h, builder.level, W

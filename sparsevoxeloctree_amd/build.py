@@ -18,7 +18,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libsvo.so")
 
-SOURCES = ["svo_capi.cpp", "svo_voxelize.hip", "svo_build.hip", "svo_march.hip", "scenegen.cpp"]
+SOURCES = ["svo_capi.cpp", "svo_voxelize.hip", "svo_build.hip", "svo_march.hip", "svo_util.hip", "scenegen.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SVO_OFFLOAD_ARCH", "gfx950")
@@ -64,7 +64,30 @@ def build_library(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_cli(verbose)
     return LIB
+
+
+CLI = os.path.join(LIB_DIR, "svo_cli")
+
+
+def build_cli(verbose=False):
+    """The C++ command-line driver over the class surface (host/svo.hpp), linked to libsvo.so."""
+    tmp = CLI + ".tmp"
+    cmd = [
+        "g++", "-O2", "-std=c++17", "-Wall",
+        "-I" + os.path.join(ROOT, "include"),
+        os.path.join(HERE, "host", "svo_cli.cpp"),
+        "-L" + LIB_DIR, "-lsvo", "-Wl,-rpath,$ORIGIN",
+        "-o", tmp,
+    ]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("svo_cli build failed:\n%s\n%s" % (" ".join(cmd), r.stderr[-6000:]))
+    os.replace(tmp, CLI)
+    if verbose:
+        print("built", CLI)
+    return CLI
 
 
 if __name__ == "__main__":

@@ -10,7 +10,7 @@
 //   kind 2  closed box room [0,1]^3, walls tessellated into (200/d)^2 quads, 400/d randomly rotated
 //           and scaled boxes, 200/d spheres of 46x28 segments (2530 triangles), all inside the
 //           room.  param = d (0 -> 1).
-//   kind 3  param spheres as kind 1 (0 -> 992) plus 4032 * param thin slivers with aspect ratio
+//   kind 3  param spheres as kind 1 (0 -> 992) plus 4032 per sphere thin slivers with aspect ratio
 //           log-uniform in [1,1000], long edge U[0.002,0.02], random orientation, centre U[0,1]^3.
 #include <algorithm>
 #include <cmath>
@@ -96,9 +96,9 @@ void sphere(Out &o, Rng &rng, const double *c, double r, int nlon, int nlat) {
 		}
 }
 
-void spheres_kind1(Out &o, Rng &rng, uint64_t n) {
+void spheres_kind1(Out &o, Rng &rng, uint64_t n, double rscale = 1.0) {
 	for (uint64_t s = 0; s < n; ++s) {
-		double r = rng.uniform(0.02, 0.1);
+		double r = rng.uniform(0.02, 0.1) * rscale;
 		double c[3] = {rng.uniform(), rng.uniform(), rng.uniform()};
 		sphere(o, rng, c, r, 64, 32);
 	}
@@ -180,9 +180,9 @@ void room_kind2(Out &o, Rng &rng, uint64_t d) {
 	}
 }
 
-void slivers(Out &o, Rng &rng, uint64_t n) {
+void slivers(Out &o, Rng &rng, uint64_t n, double lscale = 1.0) {
 	for (uint64_t s = 0; s < n; ++s) {
-		double len = rng.uniform(0.002, 0.02);
+		double len = rng.uniform(0.002, 0.02) * lscale;
 		double aspect = std::exp(rng.uniform(0.0, std::log(1000.0)));
 		double wid = len / aspect;
 		// random unit direction u and a unit vector v perpendicular to it
@@ -223,9 +223,14 @@ extern "C" int svo_gen_scene(int kind, uint64_t seed, uint64_t param, float *tri
 		room_kind2(o, rng, param);
 		break;
 	case 3: {
-		uint64_t n = param ? param : 992;
-		spheres_kind1(o, rng, n);
-		slivers(o, rng, n * 4032);
+		// param: bits 0..31 sphere count (0 -> 992); bits 32..47 sphere radius scale and bits 48..63
+		// sliver length scale, in percent (0 -> 100)
+		uint64_t n = param & 0xFFFFFFFFull;
+		if (n == 0)
+			n = 992;
+		uint64_t rs = (param >> 32) & 0xFFFF, ls = (param >> 48) & 0xFFFF;
+		spheres_kind1(o, rng, n, rs ? rs / 100.0 : 1.0);
+		slivers(o, rng, n * 4032, ls ? ls / 100.0 : 1.0);
 		break;
 	}
 	default:

@@ -357,6 +357,13 @@ int svo_free(svo_ctx *ctx, void *dptr) {
 	return SVO_OK;
 }
 
+int svo_copy_device(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+	if (!ctx || (bytes && (!dst || !src)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	return copy_device(ctx, dst, src, bytes);
+}
+
 int svo_upload(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
 	if (!ctx || (bytes && (!dst || !src)))
 		return SVO_ERR_INVALID_ARGUMENT;
@@ -703,9 +710,9 @@ int svo_build_finish(svo_ctx *ctx, svo_octree *o, uint64_t *W) {
 			ctx->pool->free(nw);
 			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree buffer growth failed");
 		}
-		if ((e = hipMemcpyAsync(nw, o->words, sizeof(uint32_t) * std::min(keep, o->cap), hipMemcpyDeviceToDevice,
-		                        ctx->stream)) != hipSuccess)
-			return hip_check(ctx, e, "octree buffer growth");
+		int crc = copy_device(ctx, nw, o->words, sizeof(uint32_t) * std::min(keep, o->cap));
+		if (crc)
+			return crc;
 		ctx->pool->free(o->words);
 		ctx->pool->free(o->aux);
 		o->words = nw;
@@ -797,6 +804,57 @@ int svo_octree_destroy(svo_octree *o) {
 	o->ctx->pool->free(o->words);
 	delete o;
 	return SVO_OK;
+}
+
+namespace {
+struct OctreeFileHeader {
+	char magic[8];
+	uint32_t level;
+	uint32_t zero;
+	uint64_t W;
+};
+static_assert(sizeof(OctreeFileHeader) == 24, "octree file header is 24 bytes");
+const char kOctreeMagic[8] = {'S', 'V', 'O', 'C', 'T', 'R', 'E', '1'};
+} // namespace
+
+int svo_octree_save(svo_ctx *ctx, const svo_octree *o, const char *path) {
+	if (!ctx || !o || !path || o->pending)
+		return SVO_ERR_INVALID_ARGUMENT;
+	std::vector<uint32_t> words(o->W);
+	int rc = svo_octree_download(ctx, o, words.data(), o->W);
+	if (rc)
+		return rc;
+	FILE *f = std::fopen(path, "wb");
+	if (!f)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+	OctreeFileHeader h{};
+	std::memcpy(h.magic, kOctreeMagic, 8);
+	h.level = o->level;
+	h.W = o->W;
+	bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(words.data(), 4, words.size(), f) == words.size();
+	ok = (std::fclose(f) == 0) && ok;
+	return ok ? SVO_OK : set_error(ctx, SVO_ERR_INVALID_ARGUMENT, std::string("write failed: ") + path);
+}
+
+int svo_octree_load(svo_ctx *ctx, const char *path, svo_octree **out) {
+	if (!ctx || !path || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	FILE *f = std::fopen(path, "rb");
+	if (!f)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+	OctreeFileHeader h{};
+	std::vector<uint32_t> words;
+	bool ok = std::fread(&h, sizeof(h), 1, f) == 1 && std::memcmp(h.magic, kOctreeMagic, 8) == 0 && h.W >= 8 &&
+	          h.W <= (1ull << 30);
+	if (ok) {
+		words.resize(h.W);
+		ok = std::fread(words.data(), 4, h.W, f) == h.W;
+	}
+	std::fclose(f);
+	if (!ok)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, std::string("not an octree file: ") + path);
+	return svo_octree_create(ctx, words.data(), h.W, h.level, out);
 }
 
 // ---- OctreeTracer

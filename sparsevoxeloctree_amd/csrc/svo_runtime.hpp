@@ -180,6 +180,7 @@ int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
                uint32_t *nfetch);
 int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits);
 int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays);
+int copy_device(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 // Device-wide exclusive scan of n (read from the device) uint32 counts; see svo_scan.hpp.
 } // namespace svo

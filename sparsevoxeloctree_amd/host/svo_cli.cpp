@@ -1,0 +1,141 @@
+// svo_cli: the reference's load path (LoaderThread::thread_func, src/LoaderThread.cpp:43-116) on the
+// C++ class surface: Scene -> Voxelizer (count pass) -> timed {CmdVoxelize, CmdBuild} ->
+// Octree::Update, then optionally one primary frame through OctreeTracer.
+//
+//   svo_cli -scene KIND[,SEED[,PARAM]] -lvl N [-frame W H] [-no-beam] [-save FILE]
+//   svo_cli -tris FILE.bin -lvl N ...      (FILE.bin: uint64 T, T*9 float32 xyz, T uint32 rgb)
+//
+// Prints the reference's timing line, "FINISHED in {total} ms (Voxelize {v} ms, Octree building
+// {b} ms)" (src/LoaderThread.cpp:92-97), measured on the device, then the fragment count and the
+// octree's used range.  -save writes the octree with svo_octree_save.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "svo.hpp"
+
+static int usage() {
+	std::fprintf(stderr, "usage: svo_cli (-scene KIND[,SEED[,PARAM]] | -tris FILE) -lvl N [-frame W H] [-no-beam] "
+	                     "[-save FILE]\n");
+	return 2;
+}
+
+int main(int argc, char **argv) {
+	int kind = 0;
+	uint64_t seed = 1, param = 0;
+	std::string tris_path, save_path;
+	uint32_t level = 0, fw = 0, fh = 0;
+	bool beam = true;
+	for (int i = 1; i < argc; ++i) {
+		if (!std::strcmp(argv[i], "-scene") && i + 1 < argc) {
+			unsigned long long s = 1, p = 0;
+			int n = std::sscanf(argv[++i], "%d,%llu,%llu", &kind, &s, &p);
+			if (n < 1)
+				return usage();
+			seed = s, param = p;
+		} else if (!std::strcmp(argv[i], "-tris") && i + 1 < argc) {
+			tris_path = argv[++i];
+		} else if (!std::strcmp(argv[i], "-lvl") && i + 1 < argc) {
+			level = (uint32_t)std::atoi(argv[++i]);
+		} else if (!std::strcmp(argv[i], "-frame") && i + 2 < argc) {
+			fw = (uint32_t)std::atoi(argv[++i]);
+			fh = (uint32_t)std::atoi(argv[++i]);
+		} else if (!std::strcmp(argv[i], "-no-beam")) {
+			beam = false;
+		} else if (!std::strcmp(argv[i], "-save") && i + 1 < argc) {
+			save_path = argv[++i];
+		} else
+			return usage();
+	}
+	if ((kind == 0) == tris_path.empty() || level == 0)
+		return usage();
+
+	std::vector<float> xyz;
+	std::vector<uint32_t> rgb;
+	uint64_t T = 0;
+	if (kind) {
+		if (svo_gen_scene(kind, seed, param, nullptr, nullptr, &T) != SVO_OK)
+			return usage();
+		xyz.resize(9 * T);
+		rgb.resize(T);
+		svo_gen_scene(kind, seed, param, xyz.data(), rgb.data(), &T);
+	} else {
+		FILE *f = std::fopen(tris_path.c_str(), "rb");
+		if (!f || std::fread(&T, 8, 1, f) != 1) {
+			std::fprintf(stderr, "cannot read %s\n", tris_path.c_str());
+			return 1;
+		}
+		xyz.resize(9 * T);
+		rgb.resize(T);
+		if (std::fread(xyz.data(), 4, 9 * T, f) != 9 * T || std::fread(rgb.data(), 4, T, f) != T) {
+			std::fprintf(stderr, "truncated %s\n", tris_path.c_str());
+			return 1;
+		}
+		std::fclose(f);
+	}
+
+	auto ctx = svo::Context::Create(0);
+	if (!ctx) {
+		std::fprintf(stderr, "no HIP device\n");
+		return 1;
+	}
+	auto scene = svo::Scene::Create(ctx, xyz.data(), rgb.data(), T);
+	if (!scene) {
+		std::fprintf(stderr, "scene: %s\n", ctx->error.c_str());
+		return 1;
+	}
+	auto voxelizer = svo::Voxelizer::Create(scene, level);
+	if (!voxelizer) {
+		std::fprintf(stderr, "voxelizer: %s\n", ctx->error.c_str());
+		return 1;
+	}
+	auto builder = svo::OctreeBuilder::Create(voxelizer);
+	// src/LoaderThread.cpp:57-75: the timed span is the fill draw plus all octree passes
+	if (!voxelizer->CmdVoxelize() || !builder->CmdBuild()) {
+		std::fprintf(stderr, "build: %s\n", ctx->error.c_str());
+		return 1;
+	}
+	auto octree = svo::Octree::Create(ctx);
+	octree->Update(builder); // waits for the build (GetOctreeRange)
+	if (octree->GetRange() == 0) {
+		std::fprintf(stderr, "build: %s\n", ctx->error.c_str());
+		return 1;
+	}
+	svo_timing t = ctx->Timing();
+	std::printf("FINISHED in %.3f ms (Voxelize %.3f ms, Octree building %.3f ms)\n", t.vox_fill_ms + t.build_ms,
+	            t.vox_fill_ms, t.build_ms);
+	std::printf("triangles %llu fragments %u octree_words %llu level %u\n", (unsigned long long)T,
+	            voxelizer->GetVoxelFragmentCount(), (unsigned long long)(octree->GetRange() / 4), level);
+
+	if (!save_path.empty()) {
+		if (!ctx->Check(svo_octree_save(ctx->h, octree->Handle(), save_path.c_str()))) {
+			std::fprintf(stderr, "save: %s\n", ctx->error.c_str());
+			return 1;
+		}
+		std::printf("saved %s\n", save_path.c_str());
+	}
+	if (fw && fh) {
+		auto camera = svo::Camera::Create();
+		camera->m_aspect_ratio = (float)fw / (float)fh;
+		camera->m_position[2] = 1.04f; // near the -z face, looking along +z
+		auto tracer = svo::OctreeTracer::Create(octree, camera);
+		tracer->m_beam_enable = beam;
+		if (!tracer->Resize(fw, fh) || !tracer->CmdBeamRenderPass() || !tracer->CmdDrawPipeline()) {
+			std::fprintf(stderr, "tracer: %s\n", ctx->error.c_str());
+			return 1;
+		}
+		std::vector<svo_hit> hits = tracer->DownloadHits();
+		uint64_t nhit = 0, iters = 0;
+		for (const svo_hit &h : hits) {
+			nhit += h.vz_flags >> 31;
+			iters += h.iter;
+		}
+		svo_timing t2 = ctx->Timing();
+		std::printf("frame %ux%u hits %llu mean_iter %.2f beam %.3f ms march %.3f ms\n", fw, fh,
+		            (unsigned long long)nhit, (double)iters / hits.size(), beam ? t2.beam_ms : 0.0, t2.march_ms);
+	}
+	return 0;
+}

@@ -1,0 +1,56 @@
+"""The C++ class surface (host/svo.hpp through the svo_cli driver) and octree files, on the GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "sparsevoxeloctree_amd", "lib", "svo_cli")
+
+
+def test_cli_load_path_matches_oracle(tmp_path):
+    import sparsevoxeloctree_amd as S
+
+    out = tmp_path / "scene.svo"
+    r = subprocess.run(
+        [CLI, "-scene", "1,11,6", "-lvl", "7", "-frame", "160", "90", "-save", str(out)],
+        capture_output=True, text=True, timeout=120,
+    )
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert re.search(r"FINISHED in [0-9.]+ ms \(Voxelize [0-9.]+ ms, Octree building [0-9.]+ ms\)", r.stdout)
+    m = re.search(r"fragments (\d+) octree_words (\d+)", r.stdout)
+    xyz, rgb = S.gen_scene(1, 11, 6)
+    ref_words = oracle.build(oracle.voxelize(oracle.normalize(xyz), rgb, 7), 7)
+    assert int(m.group(2)) == ref_words.shape[0]
+    with S.Context(0) as ctx:
+        o = S.Octree.Load(ctx, out)
+        assert o.GetLevel() == 7
+        assert np.array_equal(o.words(), ref_words)
+    assert re.search(r"frame 160x90 hits \d+", r.stdout)
+
+
+def test_octree_file_round_trip(ctx, tmp_path):
+    import sparsevoxeloctree_amd as S
+    from svo_testutil import random_frags
+
+    fr = random_frags(3, 8, 5000)
+    vox = S.Voxelizer.FromFragments(ctx, fr, 8)
+    b = S.OctreeBuilder.Create(vox)
+    b.CmdBuild()
+    o = b.GetOctree()
+    p = tmp_path / "o.svo"
+    o.save(p)
+    assert os.path.getsize(p) == 24 + 4 * o.W
+    o2 = S.Octree.Load(ctx, p)
+    assert np.array_equal(o2.words(), o.words())
+    bad = tmp_path / "bad.svo"
+    bad.write_bytes(b"not an octree file at all")
+    with pytest.raises(S.SvoError):
+        S.Octree.Load(ctx, bad)
