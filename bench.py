@@ -233,12 +233,13 @@ def main():
         "march_beam": 4 * nfetch_beam + bw * bh * 4,
         "vox_fill": 40 * T + 4 * T + 8 * F,
         "vox_count": 40 * T + 4 * T,
-        # one level pass streams every fragment once (B_build = 8FL + 12W)
-        "build_tag": 8 * F,
-        "build_tag_leaf": 8 * F,
-        # the canonical colour fold: every fragment read once and written once into its leaf bucket,
-        # then read once by the fold
-        "build_scatter": 16 * F,
+        # occupancy marks: every fragment read once
+        "bm_mark": 8 * F,
+        # leaf-block index of every fragment: fragment read, 4-byte bin written
+        "build_tag_leaf": 12 * F,
+        # bin index and fragment read, fragment written into its bin's bucket
+        "build_scatter": 20 * F,
+        # the canonical colour fold: every bucketed fragment read once (leaf blocks written: not counted)
         "build_fold": 8 * F,
     }
     nb = None

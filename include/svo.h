@@ -149,16 +149,13 @@ int svo_fraglist_download(svo_ctx *ctx, const svo_fraglist *frags, uint32_t *out
 int svo_fraglist_destroy(svo_fraglist *frags);
 
 /* ---- OctreeBuilder (OctreeBuilder::Create / CmdBuild / GetOctreeRange, src/OctreeBuilder.cpp:8-214).
- * svo_build_enqueue records every level pass on the stream without a host round trip
- * (CmdBuild); svo_build_finish waits, grows the node buffer if it overflowed and returns the used
- * range (GetOctreeRange: W = (counter + 1) * 8 words); svo_build does both. */
+ * svo_build_enqueue (CmdBuild) runs the structure phase (occupancy of every level), reads back the
+ * exact node count once, allocates exactly W words and enqueues the node phase (pointers, leaf
+ * fold); svo_build_finish waits and returns the used range (GetOctreeRange:
+ * W = (counter + 1) * 8 words); svo_build does both.  More than 2^30 words -> SVO_ERR_OVERFLOW. */
 int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
 int svo_build_finish(svo_ctx *ctx, svo_octree *oct, uint64_t *W);
 int svo_build(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
-/* Initial node-buffer size in words for later builds on this context (kOctreeNodeNumMin /
- * kOctreeNodeNumMax estimate of src/OctreeBuilder.cpp:41-47 when 0, the default).  A buffer that
- * turns out too small is grown by svo_build_finish, so this only trades memory against re-runs. */
-int svo_build_set_initial_capacity(svo_ctx *ctx, uint64_t words);
 /* An octree from host words (any node order), e.g. one read back from the reference. */
 int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t level, svo_octree **out);
 /* Octree::GetLevel / GetRange / GetBuffer (src/Octree.hpp:18-33). */

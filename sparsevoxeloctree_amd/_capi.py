@@ -108,7 +108,6 @@ _SIGS = {
     "svo_build_enqueue": (i32, [P, P, PP]),
     "svo_build_finish": (i32, [P, P, C.POINTER(u64)]),
     "svo_build": (i32, [P, P, PP]),
-    "svo_build_set_initial_capacity": (i32, [P, u64]),
     "svo_octree_create": (i32, [P, P, u64, u32, PP]),
     "svo_octree_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
     "svo_octree_download": (i32, [P, P, P, u64]),

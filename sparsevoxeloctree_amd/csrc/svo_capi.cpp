@@ -203,28 +203,12 @@ struct DeviceGuard { // makes the context's device current for the call
 	}
 };
 
-// Initial node-buffer size: the reference's estimate (src/OctreeBuilder.cpp:41-47,
-// clamp(F * (L / 3), 1e6, 5e8) words) bounded by the most words the level structure allows
-// (8 + 8 * sum_{d<L} min(F, 8^d)); grown on overflow by svo_build_finish.
-uint64_t initial_capacity(uint64_t F, uint32_t L) {
-	uint64_t bound = 8, p = 1;
-	for (uint32_t d = 1; d < L; ++d) {
-		p *= 8;
-		bound += 8 * std::min<uint64_t>(F, p);
-	}
-	uint64_t est = std::max<uint64_t>(1000000ull, F * std::max<uint32_t>(1u, L / 3u));
-	uint64_t cap = std::min(bound, est);
-	cap = std::max<uint64_t>(cap, 8);
-	return std::min<uint64_t>(cap, 1ull << 30);
-}
-
 void release_build_scratch(svo_octree *o) {
 	svo_ctx *ctx = o->ctx;
-	ctx->pool->free(o->aux);
-	ctx->pool->free(o->node);
-	ctx->pool->free(o->bucket);
-	ctx->pool->free(o->big);
-	o->aux = o->node = o->bucket = o->big = nullptr;
+	for (uint32_t **p : {&o->bm, &o->rk, &o->lvl, &o->cnt, &o->aux, &o->node, &o->bucket, &o->big}) {
+		ctx->pool->free(*p);
+		*p = nullptr;
+	}
 }
 
 } // namespace
@@ -276,7 +260,7 @@ int svo_ctx_destroy(svo_ctx *ctx) {
 	if (ctx->stream)
 		(void)hipStreamSynchronize(ctx->stream);
 	ctx->prof.reset();
-	for (StageTimer *t : {&ctx->t_normalize, &ctx->t_count, &ctx->t_fill, &ctx->t_build, &ctx->t_beam, &ctx->t_march,
+	for (StageTimer *t : {&ctx->t_normalize, &ctx->t_count, &ctx->t_fill, &ctx->t_build, &ctx->t_build2, &ctx->t_beam, &ctx->t_march,
 	                      &ctx->t_rays, &ctx->t_bounce})
 		t->destroy();
 	if (ctx->d_scan_parts)
@@ -425,7 +409,7 @@ int svo_last_timing(svo_ctx *ctx, svo_timing *out) {
 	out->normalize_ms = ctx->t_normalize.read();
 	out->vox_count_ms = ctx->t_count.read();
 	out->vox_fill_ms = ctx->t_fill.read();
-	out->build_ms = ctx->t_build.read();
+	out->build_ms = ctx->t_build.read() + ctx->t_build2.read(); // structure + node phases
 	out->beam_ms = ctx->t_beam.read();
 	out->march_ms = ctx->t_march.read();
 	out->rays_ms = ctx->t_rays.read();
@@ -655,22 +639,44 @@ int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *fl, svo_octree **out) {
 	o->ctx = ctx;
 	o->level = fl->level;
 	o->src = fl;
-	o->cap = ctx->build_cap_hint ? std::max<uint64_t>(8, std::min<uint64_t>(ctx->build_cap_hint, 1ull << 30))
-	                             : initial_capacity(fl->F, fl->level);
 	const uint64_t F = fl->F ? fl->F : 1;
-	const uint64_t big_words = 2 + F / 33u;
-	if (ctx->pool->alloc((void **)&o->words, sizeof(uint32_t) * o->cap) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->aux, sizeof(uint32_t) * o->cap) != hipSuccess ||
+	const uint64_t TW = build_bitmap_words(fl->level);
+	if (ctx->pool->alloc((void **)&o->bm, sizeof(uint32_t) * (TW ? TW : 1)) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->rk, sizeof(uint32_t) * (TW + 1)) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->lvl, sizeof(uint32_t) * 16) != hipSuccess ||
 	    ctx->pool->alloc((void **)&o->node, sizeof(uint32_t) * F) != hipSuccess ||
 	    ctx->pool->alloc((void **)&o->bucket, sizeof(uint32_t) * 2 * F) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->big, sizeof(uint32_t) * big_words) != hipSuccess) {
+	    ctx->pool->alloc((void **)&o->big, sizeof(uint32_t) * (1 + build_big_capacity(F))) != hipSuccess) {
 		svo_octree_destroy(o);
 		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	}
-	o->pending = true;
+	// structure phase: bitmaps + ranks decide W exactly; one read-back sizes the node buffer
 	ctx->t_build.start(ctx->stream);
-	int rc = build_enqueue(ctx, o, 1);
+	int rc = build_structure(ctx, o);
 	ctx->t_build.stop(ctx->stream);
+	hipError_t e = hipSuccess;
+	if (!rc && ((e = hipMemcpyAsync(ctx->h_state, ctx->d_state, sizeof(BuildState), hipMemcpyDeviceToHost,
+	                                ctx->stream)) != hipSuccess ||
+	            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
+		rc = hip_check(ctx, e, "octree structure");
+	if (!rc && ctx->h_state->overflow)
+		rc = set_error(ctx, SVO_ERR_OVERFLOW, "octree needs more than 2^30 words (30-bit child pointers)");
+	if (!rc) {
+		const BuildState st = *ctx->h_state;
+		o->W = o->cap = st.need_words; // (counter + 1) * 8, src/OctreeBuilder.cpp:212-214
+		o->nbins = st.num[o->level & 1u]; // leaf blocks
+		const uint64_t nb = o->nbins ? o->nbins : 1;
+		if (ctx->pool->alloc((void **)&o->words, sizeof(uint32_t) * o->W) != hipSuccess ||
+		    ctx->pool->alloc((void **)&o->cnt, sizeof(uint32_t) * nb) != hipSuccess ||
+		    ctx->pool->alloc((void **)&o->aux, sizeof(uint32_t) * nb) != hipSuccess)
+			rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree allocation failed");
+	}
+	if (!rc) {
+		o->pending = true;
+		ctx->t_build2.start(ctx->stream);
+		rc = build_nodes(ctx, o);
+		ctx->t_build2.stop(ctx->stream);
+	}
 	if (rc) {
 		svo_octree_destroy(o);
 		return rc;
@@ -683,56 +689,15 @@ int svo_build_finish(svo_ctx *ctx, svo_octree *o, uint64_t *W) {
 	if (!ctx || !o)
 		return SVO_ERR_INVALID_ARGUMENT;
 	DeviceGuard g(ctx);
-	while (o->pending) {
-		hipError_t e;
-		if ((e = hipMemcpyAsync(ctx->h_state, ctx->d_state, sizeof(BuildState), hipMemcpyDeviceToHost, ctx->stream)) !=
-		        hipSuccess ||
-		    (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+	if (o->pending) {
+		hipError_t e = hipStreamSynchronize(ctx->stream);
+		if (e != hipSuccess)
 			return hip_check(ctx, e, "octree build");
-		const BuildState st = *ctx->h_state;
-		if (st.overflow == 0) {
-			const uint32_t slot = o->level & 1u;
-			o->W = ((uint64_t)st.counter[slot] + 1u) * 8u; // src/OctreeBuilder.cpp:212-214
-			o->pending = false;
-			release_build_scratch(o);
-			break;
-		}
-		// The allocation of level st.overflow did not fit: grow both cap-sized buffers, keep the
-		// words written so far (everything below the next level's range), redo that allocation.
-		const uint64_t keep = st.need_words - 0; // upper bound of the written range
-		uint64_t ncap = std::max<uint64_t>(st.need_words + st.need_words / 2, o->cap * 2);
-		if (st.need_words > (1ull << 30))
-			return set_error(ctx, SVO_ERR_OVERFLOW, "octree needs more than 2^30 words (30-bit child pointers)");
-		ncap = std::min<uint64_t>(ncap, 1ull << 30);
-		uint32_t *nw = nullptr, *na = nullptr;
-		if (ctx->pool->alloc((void **)&nw, sizeof(uint32_t) * ncap) != hipSuccess ||
-		    ctx->pool->alloc((void **)&na, sizeof(uint32_t) * ncap) != hipSuccess) {
-			ctx->pool->free(nw);
-			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree buffer growth failed");
-		}
-		int crc = copy_device(ctx, nw, o->words, sizeof(uint32_t) * std::min(keep, o->cap));
-		if (crc)
-			return crc;
-		ctx->pool->free(o->words);
-		ctx->pool->free(o->aux);
-		o->words = nw;
-		o->aux = na;
-		o->cap = ncap;
-		ctx->t_build.start(ctx->stream);
-		int rc = build_enqueue(ctx, o, st.overflow);
-		ctx->t_build.stop(ctx->stream);
-		if (rc)
-			return rc;
+		o->pending = false;
+		release_build_scratch(o);
 	}
 	if (W)
 		*W = o->W;
-	return SVO_OK;
-}
-
-int svo_build_set_initial_capacity(svo_ctx *ctx, uint64_t words) {
-	if (!ctx)
-		return SVO_ERR_INVALID_ARGUMENT;
-	ctx->build_cap_hint = words;
 	return SVO_OK;
 }
 

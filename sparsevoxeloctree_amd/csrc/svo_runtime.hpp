@@ -16,18 +16,16 @@
 
 namespace svo {
 
-// Device-side build state of one octree build (the reference's build-info + counter buffers,
-// src/OctreeBuilder.cpp:24-39): the current level range, the allocated-block counter, and the
-// overflow report.
-// Level i reads slot (i & 1) and its allocation writes slot ((i + 1) & 1), so no kernel has to
-// advance the range in place (the reference's octree_modify_arg.comp does it with one thread).
+// Device-side build state (the reference's build-info + counter buffers,
+// src/OctreeBuilder.cpp:24-39).  Slot (L & 1) holds the leaf level's range {uAllocBegin, uAllocNum}
+// and the block counter uCounter after the structure phase; need_words = W.
 struct BuildState {
 	uint32_t begin[2];   // uAllocBegin: first word of the level's range
 	uint32_t num[2];     // uAllocNum: words in the level's range
 	uint32_t counter[2]; // uCounter: 8-word blocks allocated below the root block
-	uint32_t overflow;   // level whose allocation did not fit the node buffer, 0 = none
+	uint32_t overflow;   // non-zero when W exceeds 2^30 words
 	uint32_t pad;
-	uint64_t need_words; // words that allocation needed
+	uint64_t need_words; // W
 };
 
 // Error bits raised by kernels into the context's error word.
@@ -106,9 +104,8 @@ struct svo_ctx {
 	svo::BuildState *d_state = nullptr;
 	uint32_t *d_err = nullptr;
 	svo::BuildState *h_state = nullptr; // pinned mirror for read-back
-	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_beam, t_march, t_rays, t_bounce;
+	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_build2, t_beam, t_march, t_rays, t_bounce;
 	int cu_count = 256;
-	uint64_t build_cap_hint = 0; // words; 0 = automatic
 };
 
 struct svo_scene {
@@ -132,15 +129,20 @@ struct svo_fraglist {
 struct svo_octree {
 	svo_ctx *ctx;
 	uint32_t level;
-	uint64_t W;        // used words, valid after finish
+	uint64_t W;        // used words
 	uint64_t cap;      // words allocated
 	uint32_t *words;
 	// build-in-flight resources
 	const svo_fraglist *src;
-	uint32_t *aux;     // cap words: leaf offsets
-	uint32_t *node;    // F words: per-fragment node index
-	uint32_t *bucket;  // 2 * F words: fragments grouped by leaf
-	uint32_t *big;     // big-bucket work list (1 + F words, capped)
+	uint32_t *bm;      // occupancy bitmaps of depths 1..L-1
+	uint32_t *rk;      // exclusive popcount prefix of bm (+ total)
+	uint32_t *lvl;     // first block of every depth
+	uint64_t nbins;    // leaf blocks (nodes of depth L-1)
+	uint32_t *cnt;     // nbins words: fragments per leaf block
+	uint32_t *aux;     // nbins words: bucket offsets per leaf block
+	uint32_t *node;    // F words: leaf block of each fragment
+	uint32_t *bucket;  // 2 * F words: fragments grouped by leaf block
+	uint32_t *big;     // work list of leaf blocks with > 16 fragments (1 + F / 17 + 1 words)
 	bool pending;
 };
 
@@ -172,8 +174,10 @@ inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) /
 int normalize_scene(svo_ctx *ctx, svo_scene *scene);
 int voxelize_count(svo_ctx *ctx, svo_fraglist *fl);
 int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl);
-int build_enqueue(svo_ctx *ctx, svo_octree *oct, uint32_t resume_level);
-int build_finish(svo_ctx *ctx, svo_octree *oct);
+uint64_t build_bitmap_words(uint32_t level);
+uint64_t build_big_capacity(uint64_t F); // entries of the big-bin work list
+int build_structure(svo_ctx *ctx, svo_octree *oct);
+int build_nodes(svo_ctx *ctx, svo_octree *oct);
 int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
                 svo_hit *hits, float *beam_out);
 int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam,

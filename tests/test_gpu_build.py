@@ -84,18 +84,13 @@ def test_build_dense_cube_closed_form(ctx, no_device_errors):
     assert np.array_equal(w, oracle.build(fr, L))
 
 
-def test_build_overflow_resume(ctx, no_device_errors):
-    # a tiny initial node buffer forces the grow-and-resume path at several levels
-    import sparsevoxeloctree_amd as S
-    from sparsevoxeloctree_amd._capi import lib
-
-    fr = random_frags(31, 8, 20000)
-    ctx.check(lib.svo_build_set_initial_capacity(ctx.h, 16))
-    try:
-        w = gpu_build(ctx, fr, 8)
-    finally:
-        ctx.check(lib.svo_build_set_initial_capacity(ctx.h, 0))
-    assert np.array_equal(w, oracle.build(fr, 8))
+@pytest.mark.parametrize("level", [7, 9, 11, 12])
+def test_build_sparse_deep_levels(ctx, no_device_errors, level):
+    # sparse scatter at the deepest levels: Morton prefixes beyond 32 bits at 2^12, 8^(L-1)-bit
+    # occupancy bitmaps, one fragment per leaf block
+    fr = random_frags(31 + level, level, 30000)
+    print("build sparse level=%d" % level, flush=True)
+    assert np.array_equal(gpu_build(ctx, fr, level), oracle.build(fr, level))
 
 
 def test_build_repeatable(ctx, no_device_errors):
