@@ -16,10 +16,11 @@
 //   octree_modify_arg.comp    -> bm_levels: per-depth counts -> first block of every depth, the
 //                                leaf range {begin, num} and W = (counter + 1) * 8
 //   octree_init_node.comp     -> bm_write_blocks writes every block whole (pointers and zeros)
-//   octree_tag_node.comp      -> build_tag_leaf: leaf word of each fragment from the parent's rank;
-//   (level L) + leaf CAS fold    count per leaf, scan, scatter into per-leaf buckets, sort each bucket
-//                                by the 64-bit fragment and fold in that order (the reference's
-//                                order is whatever its atomics produce).
+//   octree_tag_node.comp      -> build_tag_leaf: leaf block ("bin") of each fragment = rank of its
+//   (level L) + leaf CAS fold    depth-(L-1) node; count per bin, scan, scatter into per-bin buckets;
+//                                build_fold{,_mid,_big} sort each bucket by the 64-bit fragment and
+//                                fold the 8 leaves of the block in that order (the reference's order
+//                                is whatever its atomics produce), writing the leaf block whole.
 // The node words are identical to the reference's pass structure run with a serial allocator.
 #include "svo_device.hpp"
 #include "svo_runtime.hpp"
@@ -85,27 +86,34 @@ __global__ void __launch_bounds__(kThreads) bm_zero(uint32_t *B, uint64_t n) {
 		B[i] = 0u;
 }
 
-// Each fragment marks its depth-(L-1) node.  Most marks repeat one already made (every node is hit
-// by several fragments), so a lane skips the atomic when the previous lane marks the same node or
-// when a plain load already shows the bit.
+// Each fragment marks its depth-(L-1) node.  Neighbouring fragments mostly fall into the same
+// bitmap word, so a wave ORs the bits of lanes with equal words together (segmented scan over
+// shuffles) and only the last lane of every run of equal words touches memory: one atomic when a
+// plain load does not already show all of the run's bits.
 __global__ void __launch_bounds__(kThreads)
     bm_mark(const uint2 *__restrict__ frags, uint64_t F, uint32_t L, uint32_t *Blast) {
 	const uint32_t res = 1u << L, lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < F; base += stride) {
 		const uint64_t f = base + threadIdx.x;
-		uint64_t P = ~0ull;
+		uint32_t word = 0xFFFFFFFFu, bits = 0u; // node index < 2^33, so word < 2^28
 		if (f < F) {
 			uint32_t x, y, z;
 			decode_frag(frags[f], res, x, y, z);
-			P = morton(x, y, z) >> 3;
+			const uint64_t P = morton(x, y, z) >> 3;
+			word = (uint32_t)(P >> 5);
+			bits = 1u << (uint32_t)(P & 31u);
 		}
-		const uint64_t prev = __shfl_up(P, 1u, 64);
-		if (P != ~0ull && (lane == 0u || prev != P)) {
-			const uint32_t bit = 1u << (uint32_t)(P & 31u);
-			uint32_t *w = &Blast[P >> 5];
-			if ((*w & bit) == 0u) // a stale cached word only costs a redundant atomic
-				atomicOr(w, bit);
+#pragma unroll
+		for (uint32_t d = 1; d < 64u; d <<= 1) {
+			const uint32_t ow = __shfl_up(word, d, 64), ob = __shfl_up(bits, d, 64);
+			bits |= (lane >= d && ow == word) ? ob : 0u; // any lane with the same word may contribute
+		}
+		const uint32_t next = __shfl_down(word, 1u, 64);
+		if (word != 0xFFFFFFFFu && (lane == 63u || next != word)) {
+			uint32_t *w = &Blast[word];
+			if ((*w & bits) != bits) // a stale cached word only costs a redundant atomic
+				atomicOr(w, bits);
 		}
 	}
 }
@@ -204,11 +212,11 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Leaf block ("bin") of every fragment: the rank of its depth-(L-1) node.  Fragments are counted
-// per bin (one atomic per run of equal neighbours) and binned, so the fold handles a block's 8
-// leaves together and no leaf word needs zeroing first.
+// per bin with one atomic per run of equal neighbours; the atomic's old value gives every fragment
+// its place inside the bin, so node[f] = {bin, place} and the scatter needs no atomics.
 __global__ void __launch_bounds__(kThreads)
     build_tag_leaf(const uint2 *__restrict__ frags, uint64_t F, const uint32_t *__restrict__ B,
-                   const uint32_t *__restrict__ R, Levels lv, uint32_t *__restrict__ node, uint32_t *cnt) {
+                   const uint32_t *__restrict__ R, Levels lv, uint2 *__restrict__ node, uint32_t *cnt) {
 	const uint32_t L = lv.L, res = 1u << L, lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < F; base += stride) {
@@ -218,41 +226,32 @@ __global__ void __launch_bounds__(kThreads)
 			uint32_t x, y, z;
 			decode_frag(frags[f], res, x, y, z);
 			bin = L == 1u ? 0u : node_rank(B, R, lv.off[L - 1], morton(x, y, z) >> 3);
-			node[f] = bin;
 		}
 		const uint32_t prev = __shfl_up(bin, 1u, 64);
 		const bool head = (lane == 0u) || prev != bin;
 		const uint64_t heads = __ballot(head);
 		const uint64_t after = (lane == 63u) ? 0ull : (heads >> (lane + 1u));
 		const uint32_t run = after ? (uint32_t)__ffsll((unsigned long long)after) : 64u - lane;
+		uint32_t pos = 0u;
 		if (head && bin != 0xFFFFFFFFu)
-			atomicAdd(&cnt[bin], run);
-	}
-}
-
-// Scatter every fragment into its bin's bucket (aux[j] holds the running end of bucket j).
-__global__ void __launch_bounds__(kThreads)
-    build_scatter(const uint2 *__restrict__ frags, uint64_t F, const uint32_t *__restrict__ node, uint32_t *aux,
-                  uint2 *__restrict__ bucket) {
-	const uint32_t lane = threadIdx.x & 63u;
-	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < F; base += stride) {
-		const uint64_t f = base + threadIdx.x;
-		const uint32_t j = f < F ? node[f] : 0xFFFFFFFFu;
-		const uint32_t prev = __shfl_up(j, 1u, 64);
-		const bool head = (lane == 0u) || prev != j;
-		const uint64_t heads = __ballot(head);
-		const uint64_t after = (lane == 63u) ? 0ull : (heads >> (lane + 1u));
-		const uint32_t run = after ? (uint32_t)__ffsll((unsigned long long)after) : 64u - lane;
-		uint32_t pos = 0;
-		if (head && j != 0xFFFFFFFFu)
-			pos = atomicAdd(&aux[j], run);
+			pos = atomicAdd(&cnt[bin], run);
 		// lane of this lane's run head = highest head at or below this lane
 		const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
 		const uint32_t head_lane = 63u - (uint32_t)__clzll((long long)upto);
 		pos = __shfl(pos, (int)head_lane, 64) + (lane - head_lane);
-		if (j != 0xFFFFFFFFu)
-			bucket[pos] = frags[f];
+		if (f < F)
+			node[f] = make_uint2(bin, pos);
+	}
+}
+
+// Scatter every fragment to its place in its bin's bucket (aux[j] = first slot of bin j).
+__global__ void __launch_bounds__(kThreads)
+    build_scatter(const uint2 *__restrict__ frags, uint64_t F, const uint2 *__restrict__ node,
+                  const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x; f < F; f += stride) {
+		const uint2 np = node[f];
+		bucket[aux[np.x] + np.y] = frags[f];
 	}
 }
 
@@ -312,14 +311,24 @@ __device__ __forceinline__ void store_block(uint32_t *oct, uint64_t block, const
 	dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-constexpr uint32_t kRegBucket = 16; // bins with more fragments go to build_fold_big
+constexpr uint32_t kRegBucket = 16; // bins up to this size: one thread each (build_fold)
+constexpr uint32_t kWaveBucket = 64; // up to this size: one wave each (build_fold_mid)
+
+// Work lists of bins too large for build_fold: list[0] = mid count, list[1] = big count, mid bins
+// at list[2 ..], big bins at list[2 + cap ..] (each list holds at most F / 17 bins).
+__device__ __forceinline__ void push_bin(uint32_t *list, uint64_t cap, uint32_t j, uint32_t n) {
+	const uint32_t which = n > kWaveBucket ? 1u : 0u;
+	const uint32_t q = atomicAdd(&list[which], 1u);
+	if (q < cap)
+		list[2u + which * cap + q] = j;
+}
 
 // One thread per leaf block: its fragments (<= 16) are sorted by the 64-bit fragment in registers
 // (19- or 63-comparator networks, +inf padding), then folded in that order into the 8 leaves; the
 // whole block is written, empty leaves as 0.
 __global__ void __launch_bounds__(kThreads)
     build_fold(uint32_t *oct, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux,
-               const uint2 *__restrict__ bucket, uint32_t *big, uint64_t big_cap, const BuildState *st, uint32_t L) {
+               const uint2 *__restrict__ bucket, uint32_t *list, uint64_t cap, const BuildState *st, uint32_t L) {
 	const uint32_t slot = L & 1u, res = 1u << L;
 	const uint64_t first = st->begin[slot] / 8u;
 	const uint32_t nbins = st->num[slot];
@@ -327,12 +336,10 @@ __global__ void __launch_bounds__(kThreads)
 	for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < nbins; j += stride) {
 		const uint32_t n = cnt[j];
 		if (n > kRegBucket) {
-			const uint32_t q = atomicAdd(&big[0], 1u);
-			if (q < big_cap)
-				big[1 + q] = j;
+			push_bin(list, cap, j, n);
 			continue;
 		}
-		const uint2 *b = bucket + (aux[j] - n);
+		const uint2 *b = bucket + aux[j];
 		uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}, c[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 		uint64_t k[16];
 #pragma unroll
@@ -400,22 +407,69 @@ template <class Get, class Put> __device__ void block_bitonic(uint32_t n, uint32
 	}
 }
 
-// Bins with more than 16 fragments, one block each: bitonic sort of the keys (in LDS up to 4096,
-// in place in the bucket beyond), then 8 lanes fold the 8 leaves, each lane walking the sorted keys
+// Bins of 17..64 fragments, one wave each: lane i holds key i (+inf padding), a bitonic sort over
+// lane shuffles orders them, the sorted keys go through LDS, and lane s < 8 folds the keys of leaf
+// slot s in ascending order.
+__global__ void __launch_bounds__(kThreads)
+    build_fold_mid(uint32_t *oct, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux,
+                   const uint2 *__restrict__ bucket, const uint32_t *list, uint64_t cap, const BuildState *st,
+                   uint32_t L) {
+	constexpr uint32_t kWaves = kThreads / 64;
+	__shared__ uint64_t s_keys[kWaves][64];
+	const uint32_t res = 1u << L, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const uint64_t first = st->begin[L & 1u] / 8u;
+	uint64_t count = list[0];
+	if (count > cap)
+		count = cap;
+	for (uint64_t base = (uint64_t)blockIdx.x * kWaves; base < count; base += (uint64_t)gridDim.x * kWaves) {
+		const uint64_t item = base + wave;
+		uint32_t j = 0u, n = 0u;
+		if (item < count) {
+			j = list[2u + item];
+			n = cnt[j];
+		}
+		uint64_t key = lane < n ? frag_key(bucket[aux[j] + lane]) : ~0ull;
+#pragma unroll
+		for (uint32_t k = 2; k <= 64u; k <<= 1) {
+#pragma unroll
+			for (uint32_t d = k >> 1; d > 0u; d >>= 1) {
+				const uint64_t other = __shfl_xor(key, (int)d, 64);
+				const bool up = (lane & k) == 0u, lower = (lane & d) == 0u;
+				const bool take_min = up == lower;
+				key = take_min ? (other < key ? other : key) : (other < key ? key : other);
+			}
+		}
+		s_keys[wave][lane] = key;
+		__syncthreads();
+		if (item < count && lane < 8u) {
+			uint32_t word = 0u, a = 0u;
+			for (uint32_t i = 0; i < n; ++i) {
+				const uint64_t kk = s_keys[wave][i];
+				if (key_slot(kk, res) == lane)
+					word = fold_rt(word, (uint32_t)kk & 0xffffffu, a++);
+			}
+			oct[8u * (first + j) + lane] = word;
+		}
+		__syncthreads();
+	}
+}
+
+// Bins of more than 64 fragments, one block each: bitonic sort of the keys (in LDS up to 4096, in
+// place in the bucket beyond), then 8 lanes fold the 8 leaves, each lane walking the sorted keys
 // and taking those of its slot.
 __global__ void __launch_bounds__(kThreads)
     build_fold_big(uint32_t *oct, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, uint2 *bucket,
-                   const uint32_t *big, uint64_t big_cap, const BuildState *st, uint32_t L) {
+                   const uint32_t *list, uint64_t cap, const BuildState *st, uint32_t L) {
 	__shared__ uint64_t s_keys[kLdsBucket];
 	const uint32_t res = 1u << L;
 	const uint64_t first = st->begin[L & 1u] / 8u;
-	uint64_t count = big[0];
-	if (count > big_cap)
-		count = big_cap;
+	uint64_t count = list[1];
+	if (count > cap)
+		count = cap;
 	for (uint64_t item = blockIdx.x; item < count; item += gridDim.x) {
-		const uint32_t j = big[1 + item];
+		const uint32_t j = list[2u + cap + item];
 		const uint32_t n = cnt[j];
-		uint2 *b = bucket + (aux[j] - n);
+		uint2 *b = bucket + aux[j];
 		uint32_t m = 1;
 		while (m < n)
 			m <<= 1;
@@ -444,7 +498,10 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
-__global__ void build_init_big(uint32_t *big) { big[0] = 0u; }
+__global__ void build_init_lists(uint32_t *list) {
+	list[0] = 0u;
+	list[1] = 0u;
+}
 
 // ---- scan plumbing for the bin offsets ---------------------------------------------------------
 struct BinCounts {
@@ -550,7 +607,8 @@ int build_nodes(svo_ctx *ctx, svo_octree *oct) {
 		return hip_check(ctx, e, "bin_zero");
 	const uint32_t gF = grid_for(F, ctx);
 	if (F > 0 && (e = launch(ctx, "build_tag_leaf", build_tag_leaf, dim3(gF), dim3(kThreads), 0, frags, F,
-	                         (const uint32_t *)oct->bm, (const uint32_t *)oct->rk, lv, oct->node, oct->cnt)) !=
+	                         (const uint32_t *)oct->bm, (const uint32_t *)oct->rk, lv, reinterpret_cast<uint2 *>(oct->node),
+	                         oct->cnt)) !=
 	                 hipSuccess)
 		return hip_check(ctx, e, "build_tag_leaf");
 	BinCounts bc{oct->cnt, nbins};
@@ -564,19 +622,24 @@ int build_nodes(svo_ctx *ctx, svo_octree *oct) {
 	                BinOffsets{oct->aux}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 		return hip_check(ctx, e, "bin_offsets");
 	if (F > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(gF), dim3(kThreads), 0, frags, F,
-	                         (const uint32_t *)oct->node, oct->aux, reinterpret_cast<uint2 *>(oct->bucket))) != hipSuccess)
+	                         (const uint2 *)oct->node, (const uint32_t *)oct->aux,
+	                         reinterpret_cast<uint2 *>(oct->bucket))) != hipSuccess)
 		return hip_check(ctx, e, "build_scatter");
-	if ((e = launch(ctx, "big_reset", build_init_big, dim3(1), dim3(1), 0, oct->big)) != hipSuccess)
-		return hip_check(ctx, e, "big_reset");
-	const uint64_t big_cap = build_big_capacity(F);
+	if ((e = launch(ctx, "list_reset", build_init_lists, dim3(1), dim3(1), 0, oct->big)) != hipSuccess)
+		return hip_check(ctx, e, "list_reset");
+	const uint64_t cap = build_big_capacity(F);
+	const BuildState *st = ctx->d_state;
 	if ((e = launch(ctx, "build_fold", build_fold, dim3(grid_for(nbins, ctx)), dim3(kThreads), 0, oct->words,
-	                (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux, (const uint2 *)oct->bucket, oct->big,
-	                big_cap, (const BuildState *)ctx->d_state, L)) != hipSuccess)
+	                (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux, (const uint2 *)oct->bucket, oct->big, cap,
+	                st, L)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold");
+	if ((e = launch(ctx, "build_fold_mid", build_fold_mid, dim3((uint32_t)ctx->cu_count * 8u), dim3(kThreads), 0,
+	                oct->words, (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux, (const uint2 *)oct->bucket,
+	                (const uint32_t *)oct->big, cap, st, L)) != hipSuccess)
+		return hip_check(ctx, e, "build_fold_mid");
 	if ((e = launch(ctx, "build_fold_big", build_fold_big, dim3((uint32_t)ctx->cu_count * 2u), dim3(kThreads), 0,
 	                oct->words, (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux,
-	                reinterpret_cast<uint2 *>(oct->bucket), (const uint32_t *)oct->big, big_cap,
-	                (const BuildState *)ctx->d_state, L)) != hipSuccess)
+	                reinterpret_cast<uint2 *>(oct->bucket), (const uint32_t *)oct->big, cap, st, L)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold_big");
 	return SVO_OK;
 }

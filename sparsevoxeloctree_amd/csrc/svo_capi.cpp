@@ -644,9 +644,9 @@ int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *fl, svo_octree **out) {
 	if (ctx->pool->alloc((void **)&o->bm, sizeof(uint32_t) * (TW ? TW : 1)) != hipSuccess ||
 	    ctx->pool->alloc((void **)&o->rk, sizeof(uint32_t) * (TW + 1)) != hipSuccess ||
 	    ctx->pool->alloc((void **)&o->lvl, sizeof(uint32_t) * 16) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->node, sizeof(uint32_t) * F) != hipSuccess ||
+	    ctx->pool->alloc((void **)&o->node, sizeof(uint32_t) * 2 * F) != hipSuccess ||
 	    ctx->pool->alloc((void **)&o->bucket, sizeof(uint32_t) * 2 * F) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->big, sizeof(uint32_t) * (1 + build_big_capacity(F))) != hipSuccess) {
+	    ctx->pool->alloc((void **)&o->big, sizeof(uint32_t) * (2 + 2 * build_big_capacity(F))) != hipSuccess) {
 		svo_octree_destroy(o);
 		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	}

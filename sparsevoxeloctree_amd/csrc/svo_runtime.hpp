@@ -140,9 +140,9 @@ struct svo_octree {
 	uint64_t nbins;    // leaf blocks (nodes of depth L-1)
 	uint32_t *cnt;     // nbins words: fragments per leaf block
 	uint32_t *aux;     // nbins words: bucket offsets per leaf block
-	uint32_t *node;    // F words: leaf block of each fragment
+	uint32_t *node;    // 2 * F words: {leaf block, place in its bucket} of each fragment
 	uint32_t *bucket;  // 2 * F words: fragments grouped by leaf block
-	uint32_t *big;     // work list of leaf blocks with > 16 fragments (1 + F / 17 + 1 words)
+	uint32_t *big;     // work lists of leaf blocks with 17..64 and > 64 fragments (2 + 2 * (F / 17 + 1) words)
 	bool pending;
 };
 
@@ -175,7 +175,7 @@ int normalize_scene(svo_ctx *ctx, svo_scene *scene);
 int voxelize_count(svo_ctx *ctx, svo_fraglist *fl);
 int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl);
 uint64_t build_bitmap_words(uint32_t level);
-uint64_t build_big_capacity(uint64_t F); // entries of the big-bin work list
+uint64_t build_big_capacity(uint64_t F); // entries of each large-bin work list
 int build_structure(svo_ctx *ctx, svo_octree *oct);
 int build_nodes(svo_ctx *ctx, svo_octree *oct);
 int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,

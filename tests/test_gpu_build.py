@@ -99,3 +99,27 @@ def test_build_repeatable(ctx, no_device_errors):
     w2 = gpu_build(ctx, fr[::-1].copy(), 9)  # input order must not matter
     assert np.array_equal(w1, w2)
     assert np.array_equal(w1, oracle.build(fr, 9))
+
+
+def test_build_bin_size_classes(ctx, no_device_errors):
+    # leaf blocks holding 1..16 (thread sorter), 17..64 (wave sorter), 65..300 (block sorter)
+    # fragments, spread over the 8 leaves with repeated coordinates and repeated full fragments
+    L = 7
+    rng = np.random.default_rng(77)
+    nodes = rng.choice(1 << (3 * (L - 1)), size=240, replace=False)
+    parts = []
+    for i, P in enumerate(nodes):
+        n = [rng.integers(1, 17), rng.integers(17, 65), rng.integers(65, 301)][i % 3]
+        X = Y = Z = 0
+        for b in range(L - 1):  # de-interleave the Morton index of the depth-(L-1) node
+            X |= ((int(P) >> (3 * b)) & 1) << b
+            Y |= ((int(P) >> (3 * b + 1)) & 1) << b
+            Z |= ((int(P) >> (3 * b + 2)) & 1) << b
+        s = rng.integers(0, 8, n)
+        col = rng.integers(0, 1 << 24, n)
+        col[: n // 4] = col[0]  # ties on the full 64-bit fragment
+        parts.append(pack_frags(2 * X + (s & 1), 2 * Y + ((s >> 1) & 1), 2 * Z + (s >> 2), col))
+    fr = np.concatenate(parts)
+    fr = np.ascontiguousarray(fr[rng.permutation(fr.shape[0])])
+    print("build bin classes F=%d" % fr.shape[0], flush=True)
+    assert np.array_equal(gpu_build(ctx, fr, L), oracle.build(fr, L))
