@@ -49,6 +49,12 @@ CONFIGS = {
         kind=3, seed=0x5EED0003, param=992 | (25 << 32) | (25 << 48), level=12, width=3840, height=2160,
         bounce=False, camera="orbit",
     ),
+    # many small and large scenes: batched launches against one scene per launch
+    4: dict(
+        workload="c4: 256 scenes of UV spheres (T log-U[1K,2M], level U{6..11}), voxelize + SVO build + "
+        "256x256 primary rays each; batched launches vs one scene per launch",
+        kind=4, seed=0x5EED0004, scenes=256, width=256, height=256,
+    ),
 }
 
 
@@ -138,6 +144,170 @@ def cpu_baseline(S, cfg, cam, words, budget_s=20.0):
     }
 
 
+def c4_scenes(cfg):
+    """Triangle budgets and levels of the batch: T log-uniform in [1e3, 2e6], level uniform in 6..11."""
+    rng = np.random.default_rng(cfg["seed"])
+    n = cfg["scenes"]
+    T = np.exp(rng.uniform(math.log(1e3), math.log(2e6), n)).astype(np.int64)
+    levels = rng.integers(6, 12, n)
+    return [int(t) for t in T], [int(x) for x in levels]
+
+
+def orbit_camera(S, xyz, seed, aspect):
+    """Eye on a sphere of radius 1.5 about the world point (0.5,0.5,0.5), mapped to octree space by the
+    scene normalisation (src/Scene.cpp:90-99) and v -> 1.5 + 0.5 v, looking at that point."""
+    p = xyz.reshape(-1, 3).astype(np.float64)
+    if p.shape[0] == 0:
+        return S.camera_look_at((1.5, 1.5, 3.0), (1.5, 1.5, 1.5), math.pi / 3.0, aspect)
+    mn, mx = p.min(axis=0), p.max(axis=0)
+    centre, ext = (mn + mx) * 0.5, max((mx - mn).max() * 0.5, 1e-30)
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=3)
+    v /= np.linalg.norm(v)
+    to_oct = lambda w: 1.5 + 0.5 * (w - centre) / ext  # noqa: E731
+    eye_w = np.array([0.5, 0.5, 0.5]) + 1.5 * v
+    return S.camera_look_at(tuple(to_oct(eye_w)), tuple(to_oct(np.array([0.5, 0.5, 0.5]))), math.pi / 3.0, aspect)
+
+
+def bench_batch(args, cfg):
+    """c4: the same 256 scenes voxelized, built and ray-marched (a) one scene per launch through the
+    single-scene API and (b) with the batched API (one launch per pass for all scenes, autotuned march
+    shape, march launches replayed from a HIP graph).  A step is one pass of (b) over all scenes."""
+    import sparsevoxeloctree_amd as S
+
+    Ts, levels = c4_scenes(cfg)
+    n, W, H = cfg["scenes"], cfg["width"], cfg["height"]
+    ctx = S.Context(0)
+    scenes, cams = [], []
+    for i in range(n):
+        xyz, rgb = S.gen_scene(4, cfg["seed"] + (i << 16), Ts[i])
+        scenes.append(S.Scene.Create(ctx, xyz, rgb, normalize=True))
+        cams.append(orbit_camera(S, xyz, cfg["seed"] + i, W / H))
+    ctx.synchronize()
+    Ttot = sum(sc.GetTriangleCount() for sc in scenes)
+    R = n * W * H
+    hits = ctx.alloc(R * 32)
+
+    def one_by_one():
+        octs, keep = [], []
+        t0 = time.perf_counter()
+        for sc, lv in zip(scenes, levels):
+            vox = S.Voxelizer.Create(sc, lv)
+            vox.CmdVoxelize()
+            b = S.OctreeBuilder.Create(vox)
+            b.CmdBuild()
+            b.GetOctreeRange()
+            keep.append((vox, b))
+            octs.append(b.GetOctree())
+        ctx.synchronize()
+        t_build = time.perf_counter() - t0
+        import ctypes
+
+        from sparsevoxeloctree_amd._capi import lib
+
+        beam = ctx.alloc(((W + 7) // 8 + 1) * ((H + 7) // 8 + 1) * 4)
+        ucams = [c.uniform() for c in cams]
+        t0 = time.perf_counter()
+        for i, (o, c) in enumerate(zip(octs, ucams)):
+            frame = ctypes.c_void_p(hits.ptr.value + i * W * H * 32)  # frames one after another
+            ctx.check(lib.svo_march(ctx.h, o.h, ctypes.byref(c), W, H, 1, frame, beam.ptr), "march")
+        ctx.synchronize()
+        t_march = time.perf_counter() - t0
+        return t_build, t_march, keep
+
+    def batched(plan_cache):
+        t0 = time.perf_counter()
+        voxs = S.voxelize_batch(scenes, levels)
+        ctx.synchronize()
+        t_vox = time.perf_counter() - t0
+        octs = S.build_batch(voxs, wait=False)
+        ctx.synchronize()
+        t_build = time.perf_counter() - t0
+        split.append((t_vox, t_build - t_vox, sum(v.GetVoxelFragmentCount() for v in voxs)))
+        plan = S.MarchPlan(octs, cams, W, H, beam=True)
+        plan.use_graph(True)
+        plan.run(hits)  # autotune (first time) + graph capture
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        reps = 5
+        for _ in range(reps):
+            plan.run(hits)
+        ctx.synchronize()
+        t_march = (time.perf_counter() - t0) / reps
+        plan_cache.append((plan, voxs, octs))
+        return t_build, t_march, octs
+
+    split = []
+    S.autotune(ctx, True)
+    for _ in range(max(1, args.warmup)):
+        keep = []
+        batched(keep)
+        del keep
+    seq_build, seq_march, keep_seq = one_by_one()
+    seq_hits = hits.download(S.HIT_DTYPE, R)
+    del keep_seq
+    tb, tm = [], []
+    keep = []
+    split.clear()
+    for _ in range(args.steps):
+        keep.clear()
+        b, m, octs = batched(keep)
+        tb.append(b)
+        tm.append(m)
+    # per-kernel device times of one more batched pass
+    keep.clear()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    batched(keep)
+    ctx.synchronize()
+    ctx.prof_enable(False)
+    prof = ctx.prof_stats()
+    bat_hits = hits.download(S.HIT_DTYPE, R)
+    same = bool(np.array_equal(seq_hits.view(np.uint32), bat_hits.view(np.uint32)))
+    assert ctx.device_errors() == 0
+    report = S.autotune(ctx, True)
+    Wtot = sum(o.W for o in octs) if octs and octs[0].W else None
+    t_build, t_march = float(np.median(tb)), float(np.median(tm))
+    line = {
+        "metric": METRIC,
+        "value": round(R / t_march / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round((t_build + t_march) * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, inputs resident in HBM)",
+        "config": {"workload": cfg["workload"], "scenes": n, "triangles": Ttot, "levels": levels,
+                   "width": W, "height": H, "beam": True},
+        "batched_builds_per_s": round(n / t_build, 2),
+        "batched_build_ms": round(t_build * 1e3, 3),
+        "batched_march_ms": round(t_march * 1e3, 3),
+        "sequential_builds_per_s": round(n / seq_build, 2),
+        "sequential_build_ms": round(seq_build * 1e3, 3),
+        "sequential_march_ms": round(seq_march * 1e3, 3),
+        "sequential_mrays_per_s": round(R / seq_march / 1e6, 3),
+        "speedup_build": round(seq_build / t_build, 3),
+        "speedup_march": round(seq_march / t_march, 3),
+        "speedup_total": round((seq_build + seq_march) / (t_build + t_march), 3),
+        "hits_identical_to_sequential": same,
+        "fragments": split[-1][2],
+        "batched_voxelize_ms": round(float(np.median([x[0] for x in split])) * 1e3, 3),
+        "batched_octree_ms": round(float(np.median([x[1] for x in split])) * 1e3, 3),
+        "kernel_ms_total": {k: round(v[1], 4) for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])},
+        "autotune": report.strip().splitlines(),
+        "peak_pool_bytes": ctx.memory()["peak_in_use"],
+        "cpu_baseline": None,
+        "roofline": None,
+    }
+    print(json.dumps(line))
+    sys.stdout.flush()
+    ctx.close()
+
+
 def copy_calibration(mib):
     import sparsevoxeloctree_amd as S
     from sparsevoxeloctree_amd._capi import lib
@@ -181,6 +351,9 @@ def main():
     import sparsevoxeloctree_amd as S
 
     cfg = CONFIGS[args.config]
+    if args.config == 4:
+        bench_batch(args, cfg)
+        return
     xyz, rgb = S.gen_scene(cfg["kind"], cfg["seed"], cfg["param"])
     T = rgb.shape[0]
     cam = make_camera(S, cfg, xyz)

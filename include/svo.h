@@ -141,6 +141,12 @@ int svo_scene_destroy(svo_scene *scene);
 int svo_voxelize_count(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out);
 int svo_voxelize_fill(svo_ctx *ctx, svo_fraglist *frags);
 int svo_voxelize(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out);
+/* n scenes at once (scene i at levels[i]): one count launch, one read-back of every scene's
+ * fragment count, one fill launch; out[i] receives scene i's list, identical to svo_voxelize's.
+ * The reference voxelizes one scene per Voxelizer::Create + CmdVoxelize
+ * (src/Voxelizer.cpp:5-26,166-179); this batched form is new.  n <= 1024. */
+int svo_voxelize_batch(svo_ctx *ctx, const svo_scene *const *scenes, const uint32_t *levels, uint32_t n,
+                       svo_fraglist **out);
 /* A fragment list from the host (e.g. one read back from the reference) for svo_build. */
 int svo_fraglist_create(svo_ctx *ctx, const uint32_t *frags, uint64_t F, uint32_t level, svo_fraglist **out);
 /* Voxelizer::GetLevel / GetVoxelFragmentCount / GetVoxelFragmentList (src/Voxelizer.hpp:45-52). */
@@ -156,6 +162,12 @@ int svo_fraglist_destroy(svo_fraglist *frags);
 int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
 int svo_build_finish(svo_ctx *ctx, svo_octree *oct, uint64_t *W);
 int svo_build(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
+/* n octrees at once: one launch per build pass for all of them and one read-back of every
+ * octree's size; out[i] is identical to svo_build(frags[i]) and is finished by svo_build_finish
+ * or svo_ctx_synchronize.  The reference builds one octree per OctreeBuilder::CmdBuild
+ * (src/OctreeBuilder.cpp:142-210); this batched form is new.  n <= 1024, at most 2^32-2
+ * fragments in total. */
+int svo_build_batch(svo_ctx *ctx, const svo_fraglist *const *frags, uint32_t n, svo_octree **out);
 /* An octree from host words (any node order), e.g. one read back from the reference. */
 int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t level, svo_octree **out);
 /* Octree::GetLevel / GetRange / GetBuffer (src/Octree.hpp:18-33). */
@@ -184,6 +196,35 @@ int svo_march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, u
  * floats in octree space; variant = SVO_MARCH_*. */
 int svo_march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays_dev, uint64_t R, int variant,
                    svo_hit *hits_dev);
+/* Batched tracing (new; the reference traces one octree per OctreeTracer::CmdDrawPipeline,
+ * src/OctreeTracer.cpp:226-246).  svo_march_batch: primary rays of n octrees, camera cams[i], a
+ * width x height frame each, in one beam launch and one march launch; hits_dev holds n frames of
+ * width*height records, octree i's at i * width * height, each identical to svo_march's.  */
+int svo_march_batch(svo_ctx *ctx, const svo_octree *const *octs, const svo_camera *cams, uint32_t n,
+                    uint32_t width, uint32_t height, int beam, svo_hit *hits_dev);
+/* Reusable form: the octree and camera table and the beam scratch stay on the device; cameras may
+ * change between runs; with svo_march_plan_use_graph(plan, 1) the launches are captured once as a
+ * HIP graph and replayed by every later run into the same hit buffer. */
+typedef struct svo_march_plan svo_march_plan;
+int svo_march_plan_create(svo_ctx *ctx, const svo_octree *const *octs, const svo_camera *cams, uint32_t n,
+                          uint32_t width, uint32_t height, int beam, svo_march_plan **out);
+int svo_march_plan_set_cameras(svo_ctx *ctx, svo_march_plan *plan, const svo_camera *cams);
+int svo_march_plan_use_graph(svo_march_plan *plan, int on);
+/* Workgroup tile of the march launch: 0 = 16x16 pixels (4 waves), 1 = 8x8 (1 wave), 2 = 16x8,
+ * 3 = 32x8; -1 = the default, or the autotuner's choice when svo_autotune is on. */
+int svo_march_plan_set_shape(svo_march_plan *plan, int shape);
+int svo_march_plan_run(svo_ctx *ctx, svo_march_plan *plan, svo_hit *hits_dev);
+int svo_march_plan_destroy(svo_march_plan *plan);
+/* Explicit rays against n octrees in one launch: rays [ray_begin[i], ray_begin[i+1]) of rays_dev go
+ * to octs[i] (ray_begin: host array of n + 1 non-decreasing entries, ray_begin[0] = 0). */
+int svo_march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t n, const uint64_t *ray_begin,
+                         const float *rays_dev, int variant, svo_hit *hits_dev);
+/* Launch-shape autotuner for the batched marches (off by default).  When on, the first launch of a
+ * new shape class (march batch: n, width, height, beam; ray batch: variant, log2 R) times every
+ * candidate workgroup/tile shape once and keeps the fastest; the report lists the decisions. */
+int svo_autotune(svo_ctx *ctx, int on);
+int svo_autotune_report(svo_ctx *ctx, char *buf, uint64_t cap, uint64_t *len);
+
 /* One cosine-weighted diffuse bounce per hit (Diffuse_Sample + AlignDirection,
  * shader/diffuse.glsl:7-14, shader/util.glsl:5-9) with a seeded, XOR-scrambled 2-D Sobol sample
  * per ray index.  rays_dev = R x 6 floats; a ray whose hit bit is clear gets o = 0, d = -1. */

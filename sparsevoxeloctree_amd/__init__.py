@@ -10,14 +10,20 @@ from .api import (  # noqa: F401
     Camera,
     Context,
     DeviceArray,
+    MarchPlan,
     Octree,
     OctreeBuilder,
     OctreeTracer,
     Scene,
     SvoError,
     Voxelizer,
+    autotune,
+    build_batch,
     camera_look_at,
     gen_scene,
+    march_batch,
     march_rays,
+    march_rays_batch,
+    voxelize_batch,
 )
 from ._capi import LIB_PATH, exported_symbols  # noqa: F401

@@ -101,6 +101,7 @@ _SIGS = {
     "svo_voxelize_count": (i32, [P, P, u32, PP]),
     "svo_voxelize_fill": (i32, [P, P]),
     "svo_voxelize": (i32, [P, P, u32, PP]),
+    "svo_voxelize_batch": (i32, [P, PP, C.POINTER(u32), u32, PP]),
     "svo_fraglist_create": (i32, [P, P, u64, u32, PP]),
     "svo_fraglist_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
     "svo_fraglist_download": (i32, [P, P, P, u64]),
@@ -108,6 +109,7 @@ _SIGS = {
     "svo_build_enqueue": (i32, [P, P, PP]),
     "svo_build_finish": (i32, [P, P, C.POINTER(u64)]),
     "svo_build": (i32, [P, P, PP]),
+    "svo_build_batch": (i32, [P, PP, u32, PP]),
     "svo_octree_create": (i32, [P, P, u64, u32, PP]),
     "svo_octree_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
     "svo_octree_download": (i32, [P, P, P, u64]),
@@ -118,6 +120,16 @@ _SIGS = {
     "svo_march_beam": (i32, [P, P, C.POINTER(svo_camera), u32, u32, P, P]),
     "svo_march_rays": (i32, [P, P, P, u64, i32, P]),
     "svo_bounce_rays": (i32, [P, P, u64, u64, P]),
+    "svo_march_batch": (i32, [P, PP, C.POINTER(svo_camera), u32, u32, u32, i32, P]),
+    "svo_march_plan_create": (i32, [P, PP, C.POINTER(svo_camera), u32, u32, u32, i32, PP]),
+    "svo_march_plan_set_cameras": (i32, [P, P, C.POINTER(svo_camera)]),
+    "svo_march_plan_use_graph": (i32, [P, i32]),
+    "svo_march_plan_set_shape": (i32, [P, i32]),
+    "svo_march_plan_run": (i32, [P, P, P]),
+    "svo_march_plan_destroy": (i32, [P]),
+    "svo_march_rays_batch": (i32, [P, PP, u32, C.POINTER(u64), P, i32, P]),
+    "svo_autotune": (i32, [P, i32]),
+    "svo_autotune_report": (i32, [P, C.c_char_p, u64, C.POINTER(u64)]),
     "svo_camera_from_yaw_pitch": (i32, [C.POINTER(C.c_float), f32, f32, f32, f32, C.POINTER(svo_camera)]),
     "svo_gen_scene": (i32, [i32, u64, u64, P, P, C.POINTER(u64)]),
 }

@@ -15,6 +15,9 @@ context (one HIP stream) instead of Vulkan command pools; "Cmd*" methods enqueue
     OctreeTracer.Create / Resize / CmdBeamRenderPass / CmdDrawPipeline, m_beam_enable
                                  src/OctreeTracer.hpp:15-16,57-70
 
+Batched forms (new; the reference handles one scene at a time): voxelize_batch, build_batch,
+MarchPlan / march_batch and march_rays_batch run many scenes per launch.
+
 Errors raise SvoError carrying the library's status code.
 """
 import ctypes as C
@@ -36,6 +39,12 @@ __all__ = [
     "OctreeTracer",
     "gen_scene",
     "camera_look_at",
+    "voxelize_batch",
+    "build_batch",
+    "MarchPlan",
+    "march_batch",
+    "march_rays_batch",
+    "autotune",
     "HIT_DTYPE",
     "SvoError",
 ]
@@ -411,6 +420,128 @@ def march_rays(ctx, octree, rays, variant=_capi.SVO_MARCH_LEAF_ITER):
     d_hits = ctx.alloc(R * 32)
     ctx.check(lib.svo_march_rays(ctx.h, octree.h, d_rays.ptr, R, variant, d_hits.ptr), "march rays")
     return d_hits.download(HIT_DTYPE, R)
+
+
+def _cam(c):
+    return c.uniform() if isinstance(c, Camera) else c
+
+
+def _handles(objs):
+    return (C.c_void_p * max(1, len(objs)))(*[o.h for o in objs])
+
+
+def voxelize_batch(scenes, levels):
+    """svo_voxelize_batch: voxelizes every scene (scene i at levels[i]) in one count launch and one
+    fill launch; returns filled Voxelizer objects."""
+    n = len(scenes)
+    if n == 0:
+        return []
+    ctx = scenes[0].ctx
+    lv = (C.c_uint32 * n)(*[int(x) for x in levels])
+    out = (C.c_void_p * n)()
+    ctx.check(lib.svo_voxelize_batch(ctx.h, _handles(scenes), lv, n, out), "voxelize batch")
+    return [Voxelizer(ctx, C.c_void_p(out[i]), int(levels[i]), scenes[i]) for i in range(n)]
+
+
+def build_batch(voxelizers, wait=True):
+    """svo_build_batch: builds one octree per fragment list with one launch per build pass; returns
+    Octree objects (finished when wait is set; otherwise they are ordered on the stream)."""
+    n = len(voxelizers)
+    if n == 0:
+        return []
+    ctx = voxelizers[0].ctx
+    out = (C.c_void_p * n)()
+    ctx.check(lib.svo_build_batch(ctx.h, _handles(voxelizers), n, out), "build batch")
+    octs = []
+    for i in range(n):
+        W = C.c_uint64()
+        if wait:
+            ctx.check(lib.svo_build_finish(ctx.h, out[i], C.byref(W)), "build finish")
+        o = Octree(ctx, C.c_void_p(out[i]), voxelizers[i].level, W.value)
+        o._own = True
+        octs.append(o)
+    return octs
+
+
+class MarchPlan:
+    """svo_march_plan_*: a reusable batch march of many octrees (one camera each)."""
+
+    def __init__(self, octrees, cameras, width, height, beam=True):
+        self.ctx = octrees[0].ctx
+        self.n, self.width, self.height = len(octrees), int(width), int(height)
+        cams = (svo_camera * self.n)(*[_cam(c) for c in cameras])
+        h = C.c_void_p()
+        self.ctx.check(
+            lib.svo_march_plan_create(
+                self.ctx.h, _handles(octrees), cams, self.n, self.width, self.height, 1 if beam else 0, C.byref(h)
+            ),
+            "march plan",
+        )
+        self.h = h
+        self._octrees = list(octrees)
+
+    def set_cameras(self, cameras):
+        cams = (svo_camera * self.n)(*[_cam(c) for c in cameras])
+        self.ctx.check(lib.svo_march_plan_set_cameras(self.ctx.h, self.h, cams), "plan cameras")
+
+    def use_graph(self, on=True):
+        self.ctx.check(lib.svo_march_plan_use_graph(self.h, 1 if on else 0), "plan graph")
+
+    def set_shape(self, shape):
+        """Launch tile: 0 = 16x16, 1 = 8x8, 2 = 16x8, 3 = 32x8, -1 = default / autotuned."""
+        self.ctx.check(lib.svo_march_plan_set_shape(self.h, int(shape)), "plan shape")
+
+    def run(self, hits):
+        self.ctx.check(lib.svo_march_plan_run(self.ctx.h, self.h, hits.ptr), "plan run")
+
+    def alloc_hits(self):
+        return self.ctx.alloc(self.n * self.width * self.height * 32)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib.svo_march_plan_destroy(self.h)
+            self.h = None
+
+
+def march_batch(octrees, cameras, width, height, beam=True):
+    """svo_march_batch: hit records (n, height, width) of every octree's frame."""
+    n = len(octrees)
+    ctx = octrees[0].ctx
+    hits = ctx.alloc(n * width * height * 32)
+    cams = (svo_camera * n)(*[_cam(c) for c in cameras])
+    ctx.check(
+        lib.svo_march_batch(ctx.h, _handles(octrees), cams, n, width, height, 1 if beam else 0, hits.ptr),
+        "march batch",
+    )
+    return hits.download(HIT_DTYPE, n * width * height).reshape(n, height, width)
+
+
+def march_rays_batch(octrees, ray_sets, variant=_capi.SVO_MARCH_LEAF_ITER):
+    """svo_march_rays_batch: ray_sets[i] (R_i, 6) float32 marched against octrees[i], one launch;
+    returns one hit array per set."""
+    n = len(octrees)
+    ctx = octrees[0].ctx
+    sets = [np.ascontiguousarray(r, dtype=np.float32).reshape(-1, 6) for r in ray_sets]
+    begin = np.zeros(n + 1, dtype=np.uint64)
+    begin[1:] = np.cumsum([r.shape[0] for r in sets])
+    R = int(begin[-1])
+    rays = np.concatenate(sets) if R else np.zeros((1, 6), np.float32)
+    d_rays = ctx.upload(rays)
+    d_hits = ctx.alloc(max(R, 1) * 32)
+    rb = (C.c_uint64 * (n + 1))(*[int(x) for x in begin])
+    ctx.check(lib.svo_march_rays_batch(ctx.h, _handles(octrees), n, rb, d_rays.ptr, variant, d_hits.ptr), "ray batch")
+    hits = d_hits.download(HIT_DTYPE, R) if R else np.zeros(0, HIT_DTYPE)
+    return [hits[int(begin[i]) : int(begin[i + 1])] for i in range(n)]
+
+
+def autotune(ctx, on=True):
+    """svo_autotune; returns the decision report so far."""
+    ctx.check(lib.svo_autotune(ctx.h, 1 if on else 0), "autotune")
+    n = C.c_uint64()
+    lib.svo_autotune_report(ctx.h, None, 0, C.byref(n))
+    buf = C.create_string_buffer(n.value + 1)
+    lib.svo_autotune_report(ctx.h, buf, n.value + 1, None)
+    return buf.value.decode()
 
 
 def gen_scene(kind, seed, param=0):

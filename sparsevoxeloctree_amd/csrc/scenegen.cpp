@@ -12,6 +12,9 @@
 //           room.  param = d (0 -> 1).
 //   kind 3  param spheres as kind 1 (0 -> 992) plus 4032 per sphere thin slivers with aspect ratio
 //           log-uniform in [1,1000], long edge U[0.002,0.02], random orientation, centre U[0,1]^3.
+//   kind 4  kind 1 with a triangle budget: param (>= 24) triangles as floor(param / 4032) spheres of
+//           64 x 32 segments plus, for the remainder r >= 24, one sphere of 2n x n segments,
+//           n = floor(sqrt(r / 4)) (2n(2n-1) <= r triangles).
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -231,6 +234,17 @@ extern "C" int svo_gen_scene(int kind, uint64_t seed, uint64_t param, float *tri
 		uint64_t rs = (param >> 32) & 0xFFFF, ls = (param >> 48) & 0xFFFF;
 		spheres_kind1(o, rng, n, rs ? rs / 100.0 : 1.0);
 		slivers(o, rng, n * 4032, ls ? ls / 100.0 : 1.0);
+		break;
+	}
+	case 4: {
+		const uint64_t full = param / 4032, r = param - full * 4032;
+		spheres_kind1(o, rng, full);
+		if (r >= 24) {
+			const int n = (int)std::floor(std::sqrt((double)r / 4.0));
+			double rad = rng.uniform(0.02, 0.1);
+			double c[3] = {rng.uniform(), rng.uniform(), rng.uniform()};
+			sphere(o, rng, c, rad, 2 * n, n);
+		}
 		break;
 	}
 	default:

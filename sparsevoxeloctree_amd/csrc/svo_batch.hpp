@@ -1,0 +1,153 @@
+// Scene batches: every voxelize and build kernel works on a batch of S scenes (S = 1 for the
+// single-scene entry points).  Items of all scenes (triangles, fragments, bitmap words, leaf blocks)
+// form one index space; a scene's items are contiguous, and begin[s] (non-decreasing, begin[0] = 0,
+// S + 1 entries) is the first item of scene s.  Per-scene parameters live in a device table of
+// segment descriptors, so one launch covers the whole batch whatever the scenes' sizes and levels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "svo_runtime.hpp"
+
+namespace svo {
+
+constexpr uint32_t kMaxBatch = 1024; // scenes per batch call
+constexpr uint32_t kMaxLevel = 12;
+
+// Scene of item i: the last s with begin[s] <= i (scenes without items are skipped).
+__device__ __forceinline__ uint32_t seg_find(const uint64_t *__restrict__ begin, uint32_t S, uint64_t i) {
+	uint32_t lo = 0, hi = S - 1u;
+	while (lo < hi) {
+		const uint32_t mid = (lo + hi + 1u) >> 1;
+		if (begin[mid] <= i)
+			lo = mid;
+		else
+			hi = mid - 1u;
+	}
+	return lo;
+}
+
+// Same, starting from a scene s at or before item i's (the scene of the block's first item).
+__device__ __forceinline__ uint32_t seg_walk(const uint64_t *__restrict__ begin, uint32_t S, uint32_t s, uint64_t i) {
+	while (s + 1u < S && begin[s + 1u] <= i)
+		++s;
+	return s;
+}
+
+// Camera of one march (the uniform block of src/Camera.cpp:77-95 without padding).
+struct CamArgs {
+	float px, py, pz, lx, ly, lz, sx, sy, sz, ux, uy, uz;
+};
+
+// One octree of a march batch.
+struct MarchSeg {
+	const uint32_t *oct;
+	uint64_t W;
+	CamArgs cam;
+	float orig_sz; // exp2(1 - level), the beam's cone origin size (src/OctreeTracer.cpp:206)
+	uint32_t level;
+};
+
+// One scene of a voxelize batch.  frag_begin / F are written by the count pass.
+struct VoxSeg {
+	const float *xyz;
+	const uint32_t *rgb;
+	uint2 *frags;
+	uint64_t T;
+	uint64_t frag_begin; // global index of the scene's first fragment
+	uint64_t F;
+	uint32_t level;
+	uint32_t pad;
+};
+
+// One scene of a build batch.  W, nbins, overflow and fb are written by the structure phase.
+struct BuildSeg {
+	const uint2 *frags;
+	uint32_t *words;     // the scene's octree (node phase)
+	uint64_t F;
+	uint64_t bm;         // global word offset of the scene's bitmaps (B[1] first)
+	uint64_t bin_begin;  // global index of the scene's first leaf block (node phase)
+	uint64_t W;          // octree words = (counter + 1) * 8
+	uint64_t nbins;      // leaf blocks = nodes of depth L-1 (the root when L = 1)
+	uint32_t L;
+	uint32_t overflow;   // W > 2^30
+	uint64_t off[14];    // word offset of B[d] relative to bm, d = 1..L; off[L] = the scene's bitmap words
+	uint32_t fb[16];     // first block of every depth: fb[0] = 0 (root block), fb[1] = 1
+};
+
+// Voxelize batch state, from the count pass to the fill.
+struct VoxBatch {
+	uint32_t S = 0;
+	uint64_t Ttot = 0;
+	std::vector<VoxSeg> h_segs;
+	std::vector<uint64_t> h_tbeg;
+	VoxSeg *segs = nullptr;     // device table
+	uint64_t *tbeg = nullptr;   // S + 1 triangle begins
+	uint64_t *offsets = nullptr; // Ttot exclusive fragment offsets (global)
+	uint32_t *big = nullptr;    // big-triangle list: count, then global triangle indices
+};
+
+// Build batch state, from the structure phase to the node phase.
+struct BuildBatch {
+	uint32_t S = 0;
+	uint32_t maxL = 0;
+	uint64_t Ftot = 0, TW = 0, nbins = 0;
+	std::vector<BuildSeg> h_segs;
+	// begin tables, S + 1 entries each: fbeg (fragments), bbeg (leaf blocks), dw[d] at 2 + d (words of
+	// B[d] of the scenes with L > d), dr[d] at 3 + kMaxLevel + d (same, scenes with L > d + 1)
+	std::vector<uint64_t> h_index;
+	BuildSeg *segs = nullptr;
+	uint64_t *index = nullptr;
+	uint32_t *bm = nullptr, *rk = nullptr; // bitmaps (TW words), ranks (TW + 1)
+	uint32_t *cnt = nullptr, *aux = nullptr; // per leaf block (nbins)
+	uint32_t *node = nullptr;   // 2 * Ftot: {leaf block, place in its bucket}
+	uint32_t *bucket = nullptr; // 2 * Ftot
+	uint32_t *list = nullptr;   // 2 + 2 * cap: large leaf blocks
+	const uint64_t *fbeg() const { return index; }
+	const uint64_t *bbeg() const { return index + (S + 1); }
+	const uint64_t *dw(uint32_t d) const { return index + (2 + d) * (uint64_t)(S + 1); }
+	uint64_t h_dw(uint32_t d, uint32_t s) const { return h_index[(2 + d) * (uint64_t)(S + 1) + s]; }
+};
+
+void vox_batch_release(svo_ctx *ctx, VoxBatch *vb);
+void build_batch_release(svo_ctx *ctx, BuildBatch *bb);
+
+// Voxelize: count pass over the batch (one read-back of every scene's F), then the fill.
+int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, VoxBatch *vb);
+int voxelize_fill_batch(svo_ctx *ctx, VoxBatch *vb);
+
+// Build: structure phase (one read-back of every scene's W), octree allocation, node phase.
+int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S);
+
+// A reusable batch march of S octrees, one camera each, width x height primary rays each: device
+// table of octrees and cameras, beam scratch, the launch shape (default or autotuned), and
+// optionally a captured HIP graph of the launches, replayed while the hit buffer stays the same.
+struct MarchPlan {
+	uint32_t S = 0, width = 0, height = 0, bw = 0, bh = 0;
+	int beam = 0;
+	std::vector<MarchSeg> h_segs;
+	MarchSeg *segs = nullptr;
+	float *beam_buf = nullptr;
+	int shape = -1;
+	bool use_graph = false;
+	hipGraphExec_t exec = nullptr;
+	const void *exec_hits = nullptr;
+};
+int march_plan_init(svo_ctx *ctx, MarchPlan *p, const svo_octree *const *octs, const svo_camera *cams, uint32_t S,
+                    uint32_t width, uint32_t height, int beam);
+int march_plan_set_cameras(svo_ctx *ctx, MarchPlan *p, const svo_camera *cams);
+int march_plan_run(svo_ctx *ctx, MarchPlan *p, svo_hit *hits);
+void march_plan_release(svo_ctx *ctx, MarchPlan *p);
+// Explicit rays against S octrees in one launch; ray_begin (host, S + 1 entries) splits rays.
+int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, const uint64_t *ray_begin,
+                     const float *rays, int variant, svo_hit *hits);
+
+} // namespace svo
+
+struct svo_march_plan {
+	svo_ctx *ctx;
+	svo::MarchPlan p;
+};

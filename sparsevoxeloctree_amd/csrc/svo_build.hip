@@ -1,4 +1,5 @@
-// Octree construction (OctreeBuilder::CmdBuild, src/OctreeBuilder.cpp:142-210).
+// Octree construction (OctreeBuilder::CmdBuild, src/OctreeBuilder.cpp:142-210) for a batch of
+// scenes: every kernel covers all scenes of the batch in one launch (segment tables, svo_batch.hpp).
 //
 // The canonical breadth-first node order (child blocks in ascending parent word index) is the
 // Morton order of the node prefixes at every depth: a block's 8 children sit in slot order
@@ -14,7 +15,7 @@
 //                                occupied node among its depth = its block among the allocated
 //                                children blocks); child pointer = 8 * (first block of depth + rank)
 //   octree_modify_arg.comp    -> bm_levels: per-depth counts -> first block of every depth, the
-//                                leaf range {begin, num} and W = (counter + 1) * 8
+//                                leaf blocks and W = (counter + 1) * 8
 //   octree_init_node.comp     -> bm_write_blocks writes every block whole (pointers and zeros)
 //   octree_tag_node.comp      -> build_tag_leaf: leaf block ("bin") of each fragment = rank of its
 //   (level L) + leaf CAS fold    depth-(L-1) node; count per bin, scan, scatter into per-bin buckets;
@@ -22,6 +23,15 @@
 //                                fold the 8 leaves of the block in that order (the reference's order
 //                                is whatever its atomics produce), writing the leaf block whole.
 // The node words are identical to the reference's pass structure run with a serial allocator.
+//
+// Batch layout: the scenes' fragments, bitmap words and leaf blocks each form one index space
+// (begin tables fbeg, dw/dr per depth, bbeg); the rank array R is one prefix over all bitmaps, and
+// every rank is a difference of two R entries of the same scene, so modular uint32 arithmetic
+// stays exact.
+#include <algorithm>
+#include <string>
+
+#include "svo_batch.hpp"
 #include "svo_device.hpp"
 #include "svo_runtime.hpp"
 #include "svo_scan.hpp"
@@ -30,7 +40,17 @@ namespace svo {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kLdsBucket = 4096;   // largest bucket sorted in LDS
+constexpr uint32_t kLdsBucket = 4096; // largest bucket sorted in LDS
+constexpr uint32_t kRegBucket = 16;   // leaf blocks up to this size: one thread each (build_fold)
+constexpr uint32_t kWaveBucket = 64;  // up to this size: one wave each (build_fold_mid)
+
+// Scene of item i; base is the block-uniform first item of the loop step (a cheap starting point).
+__device__ __forceinline__ uint32_t item_seg(const uint64_t *__restrict__ begin, uint32_t S, uint64_t base,
+                                             uint64_t i) {
+	if (S == 1u)
+		return 0u;
+	return seg_walk(begin, S, seg_find(begin, S, base), i);
+}
 
 // Effective voxel coordinate seen by the reference's descent: the coordinate's bits, saturating
 // at res-1 for coordinates >= res (shader/octree_tag_node.comp:18-30).
@@ -66,13 +86,6 @@ __device__ __forceinline__ void cmp_swap(uint64_t &x, uint64_t &y) {
 
 __device__ __forceinline__ uint64_t frag_key(uint2 f) { return ((uint64_t)f.x << 32) | f.y; }
 
-// Bitmap layout: B[d] (d = 1..L-1, 8^d bits) at word offset off[d]; R holds the exclusive popcount
-// prefix of every word of the concatenation.
-struct Levels {
-	uint32_t L;
-	uint64_t off[14]; // off[d] for d = 1..L, off[L] = total words
-};
-
 // Rank of node P among the occupied nodes of depth d.
 __device__ __forceinline__ uint32_t node_rank(const uint32_t *__restrict__ B, const uint32_t *__restrict__ R,
                                               uint64_t off, uint64_t P) {
@@ -91,38 +104,56 @@ __global__ void __launch_bounds__(kThreads) bm_zero(uint32_t *B, uint64_t n) {
 // shuffles) and only the last lane of every run of equal words touches memory: one atomic when a
 // plain load does not already show all of the run's bits.
 __global__ void __launch_bounds__(kThreads)
-    bm_mark(const uint2 *__restrict__ frags, uint64_t F, uint32_t L, uint32_t *Blast) {
-	const uint32_t res = 1u << L, lane = threadIdx.x & 63u;
+    bm_mark(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+            uint32_t *B) {
+	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < F; base += stride) {
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
 		const uint64_t f = base + threadIdx.x;
-		uint32_t word = 0xFFFFFFFFu, bits = 0u; // node index < 2^33, so word < 2^28
-		if (f < F) {
-			uint32_t x, y, z;
-			decode_frag(frags[f], res, x, y, z);
-			const uint64_t P = morton(x, y, z) >> 3;
-			word = (uint32_t)(P >> 5);
-			bits = 1u << (uint32_t)(P & 31u);
+		uint64_t word = ~0ull; // global bitmap word
+		uint32_t bits = 0u;
+		if (f < Ftot) {
+			const uint32_t s = item_seg(fbeg, S, base, f);
+			const BuildSeg &g = segs[s];
+			const uint32_t L = g.L;
+			if (L > 1u) {
+				uint32_t x, y, z;
+				decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
+				const uint64_t P = morton(x, y, z) >> 3;
+				word = g.bm + g.off[L - 1u] + (P >> 5);
+				bits = 1u << (uint32_t)(P & 31u);
+			}
 		}
 #pragma unroll
 		for (uint32_t d = 1; d < 64u; d <<= 1) {
-			const uint32_t ow = __shfl_up(word, d, 64), ob = __shfl_up(bits, d, 64);
+			const uint64_t ow = __shfl_up(word, d, 64);
+			const uint32_t ob = __shfl_up(bits, d, 64);
 			bits |= (lane >= d && ow == word) ? ob : 0u; // any lane with the same word may contribute
 		}
-		const uint32_t next = __shfl_down(word, 1u, 64);
-		if (word != 0xFFFFFFFFu && (lane == 63u || next != word)) {
-			uint32_t *w = &Blast[word];
+		const uint64_t next = __shfl_down(word, 1u, 64);
+		if (word != ~0ull && (lane == 63u || next != word)) {
+			uint32_t *w = B + word;
 			if ((*w & bits) != bits) // a stale cached word only costs a redundant atomic
 				atomicOr(w, bits);
 		}
 	}
 }
 
-// B[d] bit q = OR of byte q of B[d+1] (node q occupied iff one of its 8 children is).
+// Bit q of B[d] = OR of byte q of B[d+1] (node q occupied iff one of its 8 children is); items are
+// the words of B[d] of the scenes that have B[d+1] (begin table dr).
 __global__ void __launch_bounds__(kThreads)
-    bm_reduce(const uint32_t *__restrict__ Bc, uint64_t nwc, uint32_t *__restrict__ Bp, uint64_t nwp) {
+    bm_reduce(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dr, uint32_t S, uint64_t n, uint32_t d,
+              uint32_t *B) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < nwp; w += stride) {
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
+		const uint64_t i = base + threadIdx.x;
+		if (i >= n)
+			continue;
+		const uint32_t s = item_seg(dr, S, base, i);
+		const BuildSeg &g = segs[s];
+		const uint64_t w = i - dr[s];
+		const uint32_t *Bc = B + g.bm + g.off[d + 1u];
+		const uint64_t nwc = g.off[d + 2u] - g.off[d + 1u];
 		uint32_t bits = 0u;
 #pragma unroll
 		for (uint32_t k = 0; k < 8u; ++k) {
@@ -132,7 +163,7 @@ __global__ void __launch_bounds__(kThreads)
 			for (uint32_t b = 0; b < 4u; ++b)
 				bits |= (((v >> (8u * b)) & 0xffu) != 0u ? 1u : 0u) << (4u * k + b);
 		}
-		Bp[w] = bits;
+		B[g.bm + g.off[d] + w] = bits;
 	}
 }
 
@@ -153,55 +184,63 @@ struct RankDst {
 	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { R[i] = (uint32_t)k; }
 };
 
-// Per-depth node counts -> first block of every depth (fb[0] = 0 is the root block, fb[1] = 1),
-// the leaf range and W (src/OctreeBuilder.cpp:212-214: W = (counter + 1) * 8).
-__global__ void bm_levels(const uint32_t *__restrict__ R, Levels lv, uint32_t *fb, BuildState *st, uint64_t cap) {
-	if (threadIdx.x != 0)
+// One thread per scene: per-depth node counts -> first block of every depth (fb[0] = 0 is the root
+// block, fb[1] = 1), the leaf blocks and W (src/OctreeBuilder.cpp:212-214: W = (counter + 1) * 8).
+__global__ void bm_levels(BuildSeg *segs, uint32_t S, const uint32_t *__restrict__ R, uint64_t cap) {
+	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
 		return;
-	const uint32_t L = lv.L;
+	BuildSeg &g = segs[s];
+	const uint32_t L = g.L;
+	const uint32_t *Rs = R + g.bm;
 	uint64_t first = 1;
-	fb[0] = 0u;
-	fb[1] = 1u;
+	g.fb[0] = 0u;
+	g.fb[1] = 1u;
 	uint64_t nlast = 1; // nodes at depth L-1 (the root when L == 1)
 	for (uint32_t d = 1; d < L; ++d) {
-		const uint64_t nd = (uint64_t)R[lv.off[d + 1]] - R[lv.off[d]];
+		const uint64_t nd = (uint32_t)(Rs[g.off[d + 1u]] - Rs[g.off[d]]);
 		first += nd;
-		fb[d + 1] = (uint32_t)(first < 0xFFFFFFFFull ? first : 0xFFFFFFFFull);
+		g.fb[d + 1u] = (uint32_t)(first < 0xFFFFFFFFull ? first : 0xFFFFFFFFull);
 		nlast = nd;
 	}
-	const uint64_t W = 8u * first;
-	const uint32_t slot = L & 1u;
-	st->need_words = W;
-	st->overflow = W > cap ? L : 0u;
-	st->begin[slot] = (uint32_t)(8u * (L == 1u ? 0u : fb[L - 1])); // first leaf word
-	st->num[slot] = (uint32_t)nlast;                              // leaf blocks (one per depth-(L-1) node)
-	st->counter[slot] = (uint32_t)(first - 1u);
+	g.W = 8u * first;
+	g.overflow = g.W > cap ? 1u : 0u;
+	g.nbins = nlast;
 }
 
-// Blocks of depth d (the children of depth-(d-1) nodes), one thread per word of B[d] = 4 parents.
-// Every block is written whole: pointer words for occupied children, zeros otherwise.
+// Blocks of depth d (the children of depth-(d-1) nodes), one thread per word of B[d] = 4 parents;
+// items are the words of B[d] of the scenes with L > d (begin table dw).  Every block is written
+// whole: pointer words for occupied children, zeros otherwise.
 __global__ void __launch_bounds__(kThreads)
-    bm_write_blocks(const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, Levels lv, uint32_t d,
-                    const uint32_t *__restrict__ fb, uint32_t *oct) {
-	const uint64_t offd = lv.off[d], nw = lv.off[d + 1] - offd;
-	const uint32_t fbd = fb[d], fbp = fb[d - 1];
+    bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dw, uint32_t S, uint64_t n,
+                    uint32_t d, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t w = (uint64_t)blockIdx.x * kThreads + threadIdx.x; w < nw; w += stride) {
-		const uint32_t v = B[offd + w];
-		const uint32_t base = R[offd + w] - R[offd];
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
+		const uint64_t i = base + threadIdx.x;
+		if (i >= n)
+			continue;
+		const uint32_t s = item_seg(dw, S, base, i);
+		const BuildSeg &g = segs[s];
+		const uint64_t w = i - dw[s];
+		const uint32_t *Bs = B + g.bm, *Rs = R + g.bm;
+		const uint64_t offd = g.off[d];
+		const uint32_t v = Bs[offd + w];
+		const uint32_t base_rank = Rs[offd + w] - Rs[offd];
+		const uint32_t fbd = g.fb[d], fbp = g.fb[d - 1u];
+		uint32_t *oct = g.words;
 #pragma unroll
 		for (uint32_t k = 0; k < 4u; ++k) {
 			const uint32_t byte = (v >> (8u * k)) & 0xffu;
 			const uint64_t q = 4u * w + k; // parent at depth d-1
 			if (d == 1u ? q != 0u : byte == 0u)
 				continue; // the root block always exists
-			const uint32_t block = d == 1u ? 0u : fbp + node_rank(B, R, lv.off[d - 1], q);
-			uint32_t r = base + (uint32_t)__popc(v & ((1u << (8u * k)) - 1u));
+			const uint32_t block = d == 1u ? 0u : fbp + node_rank(Bs, Rs, g.off[d - 1u], q);
+			uint32_t r = base_rank + (uint32_t)__popc(v & ((1u << (8u * k)) - 1u));
 			uint32_t wd[8];
 #pragma unroll
-			for (uint32_t s = 0; s < 8u; ++s) {
-				const bool set = (byte >> s) & 1u;
-				wd[s] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
+			for (uint32_t t = 0; t < 8u; ++t) {
+				const bool set = (byte >> t) & 1u;
+				wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
 				r += set ? 1u : 0u;
 			}
 			uint4 *dst = reinterpret_cast<uint4 *>(oct + 8u * (uint64_t)block);
@@ -211,21 +250,28 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
-// Leaf block ("bin") of every fragment: the rank of its depth-(L-1) node.  Fragments are counted
-// per bin with one atomic per run of equal neighbours; the atomic's old value gives every fragment
-// its place inside the bin, so node[f] = {bin, place} and the scatter needs no atomics.
+// Leaf block ("bin") of every fragment: the rank of its depth-(L-1) node, offset by the scene's
+// first bin.  Fragments are counted per bin with one atomic per run of equal neighbours; the
+// atomic's old value gives every fragment its place inside the bin, so node[f] = {bin, place} and
+// the scatter needs no atomics.
 __global__ void __launch_bounds__(kThreads)
-    build_tag_leaf(const uint2 *__restrict__ frags, uint64_t F, const uint32_t *__restrict__ B,
-                   const uint32_t *__restrict__ R, Levels lv, uint2 *__restrict__ node, uint32_t *cnt) {
-	const uint32_t L = lv.L, res = 1u << L, lane = threadIdx.x & 63u;
+    build_tag_leaf(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+                   const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint2 *__restrict__ node,
+                   uint32_t *cnt) {
+	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < F; base += stride) {
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
 		const uint64_t f = base + threadIdx.x;
 		uint32_t bin = 0xFFFFFFFFu;
-		if (f < F) {
+		if (f < Ftot) {
+			const uint32_t s = item_seg(fbeg, S, base, f);
+			const BuildSeg &g = segs[s];
+			const uint32_t L = g.L;
 			uint32_t x, y, z;
-			decode_frag(frags[f], res, x, y, z);
-			bin = L == 1u ? 0u : node_rank(B, R, lv.off[L - 1], morton(x, y, z) >> 3);
+			decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
+			const uint32_t local =
+			    L == 1u ? 0u : node_rank(B + g.bm, R + g.bm, g.off[L - 1u], morton(x, y, z) >> 3);
+			bin = (uint32_t)g.bin_begin + local;
 		}
 		const uint32_t prev = __shfl_up(bin, 1u, 64);
 		const bool head = (lane == 0u) || prev != bin;
@@ -239,19 +285,23 @@ __global__ void __launch_bounds__(kThreads)
 		const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
 		const uint32_t head_lane = 63u - (uint32_t)__clzll((long long)upto);
 		pos = __shfl(pos, (int)head_lane, 64) + (lane - head_lane);
-		if (f < F)
+		if (f < Ftot)
 			node[f] = make_uint2(bin, pos);
 	}
 }
 
 // Scatter every fragment to its place in its bin's bucket (aux[j] = first slot of bin j).
 __global__ void __launch_bounds__(kThreads)
-    build_scatter(const uint2 *__restrict__ frags, uint64_t F, const uint2 *__restrict__ node,
-                  const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
+    build_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+                  const uint2 *__restrict__ node, const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t f = (uint64_t)blockIdx.x * kThreads + threadIdx.x; f < F; f += stride) {
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
+		const uint64_t f = base + threadIdx.x;
+		if (f >= Ftot)
+			continue;
+		const uint32_t s = item_seg(fbeg, S, base, f);
 		const uint2 np = node[f];
-		bucket[aux[np.x] + np.y] = frags[f];
+		bucket[aux[np.x] + np.y] = segs[s].frags[f - fbeg[s]];
 	}
 }
 
@@ -311,11 +361,8 @@ __device__ __forceinline__ void store_block(uint32_t *oct, uint64_t block, const
 	dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-constexpr uint32_t kRegBucket = 16; // bins up to this size: one thread each (build_fold)
-constexpr uint32_t kWaveBucket = 64; // up to this size: one wave each (build_fold_mid)
-
-// Work lists of bins too large for build_fold: list[0] = mid count, list[1] = big count, mid bins
-// at list[2 ..], big bins at list[2 + cap ..] (each list holds at most F / 17 bins).
+// Work lists of leaf blocks too large for build_fold: list[0] = mid count, list[1] = big count,
+// mid blocks at list[2 ..], big blocks at list[2 + cap ..] (each list holds at most Ftot / 17).
 __device__ __forceinline__ void push_bin(uint32_t *list, uint64_t cap, uint32_t j, uint32_t n) {
 	const uint32_t which = n > kWaveBucket ? 1u : 0u;
 	const uint32_t q = atomicAdd(&list[which], 1u);
@@ -323,22 +370,37 @@ __device__ __forceinline__ void push_bin(uint32_t *list, uint64_t cap, uint32_t 
 		list[2u + which * cap + q] = j;
 }
 
+// Octree block of global leaf block j (scene s): the scene's first leaf block + j's rank.
+struct LeafTarget {
+	uint32_t *oct;
+	uint64_t block;
+	uint32_t res;
+};
+__device__ __forceinline__ LeafTarget leaf_target(const BuildSeg &g, const uint64_t *bbeg, uint32_t s, uint32_t j) {
+	const uint32_t L = g.L;
+	return LeafTarget{g.words, (uint64_t)(L == 1u ? 0u : g.fb[L - 1u]) + (j - bbeg[s]), 1u << L};
+}
+
 // One thread per leaf block: its fragments (<= 16) are sorted by the 64-bit fragment in registers
 // (19- or 63-comparator networks, +inf padding), then folded in that order into the 8 leaves; the
 // whole block is written, empty leaves as 0.
 __global__ void __launch_bounds__(kThreads)
-    build_fold(uint32_t *oct, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux,
-               const uint2 *__restrict__ bucket, uint32_t *list, uint64_t cap, const BuildState *st, uint32_t L) {
-	const uint32_t slot = L & 1u, res = 1u << L;
-	const uint64_t first = st->begin[slot] / 8u;
-	const uint32_t nbins = st->num[slot];
-	const uint32_t stride = gridDim.x * kThreads;
-	for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < nbins; j += stride) {
+    build_fold(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S, uint64_t nbins,
+               const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
+               uint32_t *list, uint64_t cap) {
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < nbins; base += stride) {
+		const uint64_t jj = base + threadIdx.x;
+		if (jj >= nbins)
+			continue;
+		const uint32_t j = (uint32_t)jj;
 		const uint32_t n = cnt[j];
 		if (n > kRegBucket) {
 			push_bin(list, cap, j, n);
 			continue;
 		}
+		const uint32_t s = item_seg(bbeg, S, base, jj);
+		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
 		const uint2 *b = bucket + aux[j];
 		uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}, c[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
 		uint64_t k[16];
@@ -374,8 +436,8 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
 		for (uint32_t a = 0; a < 16u; ++a)
 			if (a < n)
-				slot_push(w, c, k[a], res);
-		store_block(oct, first + j, w);
+				slot_push(w, c, k[a], t.res);
+		store_block(t.oct, t.block, w);
 	}
 }
 
@@ -407,17 +469,16 @@ template <class Get, class Put> __device__ void block_bitonic(uint32_t n, uint32
 	}
 }
 
-// Bins of 17..64 fragments, one wave each: lane i holds key i (+inf padding), a bitonic sort over
-// lane shuffles orders them, the sorted keys go through LDS, and lane s < 8 folds the keys of leaf
-// slot s in ascending order.
+// Leaf blocks of 17..64 fragments, one wave each: lane i holds key i (+inf padding), a bitonic sort
+// over lane shuffles orders them, the sorted keys go through LDS, and lane s < 8 folds the keys of
+// leaf slot s in ascending order.
 __global__ void __launch_bounds__(kThreads)
-    build_fold_mid(uint32_t *oct, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux,
-                   const uint2 *__restrict__ bucket, const uint32_t *list, uint64_t cap, const BuildState *st,
-                   uint32_t L) {
+    build_fold_mid(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
+                   const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
+                   const uint32_t *list, uint64_t cap) {
 	constexpr uint32_t kWaves = kThreads / 64;
 	__shared__ uint64_t s_keys[kWaves][64];
-	const uint32_t res = 1u << L, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-	const uint64_t first = st->begin[L & 1u] / 8u;
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 	uint64_t count = list[0];
 	if (count > cap)
 		count = cap;
@@ -442,27 +503,28 @@ __global__ void __launch_bounds__(kThreads)
 		s_keys[wave][lane] = key;
 		__syncthreads();
 		if (item < count && lane < 8u) {
+			const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
+			const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
 			uint32_t word = 0u, a = 0u;
 			for (uint32_t i = 0; i < n; ++i) {
 				const uint64_t kk = s_keys[wave][i];
-				if (key_slot(kk, res) == lane)
+				if (key_slot(kk, t.res) == lane)
 					word = fold_rt(word, (uint32_t)kk & 0xffffffu, a++);
 			}
-			oct[8u * (first + j) + lane] = word;
+			t.oct[8u * t.block + lane] = word;
 		}
 		__syncthreads();
 	}
 }
 
-// Bins of more than 64 fragments, one block each: bitonic sort of the keys (in LDS up to 4096, in
-// place in the bucket beyond), then 8 lanes fold the 8 leaves, each lane walking the sorted keys
-// and taking those of its slot.
+// Leaf blocks of more than 64 fragments, one block each: bitonic sort of the keys (in LDS up to
+// 4096, in place in the bucket beyond), then 8 lanes fold the 8 leaves, each lane walking the
+// sorted keys and taking those of its slot.
 __global__ void __launch_bounds__(kThreads)
-    build_fold_big(uint32_t *oct, const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, uint2 *bucket,
-                   const uint32_t *list, uint64_t cap, const BuildState *st, uint32_t L) {
+    build_fold_big(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
+                   const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, uint2 *bucket,
+                   const uint32_t *list, uint64_t cap) {
 	__shared__ uint64_t s_keys[kLdsBucket];
-	const uint32_t res = 1u << L;
-	const uint64_t first = st->begin[L & 1u] / 8u;
 	uint64_t count = list[1];
 	if (count > cap)
 		count = cap;
@@ -486,13 +548,15 @@ __global__ void __launch_bounds__(kThreads)
 			    [&](uint32_t i, uint64_t v) { b[i] = make_uint2((uint32_t)(v >> 32), (uint32_t)v); });
 		}
 		if (threadIdx.x < 8u) {
+			const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
+			const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
 			uint32_t word = 0u, a = 0u;
 			for (uint32_t i = 0; i < n; ++i) {
 				const uint64_t key = lds ? s_keys[i] : frag_key(b[i]);
-				if (key_slot(key, res) == threadIdx.x)
+				if (key_slot(key, t.res) == threadIdx.x)
 					word = fold_rt(word, (uint32_t)key & 0xffffffu, a++);
 			}
-			oct[8u * (first + j) + threadIdx.x] = word;
+			t.oct[8u * t.block + threadIdx.x] = word;
 		}
 		__syncthreads();
 	}
@@ -527,91 +591,76 @@ uint32_t grid_for(uint64_t items, svo_ctx *ctx) {
 	return (uint32_t)(g ? g : 1);
 }
 
-Levels levels_for(uint32_t L) {
-	Levels lv{};
-	lv.L = L;
-	uint64_t off = 0;
+// Word offsets of B[1..L-1] inside one scene's bitmaps; off[L] = the scene's bitmap words.
+void levels_for(uint32_t L, uint64_t *off) {
+	uint64_t o = 0;
 	for (uint32_t d = 1; d <= L; ++d) {
-		lv.off[d] = off;
+		off[d] = o;
 		if (d < L) {
 			const uint64_t bits = 1ull << (3u * d);
-			off += bits < 32u ? 1u : bits / 32u;
+			o += bits < 32u ? 1u : bits / 32u;
 		}
 	}
-	return lv;
 }
 
-} // namespace
+uint64_t big_capacity(uint64_t F) { return F / (kRegBucket + 1u) + 1u; }
 
-uint64_t build_bitmap_words(uint32_t L) { return levels_for(L).off[L]; }
-
-uint64_t build_big_capacity(uint64_t F) { return F / (kRegBucket + 1u) + 1u; }
-
-// Structure phase: bitmaps, ranks, level table; leaves st (slot L & 1) and W on the device.
-int build_structure(svo_ctx *ctx, svo_octree *oct) {
-	const svo_fraglist *fl = oct->src;
-	const uint32_t L = oct->level;
-	const uint64_t F = fl->F;
-	const uint2 *frags = reinterpret_cast<const uint2 *>(fl->frags);
-	const Levels lv = levels_for(L);
-	const uint64_t TW = lv.off[L];
+int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
+	const uint32_t S = bb.S;
 	hipError_t e;
-	if (TW > 0) {
-		if ((e = launch(ctx, "bm_zero", bm_zero, dim3(grid_for(TW, ctx)), dim3(kThreads), 0, oct->bm, TW)) != hipSuccess)
+	if (bb.TW > 0) {
+		if ((e = launch(ctx, "bm_zero", bm_zero, dim3(grid_for(bb.TW, ctx)), dim3(kThreads), 0, bb.bm, bb.TW)) !=
+		    hipSuccess)
 			return hip_check(ctx, e, "bm_zero");
-		if (F > 0 &&
-		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_for(F, ctx)), dim3(kThreads), 0, frags, F, L,
-		                oct->bm + lv.off[L - 1])) != hipSuccess)
+		if (bb.Ftot > 0 &&
+		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_for(bb.Ftot, ctx)), dim3(kThreads), 0,
+		                (const BuildSeg *)bb.segs, bb.fbeg(), S, bb.Ftot, bb.bm)) != hipSuccess)
 			return hip_check(ctx, e, "bm_mark");
-		for (uint32_t d = L - 1; d-- > 1;) {
-			const uint64_t nwc = lv.off[d + 2] - lv.off[d + 1], nwp = lv.off[d + 1] - lv.off[d];
-			if ((e = launch(ctx, "bm_reduce", bm_reduce, dim3(grid_for(nwp, ctx)), dim3(kThreads), 0,
-			                (const uint32_t *)(oct->bm + lv.off[d + 1]), nwc, oct->bm + lv.off[d], nwp)) != hipSuccess)
+		for (uint32_t d = bb.maxL >= 3 ? bb.maxL - 2 : 0; d >= 1; --d) {
+			const uint64_t n = bb.h_index[(3 + kMaxLevel + d) * (uint64_t)(S + 1) + S];
+			if (n > 0 && (e = launch(ctx, "bm_reduce", bm_reduce, dim3(grid_for(n, ctx)), dim3(kThreads), 0,
+			                         (const BuildSeg *)bb.segs, bb.index + (3 + kMaxLevel + d) * (uint64_t)(S + 1), S, n,
+			                         d, bb.bm)) != hipSuccess)
 				return hip_check(ctx, e, "bm_reduce");
 		}
-		PopSrc src{oct->bm, TW};
+		PopSrc src{bb.bm, bb.TW};
 		if ((e = launch(ctx, "rank_reduce", scan_reduce<PopSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_reduce");
 		if ((e = launch(ctx, "rank_parts", scan_parts<RankTotal>, dim3(1), dim3(1024), 0, ctx->d_scan_parts,
-		                RankTotal{oct->rk, TW})) != hipSuccess)
+		                RankTotal{bb.rk, bb.TW})) != hipSuccess)
 			return hip_check(ctx, e, "rank_parts");
 		if ((e = launch(ctx, "rank_down", scan_down<PopSrc, RankDst>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
-		                RankDst{oct->rk}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
+		                RankDst{bb.rk}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_down");
 	}
-	if ((e = launch(ctx, "bm_levels", bm_levels, dim3(1), dim3(64), 0, (const uint32_t *)oct->rk, lv, oct->lvl,
-	                ctx->d_state, (uint64_t)(1ull << 30))) != hipSuccess)
+	if ((e = launch(ctx, "bm_levels", bm_levels, dim3(div_up(S, 64)), dim3(64), 0, bb.segs, S,
+	                (const uint32_t *)bb.rk, (uint64_t)(1ull << 30))) != hipSuccess)
 		return hip_check(ctx, e, "bm_levels");
 	return SVO_OK;
 }
 
-// Node phase: the internal blocks, then the leaf blocks through bins (count, scan, scatter, sort +
-// fold).  oct->words holds W words, oct->cnt and oct->aux hold nbins words.
-int build_nodes(svo_ctx *ctx, svo_octree *oct) {
-	const svo_fraglist *fl = oct->src;
-	const uint32_t L = oct->level;
-	const uint64_t F = fl->F, nbins = oct->nbins;
-	const uint2 *frags = reinterpret_cast<const uint2 *>(fl->frags);
-	const Levels lv = levels_for(L);
+int node_phase(svo_ctx *ctx, BuildBatch &bb) {
+	const uint32_t S = bb.S;
+	const BuildSeg *segs = bb.segs;
 	hipError_t e;
-	for (uint32_t d = 1; d < L; ++d) {
-		const uint64_t nw = lv.off[d + 1] - lv.off[d];
-		if ((e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(nw, ctx)), dim3(kThreads), 0,
-		                (const uint32_t *)oct->bm, (const uint32_t *)oct->rk, lv, d, (const uint32_t *)oct->lvl,
-		                oct->words)) != hipSuccess)
+	for (uint32_t d = 1; d + 1 <= bb.maxL; ++d) {
+		const uint64_t n = bb.h_dw(d, S);
+		if (n > 0 && (e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(n, ctx)), dim3(kThreads), 0,
+		                         segs, bb.dw(d), S, n, d, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk)) !=
+		                 hipSuccess)
 			return hip_check(ctx, e, "bm_write_blocks");
 	}
-	if ((e = launch(ctx, "bin_zero", bm_zero, dim3(grid_for(nbins, ctx)), dim3(kThreads), 0, oct->cnt, nbins)) !=
+	if ((e = launch(ctx, "bin_zero", bm_zero, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, bb.cnt, bb.nbins)) !=
 	    hipSuccess)
 		return hip_check(ctx, e, "bin_zero");
-	const uint32_t gF = grid_for(F, ctx);
-	if (F > 0 && (e = launch(ctx, "build_tag_leaf", build_tag_leaf, dim3(gF), dim3(kThreads), 0, frags, F,
-	                         (const uint32_t *)oct->bm, (const uint32_t *)oct->rk, lv, reinterpret_cast<uint2 *>(oct->node),
-	                         oct->cnt)) !=
-	                 hipSuccess)
+	const uint32_t gF = grid_for(bb.Ftot, ctx);
+	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
+	if (bb.Ftot > 0 && (e = launch(ctx, "build_tag_leaf", build_tag_leaf, dim3(gF), dim3(kThreads), 0, segs,
+	                               bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk, node,
+	                               bb.cnt)) != hipSuccess)
 		return hip_check(ctx, e, "build_tag_leaf");
-	BinCounts bc{oct->cnt, nbins};
+	BinCounts bc{bb.cnt, bb.nbins};
 	if ((e = launch(ctx, "bin_reduce", scan_reduce<BinCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
 	                ctx->d_scan_parts)) != hipSuccess)
 		return hip_check(ctx, e, "bin_reduce");
@@ -619,29 +668,154 @@ int build_nodes(svo_ctx *ctx, svo_octree *oct) {
 	    hipSuccess)
 		return hip_check(ctx, e, "bin_parts");
 	if ((e = launch(ctx, "bin_offsets", scan_down<BinCounts, BinOffsets>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
-	                BinOffsets{oct->aux}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
+	                BinOffsets{bb.aux}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 		return hip_check(ctx, e, "bin_offsets");
-	if (F > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(gF), dim3(kThreads), 0, frags, F,
-	                         (const uint2 *)oct->node, (const uint32_t *)oct->aux,
-	                         reinterpret_cast<uint2 *>(oct->bucket))) != hipSuccess)
+	uint2 *bucket = reinterpret_cast<uint2 *>(bb.bucket);
+	if (bb.Ftot > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(gF), dim3(kThreads), 0, segs, bb.fbeg(),
+	                               S, bb.Ftot, (const uint2 *)node, (const uint32_t *)bb.aux, bucket)) != hipSuccess)
 		return hip_check(ctx, e, "build_scatter");
-	if ((e = launch(ctx, "list_reset", build_init_lists, dim3(1), dim3(1), 0, oct->big)) != hipSuccess)
+	if ((e = launch(ctx, "list_reset", build_init_lists, dim3(1), dim3(1), 0, bb.list)) != hipSuccess)
 		return hip_check(ctx, e, "list_reset");
-	const uint64_t cap = build_big_capacity(F);
-	const BuildState *st = ctx->d_state;
-	if ((e = launch(ctx, "build_fold", build_fold, dim3(grid_for(nbins, ctx)), dim3(kThreads), 0, oct->words,
-	                (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux, (const uint2 *)oct->bucket, oct->big, cap,
-	                st, L)) != hipSuccess)
+	const uint64_t cap = big_capacity(bb.Ftot);
+	if ((e = launch(ctx, "build_fold", build_fold, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, segs, bb.bbeg(),
+	                S, bb.nbins, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket, bb.list,
+	                cap)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold");
 	if ((e = launch(ctx, "build_fold_mid", build_fold_mid, dim3((uint32_t)ctx->cu_count * 8u), dim3(kThreads), 0,
-	                oct->words, (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux, (const uint2 *)oct->bucket,
-	                (const uint32_t *)oct->big, cap, st, L)) != hipSuccess)
+	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket,
+	                (const uint32_t *)bb.list, cap)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold_mid");
 	if ((e = launch(ctx, "build_fold_big", build_fold_big, dim3((uint32_t)ctx->cu_count * 2u), dim3(kThreads), 0,
-	                oct->words, (const uint32_t *)oct->cnt, (const uint32_t *)oct->aux,
-	                reinterpret_cast<uint2 *>(oct->bucket), (const uint32_t *)oct->big, cap, st, L)) != hipSuccess)
+	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, bucket,
+	                (const uint32_t *)bb.list, cap)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold_big");
 	return SVO_OK;
+}
+
+} // namespace
+
+uint64_t build_bitmap_words(uint32_t L) {
+	uint64_t off[16] = {};
+	levels_for(L, off);
+	return off[L];
+}
+
+void build_batch_release(svo_ctx *ctx, BuildBatch *bb) {
+	for (void *p : {(void *)bb->segs, (void *)bb->index, (void *)bb->bm, (void *)bb->rk, (void *)bb->cnt,
+	                (void *)bb->aux, (void *)bb->node, (void *)bb->bucket, (void *)bb->list})
+		ctx->pool->free(p);
+	bb->segs = nullptr;
+	bb->index = nullptr;
+	bb->bm = bb->rk = bb->cnt = bb->aux = bb->node = bb->bucket = bb->list = nullptr;
+}
+
+// The whole build of S octrees: structure phase, one read-back of every scene's W and leaf-block
+// count, allocation of the octrees, node phase.  Scratch goes back to the pool at the end; the pool
+// hands it only to later work on this stream, so no wait is needed for the node phase to finish.
+int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
+	if (S == 0)
+		return SVO_OK;
+	if (S > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 scenes");
+	BuildBatch bb;
+	bb.S = S;
+	bb.h_segs.assign(S, BuildSeg{});
+	const uint64_t stride = S + 1;
+	bb.h_index.assign((3 + 2 * (uint64_t)kMaxLevel + 1) * stride, 0);
+	uint64_t *fbeg = bb.h_index.data();
+	for (uint32_t s = 0; s < S; ++s) {
+		const svo_fraglist *fl = octs[s]->src;
+		BuildSeg &g = bb.h_segs[s];
+		g.frags = reinterpret_cast<const uint2 *>(fl->frags);
+		g.F = fl->F;
+		g.L = fl->level;
+		g.bm = bb.TW;
+		levels_for(g.L, g.off);
+		bb.TW += g.off[g.L];
+		fbeg[s] = bb.Ftot;
+		bb.Ftot += g.F;
+		bb.maxL = std::max(bb.maxL, g.L);
+	}
+	fbeg[S] = bb.Ftot;
+	if (bb.Ftot > 0xFFFFFFFEull)
+		return set_error(ctx, SVO_ERR_OVERFLOW, "a build batch holds at most 2^32-2 fragments; split the batch");
+	// per-depth begin tables: dw[d] over scenes with B[d] (L > d), dr[d] over scenes with B[d+1] (L > d+1)
+	for (uint32_t d = 1; d <= kMaxLevel; ++d) {
+		uint64_t *dw = bb.h_index.data() + (2 + d) * stride, *dr = bb.h_index.data() + (3 + kMaxLevel + d) * stride;
+		uint64_t aw = 0, ar = 0;
+		for (uint32_t s = 0; s < S; ++s) {
+			const BuildSeg &g = bb.h_segs[s];
+			dw[s] = aw;
+			dr[s] = ar;
+			const uint64_t words = g.L > d ? g.off[d + 1] - g.off[d] : 0;
+			aw += words;
+			ar += g.L > d + 1 ? words : 0;
+		}
+		dw[S] = aw;
+		dr[S] = ar;
+	}
+	const uint64_t Fa = bb.Ftot ? bb.Ftot : 1;
+	const uint64_t cap = big_capacity(bb.Ftot);
+	Pool *pool = ctx->pool;
+	int rc = SVO_OK;
+	if (pool->alloc((void **)&bb.segs, sizeof(BuildSeg) * S) != hipSuccess ||
+	    pool->alloc((void **)&bb.index, sizeof(uint64_t) * bb.h_index.size()) != hipSuccess ||
+	    pool->alloc((void **)&bb.bm, sizeof(uint32_t) * (bb.TW ? bb.TW : 1)) != hipSuccess ||
+	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW + 1)) != hipSuccess ||
+	    pool->alloc((void **)&bb.node, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
+	    pool->alloc((void **)&bb.bucket, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
+	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (2 + 2 * cap)) != hipSuccess)
+		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
+	if (!rc)
+		rc = upload_table(ctx, bb.segs, bb.h_segs.data(), sizeof(BuildSeg) * S);
+	if (!rc)
+		rc = upload_table(ctx, bb.index, bb.h_index.data(), sizeof(uint64_t) * bb.h_index.size());
+	// structure phase: bitmaps + ranks decide every W exactly; one read-back sizes the octrees
+	if (!rc) {
+		ctx->t_build.start(ctx->stream);
+		rc = structure_phase(ctx, bb);
+		ctx->t_build.stop(ctx->stream);
+	}
+	if (!rc)
+		rc = download_table(ctx, bb.h_segs.data(), bb.segs, sizeof(BuildSeg) * S);
+	if (!rc)
+		for (uint32_t s = 0; s < S; ++s)
+			if (bb.h_segs[s].overflow) {
+				rc = set_error(ctx, SVO_ERR_OVERFLOW,
+				               "octree " + std::to_string(s) + " needs more than 2^30 words (30-bit child pointers)");
+				break;
+			}
+	uint64_t *bbeg = bb.h_index.data() + stride;
+	if (!rc) {
+		for (uint32_t s = 0; s < S; ++s) {
+			BuildSeg &g = bb.h_segs[s];
+			svo_octree *o = octs[s];
+			o->W = o->cap = g.W; // (counter + 1) * 8, src/OctreeBuilder.cpp:212-214
+			if (pool->alloc((void **)&o->words, sizeof(uint32_t) * g.W) != hipSuccess) {
+				rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree allocation failed");
+				break;
+			}
+			g.words = o->words;
+			g.bin_begin = bb.nbins;
+			bbeg[s] = bb.nbins;
+			bb.nbins += g.nbins;
+		}
+		bbeg[S] = bb.nbins;
+	}
+	if (!rc && (pool->alloc((void **)&bb.cnt, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess ||
+	            pool->alloc((void **)&bb.aux, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess))
+		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
+	if (!rc)
+		rc = upload_table(ctx, bb.segs, bb.h_segs.data(), sizeof(BuildSeg) * S);
+	if (!rc)
+		rc = upload_table(ctx, bb.index, bb.h_index.data(), sizeof(uint64_t) * bb.h_index.size());
+	if (!rc) {
+		ctx->t_build2.start(ctx->stream);
+		rc = node_phase(ctx, bb);
+		ctx->t_build2.stop(ctx->stream);
+	}
+	build_batch_release(ctx, &bb);
+	return rc;
 }
 
 } // namespace svo

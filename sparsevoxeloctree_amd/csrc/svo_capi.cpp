@@ -8,6 +8,7 @@
 #include <cstring>
 #include <new>
 
+#include "svo_batch.hpp"
 #include "svo_runtime.hpp"
 
 namespace svo {
@@ -182,6 +183,75 @@ void StageTimer::destroy() {
 	a = b = nullptr;
 }
 
+// ---- Staging: pinned ring for small stream-ordered table copies
+hipError_t Staging::init() {
+	hipError_t e = hipHostMalloc((void **)&m_buf, kSlots * kSlotBytes, hipHostMallocDefault);
+	if (e != hipSuccess)
+		return e;
+	for (uint32_t i = 0; i < kSlots; ++i)
+		if ((e = hipEventCreateWithFlags(&m_ev[i], hipEventDisableTiming)) != hipSuccess)
+			return e;
+	return hipSuccess;
+}
+
+void Staging::destroy() {
+	for (uint32_t i = 0; i < kSlots; ++i)
+		if (m_ev[i]) {
+			(void)hipEventSynchronize(m_ev[i]);
+			(void)hipEventDestroy(m_ev[i]);
+			m_ev[i] = nullptr;
+		}
+	if (m_buf)
+		(void)hipHostFree(m_buf);
+	m_buf = nullptr;
+}
+
+void *Staging::acquire(size_t bytes) {
+	if (!m_buf || bytes > kSlotBytes)
+		return nullptr;
+	m_cur = m_next;
+	m_next = (m_next + 1) % kSlots;
+	if (m_armed[m_cur]) {
+		(void)hipEventSynchronize(m_ev[m_cur]);
+		m_armed[m_cur] = false;
+	}
+	return m_buf + m_cur * kSlotBytes;
+}
+
+hipError_t Staging::release(hipStream_t s) {
+	hipError_t e = hipEventRecord(m_ev[m_cur], s);
+	m_armed[m_cur] = e == hipSuccess;
+	return e;
+}
+
+int upload_table(svo_ctx *ctx, void *dst, const void *src, size_t bytes) {
+	if (bytes == 0)
+		return SVO_OK;
+	void *slot = ctx->stage.acquire(bytes);
+	if (!slot)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "table larger than the staging slot");
+	std::memcpy(slot, src, bytes);
+	hipError_t e = hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream);
+	if (e == hipSuccess)
+		e = ctx->stage.release(ctx->stream);
+	return hip_check(ctx, e, "table upload");
+}
+
+int download_table(svo_ctx *ctx, void *dst, const void *src, size_t bytes) {
+	if (bytes == 0)
+		return SVO_OK;
+	void *slot = ctx->stage.acquire(bytes);
+	if (!slot)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "table larger than the staging slot");
+	hipError_t e = hipMemcpyAsync(slot, src, bytes, hipMemcpyDeviceToHost, ctx->stream);
+	if (e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "table read-back");
+	std::memcpy(dst, slot, bytes);
+	return SVO_OK;
+}
+
 } // namespace svo
 
 using namespace svo;
@@ -202,14 +272,6 @@ struct DeviceGuard { // makes the context's device current for the call
 			(void)hipSetDevice(prev);
 	}
 };
-
-void release_build_scratch(svo_octree *o) {
-	svo_ctx *ctx = o->ctx;
-	for (uint32_t **p : {&o->bm, &o->rk, &o->lvl, &o->cnt, &o->aux, &o->node, &o->bucket, &o->big}) {
-		ctx->pool->free(*p);
-		*p = nullptr;
-	}
-}
 
 } // namespace
 
@@ -239,7 +301,8 @@ int svo_ctx_create(int device, svo_ctx **out) {
 	if (hipMalloc((void **)&ctx->d_scan_parts, sizeof(uint64_t) * (2 * kScanBlocks + 8)) != hipSuccess ||
 	    hipMalloc((void **)&ctx->d_state, sizeof(BuildState)) != hipSuccess ||
 	    hipMalloc((void **)&ctx->d_err, sizeof(uint32_t)) != hipSuccess ||
-	    hipHostMalloc((void **)&ctx->h_state, sizeof(BuildState) + 64, hipHostMallocDefault) != hipSuccess) {
+	    hipHostMalloc((void **)&ctx->h_state, sizeof(BuildState) + 64, hipHostMallocDefault) != hipSuccess ||
+	    ctx->stage.init() != hipSuccess) {
 		svo_ctx_destroy(ctx);
 		return SVO_ERR_OUT_OF_MEMORY;
 	}
@@ -271,6 +334,7 @@ int svo_ctx_destroy(svo_ctx *ctx) {
 		(void)hipFree(ctx->d_err);
 	if (ctx->h_state)
 		(void)hipHostFree(ctx->h_state);
+	ctx->stage.destroy();
 	delete ctx->pool;
 	if (ctx->stream)
 		(void)hipStreamDestroy(ctx->stream);
@@ -534,12 +598,17 @@ int svo_voxelize_count(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo
 		return set_error(ctx, SVO_ERR_BAD_LEVEL, "level must be in [1, 12]");
 	DeviceGuard g(ctx);
 	svo_fraglist *fl = new (std::nothrow) svo_fraglist();
-	if (!fl)
+	VoxBatch *vb = new (std::nothrow) VoxBatch();
+	if (!fl || !vb) {
+		delete fl;
+		delete vb;
 		return SVO_ERR_OUT_OF_MEMORY;
+	}
 	fl->ctx = ctx;
 	fl->scene = scene;
 	fl->level = level;
-	int rc = voxelize_count(ctx, fl);
+	fl->vb = vb;
+	int rc = voxelize_count_batch(ctx, &fl, 1, vb);
 	if (rc) {
 		svo_fraglist_destroy(fl);
 		return rc;
@@ -549,10 +618,16 @@ int svo_voxelize_count(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo
 }
 
 int svo_voxelize_fill(svo_ctx *ctx, svo_fraglist *fl) {
-	if (!ctx || !fl || !fl->scene || !fl->offsets)
+	if (!ctx || !fl || !fl->scene || !fl->vb)
 		return SVO_ERR_INVALID_ARGUMENT;
 	DeviceGuard g(ctx);
-	return voxelize_fill(ctx, fl);
+	int rc = voxelize_fill_batch(ctx, fl->vb);
+	vox_batch_release(ctx, fl->vb); // stream-ordered: later work only
+	delete fl->vb;
+	fl->vb = nullptr;
+	if (!rc)
+		fl->filled = true;
+	return rc;
 }
 
 int svo_voxelize(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fraglist **out) {
@@ -563,6 +638,53 @@ int svo_voxelize(svo_ctx *ctx, const svo_scene *scene, uint32_t level, svo_fragl
 	if (rc) {
 		svo_fraglist_destroy(*out);
 		*out = nullptr;
+	}
+	return rc;
+}
+
+int svo_voxelize_batch(svo_ctx *ctx, const svo_scene *const *scenes, const uint32_t *levels, uint32_t n,
+                       svo_fraglist **out) {
+	if (!ctx || (n && (!scenes || !levels || !out)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (n > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 scenes");
+	for (uint32_t i = 0; i < n; ++i) {
+		out[i] = nullptr;
+		if (!scenes[i])
+			return SVO_ERR_INVALID_ARGUMENT;
+		if (levels[i] < 1 || levels[i] > 12)
+			return set_error(ctx, SVO_ERR_BAD_LEVEL, "level must be in [1, 12]");
+	}
+	if (n == 0)
+		return SVO_OK;
+	DeviceGuard g(ctx);
+	std::vector<svo_fraglist *> fls(n, nullptr);
+	int rc = SVO_OK;
+	for (uint32_t i = 0; i < n && !rc; ++i) {
+		fls[i] = new (std::nothrow) svo_fraglist();
+		if (!fls[i]) {
+			rc = SVO_ERR_OUT_OF_MEMORY;
+			break;
+		}
+		fls[i]->ctx = ctx;
+		fls[i]->scene = scenes[i];
+		fls[i]->level = levels[i];
+	}
+	VoxBatch vb;
+	if (!rc)
+		rc = voxelize_count_batch(ctx, fls.data(), n, &vb);
+	if (!rc)
+		rc = voxelize_fill_batch(ctx, &vb);
+	vox_batch_release(ctx, &vb);
+	for (uint32_t i = 0; i < n; ++i) {
+		if (!fls[i])
+			continue;
+		if (rc) {
+			svo_fraglist_destroy(fls[i]);
+		} else {
+			fls[i]->filled = true;
+			out[i] = fls[i];
+		}
 	}
 	return rc;
 }
@@ -619,8 +741,10 @@ int svo_fraglist_destroy(svo_fraglist *fl) {
 		return SVO_ERR_INVALID_ARGUMENT;
 	DeviceGuard g(fl->ctx);
 	fl->ctx->pool->free(fl->frags);
-	fl->ctx->pool->free(fl->offsets);
-	fl->ctx->pool->free(fl->bigtri);
+	if (fl->vb) {
+		vox_batch_release(fl->ctx, fl->vb);
+		delete fl->vb;
+	}
 	delete fl;
 	return SVO_OK;
 }
@@ -639,48 +763,12 @@ int svo_build_enqueue(svo_ctx *ctx, const svo_fraglist *fl, svo_octree **out) {
 	o->ctx = ctx;
 	o->level = fl->level;
 	o->src = fl;
-	const uint64_t F = fl->F ? fl->F : 1;
-	const uint64_t TW = build_bitmap_words(fl->level);
-	if (ctx->pool->alloc((void **)&o->bm, sizeof(uint32_t) * (TW ? TW : 1)) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->rk, sizeof(uint32_t) * (TW + 1)) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->lvl, sizeof(uint32_t) * 16) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->node, sizeof(uint32_t) * 2 * F) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->bucket, sizeof(uint32_t) * 2 * F) != hipSuccess ||
-	    ctx->pool->alloc((void **)&o->big, sizeof(uint32_t) * (2 + 2 * build_big_capacity(F))) != hipSuccess) {
-		svo_octree_destroy(o);
-		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
-	}
-	// structure phase: bitmaps + ranks decide W exactly; one read-back sizes the node buffer
-	ctx->t_build.start(ctx->stream);
-	int rc = build_structure(ctx, o);
-	ctx->t_build.stop(ctx->stream);
-	hipError_t e = hipSuccess;
-	if (!rc && ((e = hipMemcpyAsync(ctx->h_state, ctx->d_state, sizeof(BuildState), hipMemcpyDeviceToHost,
-	                                ctx->stream)) != hipSuccess ||
-	            (e = hipStreamSynchronize(ctx->stream)) != hipSuccess))
-		rc = hip_check(ctx, e, "octree structure");
-	if (!rc && ctx->h_state->overflow)
-		rc = set_error(ctx, SVO_ERR_OVERFLOW, "octree needs more than 2^30 words (30-bit child pointers)");
-	if (!rc) {
-		const BuildState st = *ctx->h_state;
-		o->W = o->cap = st.need_words; // (counter + 1) * 8, src/OctreeBuilder.cpp:212-214
-		o->nbins = st.num[o->level & 1u]; // leaf blocks
-		const uint64_t nb = o->nbins ? o->nbins : 1;
-		if (ctx->pool->alloc((void **)&o->words, sizeof(uint32_t) * o->W) != hipSuccess ||
-		    ctx->pool->alloc((void **)&o->cnt, sizeof(uint32_t) * nb) != hipSuccess ||
-		    ctx->pool->alloc((void **)&o->aux, sizeof(uint32_t) * nb) != hipSuccess)
-			rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree allocation failed");
-	}
-	if (!rc) {
-		o->pending = true;
-		ctx->t_build2.start(ctx->stream);
-		rc = build_nodes(ctx, o);
-		ctx->t_build2.stop(ctx->stream);
-	}
+	int rc = build_batch(ctx, &o, 1);
 	if (rc) {
 		svo_octree_destroy(o);
 		return rc;
 	}
+	o->pending = true;
 	*out = o;
 	return SVO_OK;
 }
@@ -694,7 +782,7 @@ int svo_build_finish(svo_ctx *ctx, svo_octree *o, uint64_t *W) {
 		if (e != hipSuccess)
 			return hip_check(ctx, e, "octree build");
 		o->pending = false;
-		release_build_scratch(o);
+		o->src = nullptr;
 	}
 	if (W)
 		*W = o->W;
@@ -709,6 +797,48 @@ int svo_build(svo_ctx *ctx, const svo_fraglist *fl, svo_octree **out) {
 	if (rc) {
 		svo_octree_destroy(*out);
 		*out = nullptr;
+	}
+	return rc;
+}
+
+int svo_build_batch(svo_ctx *ctx, const svo_fraglist *const *fls, uint32_t n, svo_octree **out) {
+	if (!ctx || (n && (!fls || !out)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (n > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 scenes");
+	for (uint32_t i = 0; i < n; ++i) {
+		out[i] = nullptr;
+		if (!fls[i])
+			return SVO_ERR_INVALID_ARGUMENT;
+		if (!fls[i]->filled)
+			return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "fragment list not filled (call svo_voxelize_fill)");
+	}
+	if (n == 0)
+		return SVO_OK;
+	DeviceGuard g(ctx);
+	std::vector<svo_octree *> octs(n, nullptr);
+	int rc = SVO_OK;
+	for (uint32_t i = 0; i < n; ++i) {
+		octs[i] = new (std::nothrow) svo_octree();
+		if (!octs[i]) {
+			rc = SVO_ERR_OUT_OF_MEMORY;
+			break;
+		}
+		octs[i]->ctx = ctx;
+		octs[i]->level = fls[i]->level;
+		octs[i]->src = fls[i];
+	}
+	if (!rc)
+		rc = build_batch(ctx, octs.data(), n);
+	for (uint32_t i = 0; i < n; ++i) {
+		if (!octs[i])
+			continue;
+		if (rc) {
+			svo_octree_destroy(octs[i]);
+		} else {
+			octs[i]->pending = true;
+			out[i] = octs[i];
+		}
 	}
 	return rc;
 }
@@ -765,7 +895,6 @@ int svo_octree_destroy(svo_octree *o) {
 	DeviceGuard g(o->ctx);
 	if (o->pending)
 		(void)hipStreamSynchronize(o->ctx->stream);
-	release_build_scratch(o);
 	o->ctx->pool->free(o->words);
 	delete o;
 	return SVO_OK;
@@ -825,7 +954,7 @@ int svo_octree_load(svo_ctx *ctx, const char *path, svo_octree **out) {
 // ---- OctreeTracer
 int svo_march(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t width, uint32_t height, int beam,
               svo_hit *hits_dev, float *beam_dev) {
-	if (!ctx || !o || !cam || !hits_dev || o->pending)
+	if (!ctx || !o || !cam || !hits_dev)
 		return SVO_ERR_INVALID_ARGUMENT;
 	if (width == 0 || height == 0)
 		return SVO_OK;
@@ -837,7 +966,7 @@ int svo_march(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t
 
 int svo_march_beam(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t width, uint32_t height,
                    float *beam_dev, uint32_t *nfetch_dev) {
-	if (!ctx || !o || !cam || !beam_dev || o->pending)
+	if (!ctx || !o || !cam || !beam_dev)
 		return SVO_ERR_INVALID_ARGUMENT;
 	if (width == 0 || height == 0)
 		return SVO_OK;
@@ -849,7 +978,7 @@ int svo_march_beam(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uin
 
 int svo_march_rays(svo_ctx *ctx, const svo_octree *o, const float *rays_dev, uint64_t R, int variant,
                    svo_hit *hits_dev) {
-	if (!ctx || !o || o->pending || (R && (!rays_dev || !hits_dev)) || variant < 0 || variant > 2)
+	if (!ctx || !o || (R && (!rays_dev || !hits_dev)) || variant < 0 || variant > 2)
 		return SVO_ERR_INVALID_ARGUMENT;
 	if (R > (uint64_t)0xFFFFFFFFull * 256u)
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
@@ -864,6 +993,141 @@ int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t 
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
 	DeviceGuard g(ctx);
 	return bounce_rays(ctx, hits_dev, R, seed, rays_dev);
+}
+
+// ---- batched tracing
+namespace {
+int check_frame_batch(svo_ctx *ctx, const svo_octree *const *octs, const svo_camera *cams, uint32_t n,
+                      uint32_t width, uint32_t height) {
+	if (n && (!octs || !cams))
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (n > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 octrees");
+	for (uint32_t i = 0; i < n; ++i)
+		if (!octs[i] || !octs[i]->words)
+			return SVO_ERR_INVALID_ARGUMENT;
+	if ((uint64_t)width * height * (n ? n : 1) > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "more than 2^32-1 rays in one batch");
+	return SVO_OK;
+}
+} // namespace
+
+int svo_march_plan_create(svo_ctx *ctx, const svo_octree *const *octs, const svo_camera *cams, uint32_t n,
+                          uint32_t width, uint32_t height, int beam, svo_march_plan **out) {
+	if (!ctx || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	int rc = check_frame_batch(ctx, octs, cams, n, width, height);
+	if (rc)
+		return rc;
+	DeviceGuard g(ctx);
+	svo_march_plan *p = new (std::nothrow) svo_march_plan();
+	if (!p)
+		return SVO_ERR_OUT_OF_MEMORY;
+	p->ctx = ctx;
+	rc = march_plan_init(ctx, &p->p, octs, cams, n, width, height, beam);
+	if (rc) {
+		svo_march_plan_destroy(p);
+		return rc;
+	}
+	*out = p;
+	return SVO_OK;
+}
+
+int svo_march_plan_set_cameras(svo_ctx *ctx, svo_march_plan *plan, const svo_camera *cams) {
+	if (!ctx || !plan || plan->ctx != ctx || (plan->p.S && !cams))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	return march_plan_set_cameras(ctx, &plan->p, cams);
+}
+
+int svo_march_plan_use_graph(svo_march_plan *plan, int on) {
+	if (!plan)
+		return SVO_ERR_INVALID_ARGUMENT;
+	plan->p.use_graph = on != 0;
+	return SVO_OK;
+}
+
+int svo_march_plan_set_shape(svo_march_plan *plan, int shape) {
+	if (!plan || shape < -1 || shape > 3)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (plan->p.exec) {
+		(void)hipGraphExecDestroy(plan->p.exec); // recaptured with the new shape
+		plan->p.exec = nullptr;
+	}
+	plan->p.shape = shape;
+	return SVO_OK;
+}
+
+int svo_march_plan_run(svo_ctx *ctx, svo_march_plan *plan, svo_hit *hits_dev) {
+	if (!ctx || !plan || plan->ctx != ctx || (plan->p.S && !hits_dev))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	return march_plan_run(ctx, &plan->p, hits_dev);
+}
+
+int svo_march_plan_destroy(svo_march_plan *plan) {
+	if (!plan)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(plan->ctx);
+	march_plan_release(plan->ctx, &plan->p);
+	delete plan;
+	return SVO_OK;
+}
+
+int svo_march_batch(svo_ctx *ctx, const svo_octree *const *octs, const svo_camera *cams, uint32_t n,
+                    uint32_t width, uint32_t height, int beam, svo_hit *hits_dev) {
+	if (!ctx || (n && !hits_dev))
+		return SVO_ERR_INVALID_ARGUMENT;
+	svo_march_plan *plan = nullptr;
+	int rc = svo_march_plan_create(ctx, octs, cams, n, width, height, beam, &plan);
+	if (rc)
+		return rc;
+	rc = svo_march_plan_run(ctx, plan, hits_dev);
+	svo_march_plan_destroy(plan); // stream-ordered pool: later work only
+	return rc;
+}
+
+int svo_march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t n, const uint64_t *ray_begin,
+                         const float *rays_dev, int variant, svo_hit *hits_dev) {
+	if (!ctx || !ray_begin || (n && !octs) || variant < 0 || variant > 2)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (n > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 octrees");
+	if (ray_begin[0] != 0)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "ray_begin[0] must be 0");
+	for (uint32_t i = 0; i < n; ++i) {
+		if (!octs[i] || !octs[i]->words || ray_begin[i + 1] < ray_begin[i])
+			return SVO_ERR_INVALID_ARGUMENT;
+	}
+	const uint64_t R = ray_begin[n];
+	if (R && (!rays_dev || !hits_dev))
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (R > 0xFFFFFFFFull * 64u)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
+	DeviceGuard g(ctx);
+	return march_rays_batch(ctx, octs, n, ray_begin, rays_dev, variant, hits_dev);
+}
+
+int svo_autotune(svo_ctx *ctx, int on) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	ctx->autotune = on != 0;
+	return SVO_OK;
+}
+
+int svo_autotune_report(svo_ctx *ctx, char *buf, uint64_t cap, uint64_t *len) {
+	if (!ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	const std::string &s = ctx->tune_log;
+	if (len)
+		*len = s.size();
+	if (buf && cap) {
+		const uint64_t n = std::min<uint64_t>(cap - 1, s.size());
+		std::memcpy(buf, s.data(), n);
+		buf[n] = 0;
+	}
+	return SVO_OK;
 }
 
 // Camera::fetch_uniform_data (src/Camera.cpp:77-95): yaw about +y, then pitch about -x;

@@ -39,6 +39,11 @@
 //  ON ANY THEORY OF LIABILITY, WHETHER IN CONTRACT, STRICT LIABILITY, OR TORT
 //  (INCLUDING NEGLIGENCE OR OTHERWISE) ARISING IN ANY WAY OUT OF THE USE OF
 //  THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+
+#include "svo_batch.hpp"
 #include "svo_device.hpp"
 #include "svo_runtime.hpp"
 
@@ -101,8 +106,8 @@ template <bool kGeEps> __device__ __forceinline__ void march_init(f3 o, f3 &d, M
 }
 
 // One trip of the traversal loop (octree.glsl:86-172 / 221-304).  Returns false on break.
-// stk: this lane's LDS column (slot s at stk[s * kBlock]).
-template <bool kCoarse>
+// stk: this lane's LDS column (slot s at stk[s * kStride], kStride = threads per block).
+template <bool kCoarse, uint32_t kStride = kBlock>
 __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uint64_t W, March &m, uint32_t *stk,
                                            uint32_t &nfetch, uint32_t &err, float orig_sz, float dir_sz) {
 	if (m.cur == 0u) {
@@ -133,7 +138,7 @@ __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uin
 			return false;
 		}
 		if (tc_max < m.h)
-			stk[(m.scale - kMinScale) * kBlock] = m.parent;
+			stk[(m.scale - kMinScale) * kStride] = m.parent;
 		m.h = tc_max;
 		m.parent = m.cur & 0x3fffffffu;
 		m.idx = 0u;
@@ -176,7 +181,7 @@ __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uin
 			return false;
 		}
 		m.scale_exp2 = bitsf((m.scale - kStackSize + 127u) << 23u);
-		m.parent = stk[(m.scale - kMinScale) * kBlock];
+		m.parent = stk[(m.scale - kMinScale) * kStride];
 		uint32_t shx = fbits(m.pos.x) >> m.scale, shy = fbits(m.pos.y) >> m.scale, shz = fbits(m.pos.z) >> m.scale;
 		m.pos.x = bitsf(shx << m.scale);
 		m.pos.y = bitsf(shy << m.scale);
@@ -190,7 +195,7 @@ __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uin
 
 // Octree_RayMarchLeaf (variant 0: 6-arg with iteration count; 1: 5-arg) and
 // Octree_RayMarchOcclude (variant 2), with the outputs of octree.glsl:307-339.
-template <int kVariant>
+template <int kVariant, uint32_t kStride = kBlock>
 __device__ __forceinline__ void march_leaf(const uint32_t *__restrict__ oct, uint64_t W, f3 o, f3 d, uint32_t *stk,
                                            uint32_t &err, uint4 &out0, uint4 &out1) {
 	March m;
@@ -202,7 +207,7 @@ __device__ __forceinline__ void march_leaf(const uint32_t *__restrict__ oct, uin
 			err |= kErrIterCap;
 			break;
 		}
-		if (!march_step<false>(oct, W, m, stk, nfetch, err, 0.0f, 0.0f))
+		if (!march_step<false, kStride>(oct, W, m, stk, nfetch, err, 0.0f, 0.0f))
 			break;
 	}
 	const bool hit = m.scale < kStackSize && m.t_min <= m.t_max;
@@ -252,10 +257,6 @@ __device__ __forceinline__ void march_leaf(const uint32_t *__restrict__ oct, uin
 	out1 = make_uint4(vxy, vzf, m.cur, nfetch);
 }
 
-struct CamArgs {
-	float px, py, pz, lx, ly, lz, sx, sy, sz, ux, uy, uz;
-};
-
 // Camera_GenRay (shader/camera.glsl:11-14)
 __device__ __forceinline__ f3 gen_ray(const CamArgs &c, float cx, float cy) {
 	cx = cx * 2.0f - 1.0f;
@@ -274,19 +275,13 @@ __device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nblocks) {
 	return start + k;
 }
 
-__global__ void __launch_bounds__(kBlock)
-    march_beam_kernel(const uint32_t *__restrict__ oct, uint64_t W, CamArgs cam, uint32_t width, uint32_t height,
-                      uint32_t bw, uint32_t bh, float orig_sz, float *__restrict__ beam, uint32_t *__restrict__ nf_out,
-                      uint32_t *err_out) {
-	__shared__ uint32_t s_stack[kStackSlots * kBlock];
-	const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-	if (k >= bw * bh)
-		return;
-	const uint32_t bx = k % bw, by = k / bw;
+// One beam cell (octree_tracer_beam.frag:12-29): the distance every ray of the cell can skip.
+template <uint32_t kStride>
+__device__ __forceinline__ float beam_cell(const uint32_t *__restrict__ oct, uint64_t W, const CamArgs &cam,
+                                           uint32_t width, uint32_t height, uint32_t bx, uint32_t by, float orig_sz,
+                                           uint32_t *stk, uint32_t &err, uint32_t &nfetch) {
 	const float fx = (float)bx, fy = (float)by, fw = (float)width, fh = (float)height, bs = 8.0f;
-	// octree_tracer_beam.frag:12-29
 	f3 d = gen_ray(cam, fx / fw * bs, fy / fh * bs);
-	float tmax = 0.0f;
 	const float ox[4] = {0.5f, -0.5f, 0.5f, -0.5f}, oy[4] = {0.5f, 0.5f, -0.5f, -0.5f};
 	float tg[4];
 #pragma unroll
@@ -295,24 +290,52 @@ __global__ void __launch_bounds__(kBlock)
 		float c = dot3(d, y);
 		tg[q] = __builtin_sqrtf(1.0f - c * c) / fabsf(c);
 	}
-	tmax = gmax(gmax(tg[0], tg[1]), gmax(tg[2], tg[3]));
+	const float tmax = gmax(gmax(tg[0], tg[1]), gmax(tg[2], tg[3]));
 	const float dir_sz = 2.0f * tmax;
-	uint32_t err = 0u, nfetch = 0u;
 	March m;
 	f3 o{cam.px, cam.py, cam.pz};
 	march_init<false>(o, d, m);
-	uint32_t *stk = s_stack + threadIdx.x;
 	uint32_t iter = 0u;
 	for (;;) {
 		if (++iter > kMaxIter) {
 			err |= kErrIterCap;
 			break;
 		}
-		if (!march_step<true>(oct, W, m, stk, nfetch, err, orig_sz, dir_sz))
+		if (!march_step<true, kStride>(oct, W, m, stk, nfetch, err, orig_sz, dir_sz))
 			break;
 	}
 	const bool hit = m.scale < kStackSize && m.t_min <= m.t_max;
-	beam[k] = hit ? gmax(0.0f, m.t_min - m.scale_exp2) : 1e10f;
+	return hit ? gmax(0.0f, m.t_min - m.scale_exp2) : 1e10f;
+}
+
+// One pixel of the primary march (octree_tracer.frag:23-37): ray through the pixel corner, start
+// moved to the minimum of the 4 beam texels around it, 6-argument leaf march.
+template <uint32_t kStride>
+__device__ __forceinline__ void frame_pixel(const uint32_t *__restrict__ oct, uint64_t W, const CamArgs &cam,
+                                            uint32_t width, uint32_t height, const float *__restrict__ beam, uint32_t bw,
+                                            uint32_t px, uint32_t py, uint32_t *stk, uint32_t &err, uint4 &r0,
+                                            uint4 &r1) {
+	f3 o{cam.px, cam.py, cam.pz};
+	f3 d = gen_ray(cam, (float)px / (float)width, (float)py / (float)height);
+	if (beam) {
+		const uint32_t bx = px >> 3, by = py >> 3;
+		const float *b0 = beam + by * bw + bx, *b1 = b0 + bw;
+		float bm = gmin(gmin(b0[0], b0[1]), gmin(b1[0], b1[1]));
+		o = f3{o.x + d.x * bm, o.y + d.y * bm, o.z + d.z * bm};
+	}
+	march_leaf<0, kStride>(oct, W, o, d, stk, err, r0, r1);
+}
+
+__global__ void __launch_bounds__(kBlock)
+    march_beam_kernel(const uint32_t *__restrict__ oct, uint64_t W, CamArgs cam, uint32_t width, uint32_t height,
+                      uint32_t bw, uint32_t bh, float orig_sz, float *__restrict__ beam, uint32_t *__restrict__ nf_out,
+                      uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * kBlock];
+	const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	if (k >= bw * bh)
+		return;
+	uint32_t err = 0u, nfetch = 0u;
+	beam[k] = beam_cell<kBlock>(oct, W, cam, width, height, k % bw, k / bw, orig_sz, s_stack + threadIdx.x, err, nfetch);
 	if (nf_out)
 		nf_out[k] = nfetch;
 	if (err)
@@ -332,19 +355,79 @@ __global__ void __launch_bounds__(kBlock)
 	const uint32_t py = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
 	if (px >= width || py >= height)
 		return;
-	// octree_tracer.frag:23-32
-	f3 o{cam.px, cam.py, cam.pz};
-	f3 d = gen_ray(cam, (float)px / (float)width, (float)py / (float)height);
-	if (beam) {
-		const uint32_t bx = px >> 3, by = py >> 3;
-		const float *b0 = beam + by * bw + bx, *b1 = b0 + bw;
-		float bm = gmin(gmin(b0[0], b0[1]), gmin(b1[0], b1[1]));
-		o = f3{o.x + d.x * bm, o.y + d.y * bm, o.z + d.z * bm};
-	}
 	uint32_t err = 0u;
 	uint4 r0, r1;
-	march_leaf<0>(oct, W, o, d, s_stack + threadIdx.x, err, r0, r1);
+	frame_pixel<kBlock>(oct, W, cam, width, height, beam, bw, px, py, s_stack + threadIdx.x, err, r0, r1);
 	const uint64_t k = (uint64_t)py * width + px;
+	hits[2 * k] = r0;
+	hits[2 * k + 1] = r1;
+	if (err)
+		atomicOr(err_out, err);
+}
+
+// ---- batches: many octrees, one launch ----------------------------------------------------------
+// Beam cells of S scenes: blocks [s * bps, (s + 1) * bps) belong to scene s.
+__global__ void __launch_bounds__(kBlock)
+    march_beam_batch(const MarchSeg *__restrict__ segs, uint32_t width, uint32_t height, uint32_t bw, uint32_t bh,
+                     uint32_t bps, float *__restrict__ beam, uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * kBlock];
+	const uint32_t s = blockIdx.x / bps;
+	const uint32_t k = (blockIdx.x % bps) * kBlock + threadIdx.x;
+	if (k >= bw * bh)
+		return;
+	const MarchSeg &g = segs[s];
+	uint32_t err = 0u, nfetch = 0u;
+	beam[(uint64_t)s * bw * bh + k] =
+	    beam_cell<kBlock>(g.oct, g.W, g.cam, width, height, k % bw, k / bw, g.orig_sz, s_stack + threadIdx.x, err, nfetch);
+	if (err)
+		atomicOr(err_out, err);
+}
+
+// Frames of S scenes: (8 WX) x (8 WY)-pixel tiles, one 8x8 quad per wave; the tiles of all scenes
+// form one XCD-banded sequence, so an XCD works on few scenes at a time.
+template <uint32_t WX, uint32_t WY>
+__global__ void __launch_bounds__(64 * WX * WY)
+    march_frame_batch(const MarchSeg *__restrict__ segs, uint32_t width, uint32_t height,
+                      const float *__restrict__ beam, uint32_t bw, uint32_t bh, uint32_t tiles_x,
+                      uint32_t tiles_per_scene, uint4 *__restrict__ hits, uint32_t *err_out) {
+	constexpr uint32_t kT = 64 * WX * WY;
+	__shared__ uint32_t s_stack[kStackSlots * kT];
+	const uint32_t tile_g = xcd_tile(blockIdx.x, gridDim.x);
+	const uint32_t s = tile_g / tiles_per_scene, tile = tile_g % tiles_per_scene;
+	const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
+	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+	const uint32_t px = tx * (8u * WX) + (wave % WX) * 8u + (lane & 7u);
+	const uint32_t py = ty * (8u * WY) + (wave / WX) * 8u + (lane >> 3);
+	if (px >= width || py >= height)
+		return;
+	const MarchSeg &g = segs[s];
+	uint32_t err = 0u;
+	uint4 r0, r1;
+	frame_pixel<kT>(g.oct, g.W, g.cam, width, height, beam ? beam + (uint64_t)s * bw * bh : nullptr, bw, px, py,
+	                s_stack + threadIdx.x, err, r0, r1);
+	const uint64_t k = ((uint64_t)s * height + py) * width + px;
+	hits[2 * k] = r0;
+	hits[2 * k + 1] = r1;
+	if (err)
+		atomicOr(err_out, err);
+}
+
+// Explicit rays against S octrees: rays [rbeg[s], rbeg[s+1]) go to octree s.
+template <int kVariant, uint32_t kT>
+__global__ void __launch_bounds__(kT)
+    march_rays_batch(const MarchSeg *__restrict__ segs, const uint64_t *__restrict__ rbeg, uint32_t S,
+                     const float *__restrict__ rays, uint64_t R, uint4 *__restrict__ hits, uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * kT];
+	const uint64_t base = (uint64_t)blockIdx.x * kT, k = base + threadIdx.x;
+	if (k >= R)
+		return;
+	const uint32_t s = S == 1u ? 0u : seg_walk(rbeg, S, seg_find(rbeg, S, base), k);
+	const MarchSeg &g = segs[s];
+	const float *r = rays + 6 * k;
+	f3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
+	uint32_t err = 0u;
+	uint4 r0, r1;
+	march_leaf<kVariant, kT>(g.oct, g.W, o, d, s_stack + threadIdx.x, err, r0, r1);
 	hits[2 * k] = r0;
 	hits[2 * k + 1] = r1;
 	if (err)
@@ -532,6 +615,252 @@ int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, fl
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "bounce");
 	return SVO_OK;
+}
+
+// ---- batched marches ----------------------------------------------------------------------------
+namespace {
+
+constexpr int kFrameShapes = 4;
+const char *const kFrameShapeName[kFrameShapes] = {"16x16", "8x8", "16x8", "32x8"};
+constexpr uint32_t kShapeWX[kFrameShapes] = {2, 1, 2, 4};
+constexpr uint32_t kShapeWY[kFrameShapes] = {2, 1, 1, 1};
+constexpr int kRayShapes = 3;
+constexpr uint32_t kRayBlock[kRayShapes] = {256, 128, 64};
+
+MarchSeg march_seg(const svo_octree *o, const svo_camera *cam) {
+	MarchSeg g{};
+	g.oct = o->words;
+	g.W = o->W;
+	if (cam)
+		g.cam = cam_args(cam);
+	g.orig_sz = ldexpf(1.0f, 1 - (int)o->level); // exp2f(1.0f - level), src/OctreeTracer.cpp:206
+	g.level = o->level;
+	return g;
+}
+
+// Enqueues a plan's kernels with frame launch shape `shape`; prof = false bypasses the per-kernel
+// profiler (graph capture, tuning).
+hipError_t plan_enqueue(svo_ctx *ctx, const MarchPlan &p, int shape, svo_hit *hits, bool prof) {
+	const bool saved = ctx->prof.enabled;
+	if (!prof)
+		ctx->prof.enabled = false;
+	hipError_t e = hipSuccess;
+	const MarchSeg *segs = p.segs;
+	if (p.beam) {
+		const uint32_t bps = div_up((uint64_t)p.bw * p.bh, kBlock);
+		e = launch(ctx, "march_beam_batch", march_beam_batch, dim3(bps * p.S), dim3(kBlock), 0, segs, p.width,
+		           p.height, p.bw, p.bh, bps, p.beam_buf, ctx->d_err);
+	}
+	if (e == hipSuccess) {
+		const uint32_t WX = kShapeWX[shape], WY = kShapeWY[shape];
+		const uint32_t tiles_x = div_up(p.width, 8 * WX), tiles_y = div_up(p.height, 8 * WY);
+		const uint32_t tps = tiles_x * tiles_y;
+		const dim3 grid(tps * p.S), block(64 * WX * WY);
+		const float *bb = p.beam ? p.beam_buf : nullptr;
+		uint4 *out = reinterpret_cast<uint4 *>(hits);
+		switch (shape) {
+		case 0:
+			e = launch(ctx, "march_frame_batch", march_frame_batch<2, 2>, grid, block, 0, segs, p.width, p.height, bb,
+			           p.bw, p.bh, tiles_x, tps, out, ctx->d_err);
+			break;
+		case 1:
+			e = launch(ctx, "march_frame_batch", march_frame_batch<1, 1>, grid, block, 0, segs, p.width, p.height, bb,
+			           p.bw, p.bh, tiles_x, tps, out, ctx->d_err);
+			break;
+		case 2:
+			e = launch(ctx, "march_frame_batch", march_frame_batch<2, 1>, grid, block, 0, segs, p.width, p.height, bb,
+			           p.bw, p.bh, tiles_x, tps, out, ctx->d_err);
+			break;
+		default:
+			e = launch(ctx, "march_frame_batch", march_frame_batch<4, 1>, grid, block, 0, segs, p.width, p.height, bb,
+			           p.bw, p.bh, tiles_x, tps, out, ctx->d_err);
+			break;
+		}
+	}
+	ctx->prof.enabled = saved;
+	return e;
+}
+
+// Times fn(shape) for every candidate (one warm-up, best of two timed runs) and returns the fastest;
+// the decision is kept per key and logged.
+template <class Fn>
+int tune(svo_ctx *ctx, const std::string &key, int nshapes, const char *const *names, Fn fn, int &choice) {
+	auto it = ctx->tune_choice.find(key);
+	if (it != ctx->tune_choice.end()) {
+		choice = it->second;
+		return SVO_OK;
+	}
+	hipEvent_t a = nullptr, b = nullptr;
+	hipError_t e = hipEventCreate(&a);
+	if (e == hipSuccess)
+		e = hipEventCreate(&b);
+	float best = 3.4e38f;
+	int bi = 0;
+	std::string line = key + ":";
+	for (int sh = 0; sh < nshapes && e == hipSuccess; ++sh) {
+		if ((e = fn(sh)) != hipSuccess)
+			break;
+		float tmin = 3.4e38f;
+		for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+			if ((e = hipEventRecord(a, ctx->stream)) != hipSuccess || (e = fn(sh)) != hipSuccess ||
+			    (e = hipEventRecord(b, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess)
+				break;
+			float ms = 0.0f;
+			if ((e = hipEventElapsedTime(&ms, a, b)) == hipSuccess)
+				tmin = std::min(tmin, ms);
+		}
+		char buf[64];
+		std::snprintf(buf, sizeof buf, " %s %.4f ms", names[sh], (double)tmin);
+		line += buf;
+		if (tmin < best) {
+			best = tmin;
+			bi = sh;
+		}
+	}
+	if (a)
+		(void)hipEventDestroy(a);
+	if (b)
+		(void)hipEventDestroy(b);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "autotune");
+	ctx->tune_choice[key] = bi;
+	ctx->tune_log += line + " -> " + names[bi] + "\n";
+	choice = bi;
+	return SVO_OK;
+}
+
+} // namespace
+
+int march_plan_init(svo_ctx *ctx, MarchPlan *p, const svo_octree *const *octs, const svo_camera *cams, uint32_t S,
+                    uint32_t width, uint32_t height, int beam) {
+	p->S = S;
+	p->width = width;
+	p->height = height;
+	p->beam = beam ? 1 : 0;
+	p->bw = (width + 7u) / 8u + 1u; // src/OctreeTracer.cpp:6
+	p->bh = (height + 7u) / 8u + 1u;
+	p->h_segs.resize(S);
+	for (uint32_t s = 0; s < S; ++s)
+		p->h_segs[s] = march_seg(octs[s], &cams[s]);
+	if (ctx->pool->alloc((void **)&p->segs, sizeof(MarchSeg) * S) != hipSuccess ||
+	    (p->beam && ctx->pool->alloc((void **)&p->beam_buf, sizeof(float) * S * (uint64_t)p->bw * p->bh) != hipSuccess))
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "march plan allocation failed");
+	return upload_table(ctx, p->segs, p->h_segs.data(), sizeof(MarchSeg) * S);
+}
+
+int march_plan_set_cameras(svo_ctx *ctx, MarchPlan *p, const svo_camera *cams) {
+	for (uint32_t s = 0; s < p->S; ++s)
+		p->h_segs[s].cam = cam_args(&cams[s]);
+	return upload_table(ctx, p->segs, p->h_segs.data(), sizeof(MarchSeg) * p->S);
+}
+
+int march_plan_run(svo_ctx *ctx, MarchPlan *p, svo_hit *hits) {
+	if (p->S == 0 || p->width == 0 || p->height == 0)
+		return SVO_OK;
+	if (p->shape < 0) {
+		p->shape = 0;
+		if (ctx->autotune) {
+			char key[96];
+			std::snprintf(key, sizeof key, "march_frame_batch S=%u %ux%u beam=%d", p->S, p->width, p->height, p->beam);
+			int rc = tune(
+			    ctx, key, kFrameShapes, kFrameShapeName,
+			    [&](int sh) { return plan_enqueue(ctx, *p, sh, hits, false); }, p->shape);
+			if (rc)
+				return rc;
+		}
+	}
+	hipError_t e = hipSuccess;
+	ctx->t_march.start(ctx->stream);
+	if (p->use_graph) {
+		if (!p->exec || p->exec_hits != hits) {
+			if (p->exec) {
+				(void)hipGraphExecDestroy(p->exec);
+				p->exec = nullptr;
+			}
+			hipGraph_t g = nullptr;
+			e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+			if (e == hipSuccess) {
+				const hipError_t e2 = plan_enqueue(ctx, *p, p->shape, hits, false);
+				e = hipStreamEndCapture(ctx->stream, &g);
+				if (e == hipSuccess)
+					e = e2;
+			}
+			if (e == hipSuccess)
+				e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+			if (g)
+				(void)hipGraphDestroy(g);
+			p->exec_hits = e == hipSuccess ? hits : nullptr;
+		}
+		if (e == hipSuccess)
+			e = hipGraphLaunch(p->exec, ctx->stream);
+	} else {
+		e = plan_enqueue(ctx, *p, p->shape, hits, true);
+	}
+	ctx->t_march.stop(ctx->stream);
+	return hip_check(ctx, e, "march batch");
+}
+
+void march_plan_release(svo_ctx *ctx, MarchPlan *p) {
+	if (p->exec)
+		(void)hipGraphExecDestroy(p->exec);
+	p->exec = nullptr;
+	ctx->pool->free(p->segs);
+	ctx->pool->free(p->beam_buf);
+	p->segs = nullptr;
+	p->beam_buf = nullptr;
+}
+
+int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, const uint64_t *ray_begin,
+                     const float *rays, int variant, svo_hit *hits) {
+	const uint64_t R = ray_begin[S];
+	if (S == 0 || R == 0)
+		return SVO_OK;
+	std::vector<MarchSeg> h(S);
+	for (uint32_t s = 0; s < S; ++s)
+		h[s] = march_seg(octs[s], nullptr);
+	MarchSeg *segs = nullptr;
+	uint64_t *rbeg = nullptr;
+	if (ctx->pool->alloc((void **)&segs, sizeof(MarchSeg) * S) != hipSuccess ||
+	    ctx->pool->alloc((void **)&rbeg, sizeof(uint64_t) * (S + 1)) != hipSuccess) {
+		ctx->pool->free(segs);
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "ray batch allocation failed");
+	}
+	int rc = upload_table(ctx, segs, h.data(), sizeof(MarchSeg) * S);
+	if (!rc)
+		rc = upload_table(ctx, rbeg, ray_begin, sizeof(uint64_t) * (S + 1));
+	uint4 *out = reinterpret_cast<uint4 *>(hits);
+	auto run = [&](int sh) -> hipError_t {
+		const uint32_t bs = kRayBlock[sh];
+		const dim3 grid(div_up(R, bs)), block(bs);
+#define SVO_RAYS(V, T)                                                                                                 \
+	launch(ctx, "march_rays_batch", march_rays_batch<V, T>, grid, block, 0, (const MarchSeg *)segs,                    \
+	       (const uint64_t *)rbeg, S, rays, R, out, ctx->d_err)
+		if (variant == SVO_MARCH_LEAF_ITER)
+			return bs == 256 ? SVO_RAYS(0, 256) : bs == 128 ? SVO_RAYS(0, 128) : SVO_RAYS(0, 64);
+		if (variant == SVO_MARCH_LEAF)
+			return bs == 256 ? SVO_RAYS(1, 256) : bs == 128 ? SVO_RAYS(1, 128) : SVO_RAYS(1, 64);
+		return bs == 256 ? SVO_RAYS(2, 256) : bs == 128 ? SVO_RAYS(2, 128) : SVO_RAYS(2, 64);
+#undef SVO_RAYS
+	};
+	int shape = 0;
+	if (!rc && ctx->autotune) {
+		uint32_t lg = 0;
+		while ((1ull << (lg + 1)) <= R)
+			++lg;
+		char key[96];
+		std::snprintf(key, sizeof key, "march_rays_batch variant=%d R~2^%u", variant, lg);
+		static const char *const names[kRayShapes] = {"block256", "block128", "block64"};
+		rc = tune(ctx, key, kRayShapes, names, run, shape);
+	}
+	if (!rc) {
+		ctx->t_rays.start(ctx->stream);
+		const hipError_t e = run(shape);
+		ctx->t_rays.stop(ctx->stream);
+		rc = hip_check(ctx, e, "march_rays_batch");
+	}
+	ctx->pool->free(segs);
+	ctx->pool->free(rbeg);
+	return rc;
 }
 
 } // namespace svo

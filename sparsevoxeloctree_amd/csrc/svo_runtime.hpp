@@ -81,6 +81,27 @@ private:
 	hipEvent_t get_event();
 };
 
+// Pinned host staging for small tables copied to or from the device in stream order (segment
+// descriptors, per-scene results).  A ring of slots, each guarded by an event recorded after its
+// last copy, so a slot is reused only once that copy has executed.
+class Staging {
+public:
+	static constexpr uint32_t kSlots = 8;
+	static constexpr size_t kSlotBytes = size_t(512) << 10;
+	hipError_t init();
+	void destroy();
+	// Next free slot (waits for its previous copy); null when bytes exceed a slot.
+	void *acquire(size_t bytes);
+	// Marks the slot returned by the last acquire as in use by work enqueued on s so far.
+	hipError_t release(hipStream_t s);
+
+private:
+	uint8_t *m_buf = nullptr;
+	hipEvent_t m_ev[kSlots] = {};
+	bool m_armed[kSlots] = {};
+	uint32_t m_next = 0, m_cur = 0;
+};
+
 // Stage timer: a pair of events around a stage, read lazily.
 struct StageTimer {
 	hipEvent_t a = nullptr, b = nullptr;
@@ -105,8 +126,18 @@ struct svo_ctx {
 	uint32_t *d_err = nullptr;
 	svo::BuildState *h_state = nullptr; // pinned mirror for read-back
 	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_build2, t_beam, t_march, t_rays, t_bounce;
+	svo::Staging stage;
 	int cu_count = 256;
+	// launch-shape autotuner: off by default; when on, a batched launch of a new shape class times
+	// its candidate shapes once and keeps the fastest
+	bool autotune = false;
+	std::map<std::string, int> tune_choice;
+	std::string tune_log;
 };
+
+namespace svo {
+struct VoxBatch;
+}
 
 struct svo_scene {
 	svo_ctx *ctx;
@@ -121,8 +152,7 @@ struct svo_fraglist {
 	uint32_t level;
 	uint64_t F;
 	uint32_t *frags;   // 2 * F words
-	uint32_t *offsets; // per-triangle exclusive offsets from the count pass (T), or null
-	uint32_t *bigtri;  // count + indices of the triangles with many fragments (1 + T), or null
+	svo::VoxBatch *vb; // count-pass state until the fill (single-scene voxelize), or null
 	bool filled;
 };
 
@@ -132,18 +162,8 @@ struct svo_octree {
 	uint64_t W;        // used words
 	uint64_t cap;      // words allocated
 	uint32_t *words;
-	// build-in-flight resources
-	const svo_fraglist *src;
-	uint32_t *bm;      // occupancy bitmaps of depths 1..L-1
-	uint32_t *rk;      // exclusive popcount prefix of bm (+ total)
-	uint32_t *lvl;     // first block of every depth
-	uint64_t nbins;    // leaf blocks (nodes of depth L-1)
-	uint32_t *cnt;     // nbins words: fragments per leaf block
-	uint32_t *aux;     // nbins words: bucket offsets per leaf block
-	uint32_t *node;    // 2 * F words: {leaf block, place in its bucket} of each fragment
-	uint32_t *bucket;  // 2 * F words: fragments grouped by leaf block
-	uint32_t *big;     // work lists of leaf blocks with 17..64 and > 64 fragments (2 + 2 * (F / 17 + 1) words)
-	bool pending;
+	const svo_fraglist *src; // fragment list of an enqueued build, or null
+	bool pending;            // build enqueued, not yet finished
 };
 
 namespace svo {
@@ -170,14 +190,14 @@ inline hipError_t launch(svo_ctx *ctx, const char *name, Kernel k, dim3 grid, di
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Stream-ordered copy of a small host table into device memory through the staging ring.
+int upload_table(svo_ctx *ctx, void *dst, const void *src, size_t bytes);
+// Copy of a small device table to the host; waits for the stream.
+int download_table(svo_ctx *ctx, void *dst, const void *src, size_t bytes);
+
 // Entry points implemented per stage.
 int normalize_scene(svo_ctx *ctx, svo_scene *scene);
-int voxelize_count(svo_ctx *ctx, svo_fraglist *fl);
-int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl);
 uint64_t build_bitmap_words(uint32_t level);
-uint64_t build_big_capacity(uint64_t F); // entries of each large-bin work list
-int build_structure(svo_ctx *ctx, svo_octree *oct);
-int build_nodes(svo_ctx *ctx, svo_octree *oct);
 int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
                 svo_hit *hits, float *beam_out);
 int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam,

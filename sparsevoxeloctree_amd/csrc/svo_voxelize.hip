@@ -12,9 +12,14 @@
 //     clamps to [0,res-1] and to the depth range, swizzle back, fragment packing.
 // Two passes like the reference's count-only + fill draws: vox_count writes the per-triangle count,
 // a device scan turns counts into offsets, vox_fill writes each triangle's fragments at its offset
-// (triangle order, row-major inside a triangle).
+// (triangle order, row-major inside a triangle).  Both passes cover a batch of scenes in one launch
+// (segment tables, svo_batch.hpp): the triangles of all scenes form one index space, and every
+// scene's fragments land in its own list.
 //
 // Scene normalisation (src/Scene.cpp:62-99) runs here too: bounding-box reduction + transform.
+#include <string>
+
+#include "svo_batch.hpp"
 #include "svo_device.hpp"
 #include "svo_runtime.hpp"
 #include "svo_scan.hpp"
@@ -144,17 +149,29 @@ __device__ __forceinline__ void row_span(const Setup &s, int32_t j, int32_t &i0,
 
 __global__ void vox_zero_counter(uint32_t *c) { c[0] = 0u; }
 
+// Scene of triangle t; base is the block-uniform first item of the loop step.
+__device__ __forceinline__ uint32_t tri_seg(const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t base, uint64_t t) {
+	if (S == 1u)
+		return 0u;
+	return seg_walk(tbeg, S, seg_find(tbeg, S, base), t);
+}
+
 // Count pass.  Triangles with more than kBigTri fragments are also appended to `big` (big[0] is
 // the count) for vox_fill_big; the per-lane fill skips them.
 constexpr uint32_t kBigTri = 128;
 
 __global__ void __launch_bounds__(kThreads)
-    vox_count(const float *__restrict__ tri, uint64_t T, uint32_t res, uint32_t *__restrict__ counts,
-              uint32_t *__restrict__ big) {
+    vox_count(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
+              uint64_t *__restrict__ counts, uint32_t *__restrict__ big) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x; t < T; t += stride) {
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < T; base += stride) {
+		const uint64_t t = base + threadIdx.x;
+		if (t >= T)
+			continue;
+		const uint32_t sg = tri_seg(tbeg, S, base, t);
+		const VoxSeg &g = segs[sg];
 		Setup s;
-		tri_setup(tri, t, res, s);
+		tri_setup(g.xyz, t - tbeg[sg], 1u << g.level, s);
 		uint32_t c = 0;
 		if (!s.empty)
 			for (int32_t j = s.jlo; j <= s.jhi; ++j) {
@@ -167,6 +184,18 @@ __global__ void __launch_bounds__(kThreads)
 		if (c > kBigTri)
 			big[1 + atomicAdd(&big[0], 1u)] = (uint32_t)t;
 	}
+}
+
+// Per scene: first fragment and fragment count from the scanned offsets (total = all fragments).
+__global__ void vox_seg_totals(VoxSeg *segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
+                               const uint64_t *__restrict__ offsets, const uint64_t *__restrict__ total) {
+	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
+		return;
+	const uint64_t b = tbeg[s] < T ? offsets[tbeg[s]] : *total;
+	const uint64_t e = tbeg[s + 1u] < T ? offsets[tbeg[s + 1u]] : *total;
+	segs[s].frag_begin = b;
+	segs[s].F = e - b;
 }
 
 // One fragment: depth at the pixel centre, clamps, swizzle back (voxelizer.frag:17-25) and packing
@@ -188,25 +217,35 @@ __device__ __forceinline__ uint2 make_frag(const Setup &s, int32_t i, int32_t j,
 }
 
 // Fill of the small triangles: each lane walks its triangle's rows and writes its fragments at the
-// triangle's offset (triangle order, row-major inside the triangle).
+// triangle's offset (triangle order, row-major inside the triangle).  Offsets are global over the
+// batch; Ftot is the batch's fragment total.
 __global__ void __launch_bounds__(kThreads)
-    vox_fill(const float *__restrict__ tri, const uint32_t *__restrict__ rgb, uint64_t T, uint32_t res,
-             const uint32_t *__restrict__ offsets, uint64_t F, uint2 *__restrict__ frags) {
+    vox_fill(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
+             const uint64_t *__restrict__ offsets, uint64_t Ftot) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x; t < T; t += stride) {
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < T; base += stride) {
+		const uint64_t t = base + threadIdx.x;
+		if (t >= T)
+			continue;
 		uint64_t k = offsets[t];
-		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : F;
+		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : Ftot;
 		if (k == kend || kend - k > kBigTri)
 			continue;
+		const uint32_t sg = tri_seg(tbeg, S, base, t);
+		const VoxSeg &g = segs[sg];
+		const uint64_t tl = t - tbeg[sg];
+		const uint32_t res = 1u << g.level;
+		uint2 *frags = g.frags;
+		const uint64_t fb = g.frag_begin; // offsets are global over the batch
 		Setup s;
-		tri_setup(tri, t, res, s);
-		const uint32_t colour = rgb[t] & 0x00ffffffu;
+		tri_setup(g.xyz, tl, res, s);
+		const uint32_t colour = g.rgb[tl] & 0x00ffffffu;
 		for (int32_t j = s.jlo; j <= s.jhi && k < kend; ++j) {
 			int32_t i0, i1;
 			row_span(s, j, i0, i1);
 			const float dy = s.dzdy * (((float)j + 0.5f) - s.py0);
 			for (int32_t i = i0; i <= i1 && k < kend; ++i)
-				frags[k++] = make_frag(s, i, j, dy, res, colour);
+				frags[(k++) - fb] = make_frag(s, i, j, dy, res, colour);
 		}
 	}
 }
@@ -219,9 +258,8 @@ __global__ void __launch_bounds__(kThreads)
 constexpr int kRowChunk = 1024;
 
 __global__ void __launch_bounds__(kThreads)
-    vox_fill_big(const float *__restrict__ tri, const uint32_t *__restrict__ rgb, uint64_t T, uint32_t res,
-                 const uint32_t *__restrict__ offsets, const uint32_t *__restrict__ big, uint64_t F,
-                 uint2 *__restrict__ frags) {
+    vox_fill_big(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
+                 const uint64_t *__restrict__ offsets, const uint32_t *__restrict__ big, uint64_t Ftot) {
 	__shared__ int32_t s_i0[kRowChunk];
 	__shared__ uint32_t s_off[kRowChunk + 1];
 	__shared__ uint32_t s_wave[kThreads / 64];
@@ -230,11 +268,17 @@ __global__ void __launch_bounds__(kThreads)
 		const uint64_t t = big[1 + b];
 		if (t >= T)
 			continue;
+		const uint32_t sg = S == 1u ? 0u : seg_find(tbeg, S, t);
+		const VoxSeg &g = segs[sg];
+		const uint64_t tl = t - tbeg[sg];
+		const uint32_t res = 1u << g.level;
+		uint2 *frags = g.frags;
+		const uint64_t fb = g.frag_begin; // offsets are global over the batch
 		Setup s;
-		tri_setup(tri, t, res, s);
-		const uint32_t colour = rgb[t] & 0x00ffffffu;
+		tri_setup(g.xyz, tl, res, s);
+		const uint32_t colour = g.rgb[tl] & 0x00ffffffu;
 		uint64_t k = offsets[t];
-		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : F;
+		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : Ftot;
 		for (int32_t r0 = s.jlo; r0 <= s.jhi && k < kend; r0 += kRowChunk) {
 			const int32_t nrows = (s.jhi - r0 + 1) < kRowChunk ? (s.jhi - r0 + 1) : kRowChunk;
 			// spans and counts, kRowChunk / kThreads rows per thread
@@ -293,7 +337,7 @@ __global__ void __launch_bounds__(kThreads)
 				const int32_t j = r0 + lo;
 				const int32_t i = s_i0[lo] + (int32_t)(p - s_off[lo]);
 				const float dy = s.dzdy * (((float)j + 0.5f) - s.py0);
-				frags[k + p] = make_frag(s, i, j, dy, res, colour);
+				frags[k + p - fb] = make_frag(s, i, j, dy, res, colour);
 			}
 			k += total;
 			__syncthreads();
@@ -302,19 +346,19 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 struct CountSrc {
-	const uint32_t *counts;
+	const uint64_t *counts;
 	uint64_t n;
 	__device__ uint64_t size() const { return n; }
-	__device__ uint32_t operator()(uint64_t i) const { return counts[i]; }
+	__device__ uint32_t operator()(uint64_t i) const { return (uint32_t)counts[i]; }
 };
 struct TotalCommit {
 	uint64_t *total;
 	__device__ void operator()(uint64_t t) const { *total = t; }
 };
 struct OffsetDst {
-	uint32_t *offsets;
+	uint64_t *offsets;
 	__device__ bool active() const { return true; }
-	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { offsets[i] = (uint32_t)k; }
+	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { offsets[i] = k; }
 };
 
 // ---- normalisation (src/Scene.cpp:62-99) -------------------------------------------------------
@@ -417,26 +461,62 @@ int normalize_scene(svo_ctx *ctx, svo_scene *scene) {
 	return SVO_OK;
 }
 
-// Count pass + scan + read-back of F + allocation of the list (Voxelizer::count_and_create_fragment_list).
-int voxelize_count(svo_ctx *ctx, svo_fraglist *fl) {
-	const svo_scene *sc = fl->scene;
-	const uint64_t T = sc->T;
-	const uint32_t res = 1u << fl->level;
-	hipError_t e;
+void vox_batch_release(svo_ctx *ctx, VoxBatch *vb) {
+	for (void *p : {(void *)vb->segs, (void *)vb->tbeg, (void *)vb->offsets, (void *)vb->big})
+		ctx->pool->free(p);
+	vb->segs = nullptr;
+	vb->tbeg = nullptr;
+	vb->offsets = nullptr;
+	vb->big = nullptr;
+}
+
+// Count pass over a batch + scan + one read-back of every scene's F + allocation of the lists
+// (Voxelizer::count_and_create_fragment_list, src/Voxelizer.cpp:134-179).
+int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, VoxBatch *vb) {
+	if (S == 0)
+		return SVO_OK;
+	if (S > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 scenes");
+	vb->S = S;
+	vb->h_segs.assign(S, VoxSeg{});
+	vb->h_tbeg.assign(S + 1, 0);
+	uint64_t T = 0;
+	for (uint32_t s = 0; s < S; ++s) {
+		const svo_scene *sc = fls[s]->scene;
+		VoxSeg &g = vb->h_segs[s];
+		g.xyz = sc->xyz;
+		g.rgb = sc->rgb;
+		g.T = sc->T;
+		g.level = fls[s]->level;
+		vb->h_tbeg[s] = T;
+		T += sc->T;
+	}
+	vb->h_tbeg[S] = T;
+	vb->Ttot = T;
 	if (T > 0xFFFFFFFFull)
-		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 triangles");
-	if ((e = ctx->pool->alloc((void **)&fl->offsets, sizeof(uint32_t) * (T ? T : 1))) != hipSuccess ||
-	    (e = ctx->pool->alloc((void **)&fl->bigtri, sizeof(uint32_t) * (T + 1))) != hipSuccess)
-		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "offsets allocation failed");
+		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 triangles in one voxelize call");
+	Pool *pool = ctx->pool;
+	hipError_t e;
+	if (pool->alloc((void **)&vb->segs, sizeof(VoxSeg) * S) != hipSuccess ||
+	    pool->alloc((void **)&vb->tbeg, sizeof(uint64_t) * (S + 1)) != hipSuccess ||
+	    pool->alloc((void **)&vb->offsets, sizeof(uint64_t) * (T ? T : 1)) != hipSuccess ||
+	    pool->alloc((void **)&vb->big, sizeof(uint32_t) * (T + 1)) != hipSuccess)
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "count-pass allocation failed");
+	int rc = upload_table(ctx, vb->segs, vb->h_segs.data(), sizeof(VoxSeg) * S);
+	if (!rc)
+		rc = upload_table(ctx, vb->tbeg, vb->h_tbeg.data(), sizeof(uint64_t) * (S + 1));
+	if (rc)
+		return rc;
+	uint64_t *total = ctx->d_scan_parts + 2 * kScanBlocks;
 	ctx->t_count.start(ctx->stream);
-	if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0, fl->bigtri)) != hipSuccess)
+	if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0, vb->big)) != hipSuccess)
 		return hip_check(ctx, e, "vox_zero");
 	if (T > 0) {
 		if ((e = launch(ctx, "vox_count", vox_count, dim3(grid_for(T, ctx)), dim3(kThreads), 0,
-		                (const float *)sc->xyz, T, res, fl->offsets, fl->bigtri)) != hipSuccess)
+		                (const VoxSeg *)vb->segs, (const uint64_t *)vb->tbeg, S, T, vb->offsets, vb->big)) !=
+		    hipSuccess)
 			return hip_check(ctx, e, "vox_count");
-		CountSrc src{fl->offsets, T};
-		uint64_t *total = ctx->d_scan_parts + 2 * kScanBlocks;
+		CountSrc src{vb->offsets, T};
 		if ((e = launch(ctx, "vox_scan_reduce", scan_reduce<CountSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "vox_scan_reduce");
@@ -444,46 +524,53 @@ int voxelize_count(svo_ctx *ctx, svo_fraglist *fl) {
 		                TotalCommit{total})) != hipSuccess)
 			return hip_check(ctx, e, "vox_scan_parts");
 		if ((e = launch(ctx, "vox_scan_down", scan_down<CountSrc, OffsetDst>, dim3(kScanBlocks), dim3(kScanThreads),
-		                0, src, OffsetDst{fl->offsets}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
+		                0, src, OffsetDst{vb->offsets}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "vox_scan_down");
-		uint64_t F = 0;
-		if ((e = hipMemcpyAsync(&ctx->h_state->need_words, total, sizeof(uint64_t), hipMemcpyDeviceToHost,
-		                        ctx->stream)) != hipSuccess)
-			return hip_check(ctx, e, "read fragment count");
-		ctx->t_count.stop(ctx->stream);
-		if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
-			return hip_check(ctx, e, "count pass");
-		F = ctx->h_state->need_words;
-		if (F > 0xFFFFFFFFull)
-			return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 fragments");
-		fl->F = F;
-	} else {
-		ctx->t_count.stop(ctx->stream);
-		fl->F = 0;
+	} else if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0,
+	                       reinterpret_cast<uint32_t *>(total))) != hipSuccess ||
+	           (e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0,
+	                       reinterpret_cast<uint32_t *>(total) + 1)) != hipSuccess) {
+		return hip_check(ctx, e, "vox_zero");
 	}
-	if ((e = ctx->pool->alloc((void **)&fl->frags, sizeof(uint32_t) * 2 * (fl->F ? fl->F : 1))) != hipSuccess)
-		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "fragment list allocation failed");
-	return SVO_OK;
+	if ((e = launch(ctx, "vox_seg_totals", vox_seg_totals, dim3(div_up(S, 64)), dim3(64), 0, vb->segs,
+	                (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets, (const uint64_t *)total)) !=
+	    hipSuccess)
+		return hip_check(ctx, e, "vox_seg_totals");
+	ctx->t_count.stop(ctx->stream);
+	if ((rc = download_table(ctx, vb->h_segs.data(), vb->segs, sizeof(VoxSeg) * S)) != SVO_OK)
+		return rc;
+	for (uint32_t s = 0; s < S; ++s) {
+		VoxSeg &g = vb->h_segs[s];
+		if (g.F > 0xFFFFFFFFull)
+			return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 fragments in scene " + std::to_string(s));
+		svo_fraglist *fl = fls[s];
+		fl->F = g.F;
+		if (pool->alloc((void **)&fl->frags, sizeof(uint32_t) * 2 * (g.F ? g.F : 1)) != hipSuccess)
+			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "fragment list allocation failed");
+		g.frags = reinterpret_cast<uint2 *>(fl->frags);
+	}
+	return upload_table(ctx, vb->segs, vb->h_segs.data(), sizeof(VoxSeg) * S);
 }
 
-int voxelize_fill(svo_ctx *ctx, svo_fraglist *fl) {
-	const svo_scene *sc = fl->scene;
-	const uint64_t T = sc->T;
+// Fill pass of a counted batch (CmdVoxelize, src/Voxelizer.cpp:166-179).
+int voxelize_fill_batch(svo_ctx *ctx, VoxBatch *vb) {
+	const uint64_t T = vb->Ttot;
+	const uint32_t S = vb->S;
+	uint64_t Ftot = 0;
+	for (uint32_t s = 0; s < S; ++s)
+		Ftot += vb->h_segs[s].F;
 	hipError_t e;
 	ctx->t_fill.start(ctx->stream);
-	if (T > 0 && fl->F > 0) {
-		if ((e = launch(ctx, "vox_fill", vox_fill, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const float *)sc->xyz,
-		                (const uint32_t *)sc->rgb, T, 1u << fl->level, (const uint32_t *)fl->offsets, fl->F,
-		                reinterpret_cast<uint2 *>(fl->frags))) != hipSuccess)
+	if (T > 0 && Ftot > 0) {
+		if ((e = launch(ctx, "vox_fill", vox_fill, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const VoxSeg *)vb->segs,
+		                (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets, Ftot)) != hipSuccess)
 			return hip_check(ctx, e, "vox_fill");
 		if ((e = launch(ctx, "vox_fill_big", vox_fill_big, dim3((uint32_t)ctx->cu_count * 4u), dim3(kThreads), 0,
-		                (const float *)sc->xyz, (const uint32_t *)sc->rgb, T, 1u << fl->level,
-		                (const uint32_t *)fl->offsets, (const uint32_t *)fl->bigtri, fl->F,
-		                reinterpret_cast<uint2 *>(fl->frags))) != hipSuccess)
+		                (const VoxSeg *)vb->segs, (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets,
+		                (const uint32_t *)vb->big, Ftot)) != hipSuccess)
 			return hip_check(ctx, e, "vox_fill_big");
 	}
 	ctx->t_fill.stop(ctx->stream);
-	fl->filled = true;
 	return SVO_OK;
 }
 

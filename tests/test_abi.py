@@ -51,7 +51,15 @@ def test_record_layouts():
 def test_scene_generator_sizes_and_determinism():
     import sparsevoxeloctree_amd as S
 
-    for kind, param, T in ((1, 64, 258048), (2, 1, 990800), (1, 2, 8064), (2, 10, 6 * 20 * 20 * 2 + 40 * 12 + 20 * 2530)):
+    for kind, param, T in (
+        (1, 64, 258048),
+        (2, 1, 990800),
+        (1, 2, 8064),
+        (2, 10, 6 * 20 * 20 * 2 + 40 * 12 + 20 * 2530),
+        (4, 1000, 30 * 29),
+        (4, 5000, 4032 + 30 * 29),
+        (4, 20, 0),
+    ):
         xyz, rgb = S.gen_scene(kind, 0x5EED0000 + kind, param)
         assert rgb.shape == (T,), (kind, param, rgb.shape)
         assert xyz.shape == (T, 9)
