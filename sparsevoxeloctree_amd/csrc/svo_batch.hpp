@@ -30,11 +30,45 @@ __device__ __forceinline__ uint32_t seg_find(const uint64_t *__restrict__ begin,
 	return lo;
 }
 
+// Scene of item i knowing a scene s at or before it (a loop's previous scene): s itself when i is
+// still inside it, else a binary search over the scenes after s.
+__device__ __forceinline__ uint32_t seg_advance(const uint64_t *__restrict__ begin, uint32_t S, uint32_t s, uint64_t i) {
+	if (s + 1u >= S || begin[s + 1u] > i)
+		return s;
+	uint32_t lo = s + 1u, hi = S - 1u;
+	while (lo < hi) {
+		const uint32_t mid = (lo + hi + 1u) >> 1;
+		if (begin[mid] <= i)
+			lo = mid;
+		else
+			hi = mid - 1u;
+	}
+	return lo;
+}
+
 // Same, starting from a scene s at or before item i's (the scene of the block's first item).
 __device__ __forceinline__ uint32_t seg_walk(const uint64_t *__restrict__ begin, uint32_t S, uint32_t s, uint64_t i) {
 	while (s + 1u < S && begin[s + 1u] <= i)
 		++s;
 	return s;
+}
+
+// Calls body(s) with the scene s of item i, inside a loop step of a block that covers the items
+// [base, base + kT).  hint (0 before the loop) tracks the scene of base.  When all kT items of the
+// step lie in one scene, s is passed wave-uniform (readfirstlane), so the body's loads of that
+// scene's descriptor are scalar; otherwise each lane walks to its own scene.
+template <uint32_t kT, class Body>
+__device__ __forceinline__ void with_seg(const uint64_t *__restrict__ begin, uint32_t S, uint32_t &hint, uint64_t base,
+                                         uint64_t i, Body body) {
+	if (S == 1u) {
+		body(0u);
+		return;
+	}
+	hint = seg_advance(begin, S, hint, base);
+	if (base + kT <= begin[hint + 1u])
+		body((uint32_t)__builtin_amdgcn_readfirstlane(hint));
+	else
+		body(seg_walk(begin, S, hint, i));
 }
 
 // Camera of one march (the uniform block of src/Camera.cpp:77-95 without padding).

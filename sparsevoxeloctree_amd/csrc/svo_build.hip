@@ -44,14 +44,6 @@ constexpr uint32_t kLdsBucket = 4096; // largest bucket sorted in LDS
 constexpr uint32_t kRegBucket = 16;   // leaf blocks up to this size: one thread each (build_fold)
 constexpr uint32_t kWaveBucket = 64;  // up to this size: one wave each (build_fold_mid)
 
-// Scene of item i; base is the block-uniform first item of the loop step (a cheap starting point).
-__device__ __forceinline__ uint32_t item_seg(const uint64_t *__restrict__ begin, uint32_t S, uint64_t base,
-                                             uint64_t i) {
-	if (S == 1u)
-		return 0u;
-	return seg_walk(begin, S, seg_find(begin, S, base), i);
-}
-
 // Effective voxel coordinate seen by the reference's descent: the coordinate's bits, saturating
 // at res-1 for coordinates >= res (shader/octree_tag_node.comp:18-30).
 __device__ __forceinline__ void decode_frag(uint2 fr, uint32_t res, uint32_t &x, uint32_t &y, uint32_t &z) {
@@ -108,22 +100,23 @@ __global__ void __launch_bounds__(kThreads)
             uint32_t *B) {
 	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
 		const uint64_t f = base + threadIdx.x;
 		uint64_t word = ~0ull; // global bitmap word
 		uint32_t bits = 0u;
-		if (f < Ftot) {
-			const uint32_t s = item_seg(fbeg, S, base, f);
-			const BuildSeg &g = segs[s];
-			const uint32_t L = g.L;
-			if (L > 1u) {
-				uint32_t x, y, z;
-				decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
-				const uint64_t P = morton(x, y, z) >> 3;
-				word = g.bm + g.off[L - 1u] + (P >> 5);
-				bits = 1u << (uint32_t)(P & 31u);
-			}
-		}
+		if (f < Ftot)
+			with_seg<kThreads>(fbeg, S, hint, base, f, [&](uint32_t s) {
+				const BuildSeg &g = segs[s];
+				const uint32_t L = g.L;
+				if (L > 1u) {
+					uint32_t x, y, z;
+					decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
+					const uint64_t P = morton(x, y, z) >> 3;
+					word = g.bm + g.off[L - 1u] + (P >> 5);
+					bits = 1u << (uint32_t)(P & 31u);
+				}
+			});
 #pragma unroll
 		for (uint32_t d = 1; d < 64u; d <<= 1) {
 			const uint64_t ow = __shfl_up(word, d, 64);
@@ -145,25 +138,27 @@ __global__ void __launch_bounds__(kThreads)
     bm_reduce(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dr, uint32_t S, uint64_t n, uint32_t d,
               uint32_t *B) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
 		const uint64_t i = base + threadIdx.x;
 		if (i >= n)
 			continue;
-		const uint32_t s = item_seg(dr, S, base, i);
-		const BuildSeg &g = segs[s];
-		const uint64_t w = i - dr[s];
-		const uint32_t *Bc = B + g.bm + g.off[d + 1u];
-		const uint64_t nwc = g.off[d + 2u] - g.off[d + 1u];
-		uint32_t bits = 0u;
+		with_seg<kThreads>(dr, S, hint, base, i, [&](uint32_t s) {
+			const BuildSeg &g = segs[s];
+			const uint64_t w = i - dr[s];
+			const uint32_t *Bc = B + g.bm + g.off[d + 1u];
+			const uint64_t nwc = g.off[d + 2u] - g.off[d + 1u];
+			uint32_t bits = 0u;
 #pragma unroll
-		for (uint32_t k = 0; k < 8u; ++k) {
-			const uint64_t cw = 8u * w + k;
-			const uint32_t v = cw < nwc ? Bc[cw] : 0u;
+			for (uint32_t k = 0; k < 8u; ++k) {
+				const uint64_t cw = 8u * w + k;
+				const uint32_t v = cw < nwc ? Bc[cw] : 0u;
 #pragma unroll
-			for (uint32_t b = 0; b < 4u; ++b)
-				bits |= (((v >> (8u * b)) & 0xffu) != 0u ? 1u : 0u) << (4u * k + b);
-		}
-		B[g.bm + g.off[d] + w] = bits;
+				for (uint32_t b = 0; b < 4u; ++b)
+					bits |= (((v >> (8u * b)) & 0xffu) != 0u ? 1u : 0u) << (4u * k + b);
+			}
+			B[g.bm + g.off[d] + w] = bits;
+		});
 	}
 }
 
@@ -215,38 +210,40 @@ __global__ void __launch_bounds__(kThreads)
     bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dw, uint32_t S, uint64_t n,
                     uint32_t d, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
 		const uint64_t i = base + threadIdx.x;
 		if (i >= n)
 			continue;
-		const uint32_t s = item_seg(dw, S, base, i);
-		const BuildSeg &g = segs[s];
-		const uint64_t w = i - dw[s];
-		const uint32_t *Bs = B + g.bm, *Rs = R + g.bm;
-		const uint64_t offd = g.off[d];
-		const uint32_t v = Bs[offd + w];
-		const uint32_t base_rank = Rs[offd + w] - Rs[offd];
-		const uint32_t fbd = g.fb[d], fbp = g.fb[d - 1u];
-		uint32_t *oct = g.words;
+		with_seg<kThreads>(dw, S, hint, base, i, [&](uint32_t s) {
+			const BuildSeg &g = segs[s];
+			const uint64_t w = i - dw[s];
+			const uint32_t *Bs = B + g.bm, *Rs = R + g.bm;
+			const uint64_t offd = g.off[d];
+			const uint32_t v = Bs[offd + w];
+			const uint32_t base_rank = Rs[offd + w] - Rs[offd];
+			const uint32_t fbd = g.fb[d], fbp = g.fb[d - 1u];
+			uint32_t *oct = g.words;
 #pragma unroll
-		for (uint32_t k = 0; k < 4u; ++k) {
-			const uint32_t byte = (v >> (8u * k)) & 0xffu;
-			const uint64_t q = 4u * w + k; // parent at depth d-1
-			if (d == 1u ? q != 0u : byte == 0u)
-				continue; // the root block always exists
-			const uint32_t block = d == 1u ? 0u : fbp + node_rank(Bs, Rs, g.off[d - 1u], q);
-			uint32_t r = base_rank + (uint32_t)__popc(v & ((1u << (8u * k)) - 1u));
-			uint32_t wd[8];
+			for (uint32_t k = 0; k < 4u; ++k) {
+				const uint32_t byte = (v >> (8u * k)) & 0xffu;
+				const uint64_t q = 4u * w + k; // parent at depth d-1
+				if (d == 1u ? q != 0u : byte == 0u)
+					continue; // the root block always exists
+				const uint32_t block = d == 1u ? 0u : fbp + node_rank(Bs, Rs, g.off[d - 1u], q);
+				uint32_t r = base_rank + (uint32_t)__popc(v & ((1u << (8u * k)) - 1u));
+				uint32_t wd[8];
 #pragma unroll
-			for (uint32_t t = 0; t < 8u; ++t) {
-				const bool set = (byte >> t) & 1u;
-				wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
-				r += set ? 1u : 0u;
+				for (uint32_t t = 0; t < 8u; ++t) {
+					const bool set = (byte >> t) & 1u;
+					wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
+					r += set ? 1u : 0u;
+				}
+				uint4 *dst = reinterpret_cast<uint4 *>(oct + 8u * (uint64_t)block);
+				dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+				dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
 			}
-			uint4 *dst = reinterpret_cast<uint4 *>(oct + 8u * (uint64_t)block);
-			dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-			dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
-		}
+		});
 	}
 }
 
@@ -260,19 +257,20 @@ __global__ void __launch_bounds__(kThreads)
                    uint32_t *cnt) {
 	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
 		const uint64_t f = base + threadIdx.x;
 		uint32_t bin = 0xFFFFFFFFu;
-		if (f < Ftot) {
-			const uint32_t s = item_seg(fbeg, S, base, f);
-			const BuildSeg &g = segs[s];
-			const uint32_t L = g.L;
-			uint32_t x, y, z;
-			decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
-			const uint32_t local =
-			    L == 1u ? 0u : node_rank(B + g.bm, R + g.bm, g.off[L - 1u], morton(x, y, z) >> 3);
-			bin = (uint32_t)g.bin_begin + local;
-		}
+		if (f < Ftot)
+			with_seg<kThreads>(fbeg, S, hint, base, f, [&](uint32_t s) {
+				const BuildSeg &g = segs[s];
+				const uint32_t L = g.L;
+				uint32_t x, y, z;
+				decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
+				const uint32_t local =
+				    L == 1u ? 0u : node_rank(B + g.bm, R + g.bm, g.off[L - 1u], morton(x, y, z) >> 3);
+				bin = (uint32_t)g.bin_begin + local;
+			});
 		const uint32_t prev = __shfl_up(bin, 1u, 64);
 		const bool head = (lane == 0u) || prev != bin;
 		const uint64_t heads = __ballot(head);
@@ -295,13 +293,14 @@ __global__ void __launch_bounds__(kThreads)
     build_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
                   const uint2 *__restrict__ node, const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
 		const uint64_t f = base + threadIdx.x;
 		if (f >= Ftot)
 			continue;
-		const uint32_t s = item_seg(fbeg, S, base, f);
 		const uint2 np = node[f];
-		bucket[aux[np.x] + np.y] = segs[s].frags[f - fbeg[s]];
+		with_seg<kThreads>(fbeg, S, hint, base, f,
+		                   [&](uint32_t s) { bucket[aux[np.x] + np.y] = segs[s].frags[f - fbeg[s]]; });
 	}
 }
 
@@ -317,17 +316,25 @@ __constant__ uint32_t kDivMagic[65] = {
     79536432u,  78090315u,  76695845u,   75350304u,   74051161u,   72796056u,  71582789u,  70409300u,  69273667u,
     68174085u,  67108864u};
 
-// Running average of the reference's CAS loop (octree_tag_node.comp:43-57): fold colour c into a
-// leaf word that has folded a fragments so far.  In a sequence the a-th fragment (a >= 1) meets the
-// word's 6-bit count n = min(a, 63); avg' = (avg * n + c) / (n + 1) per channel.
-__device__ __forceinline__ uint32_t fold_rt(uint32_t prev, uint32_t c, uint32_t a) {
-	if (a == 0u)
+// Running average of the reference's CAS loop (octree_tag_node.comp:43-57): fold colour c into
+// leaf word prev (0 = empty leaf).  The word's 6-bit count n = min(a, 63) after a fragments is the
+// weight the next fragment meets: avg' = (avg * n + c) / (n + 1) per channel, count' = min(n + 1, 63).
+// magic = kDivMagic staged in LDS (a per-lane index into __constant__ memory would be a global load).
+__device__ __forceinline__ uint32_t fold_word(uint32_t prev, uint32_t c, const uint32_t *magic) {
+	if (prev == 0u)
 		return 0xC1000000u | c;
-	const uint32_t n = a < 63u ? a : 63u, w = n + 1u, m = kDivMagic[w];
+	const uint32_t n = (prev >> 24u) & 0x3fu, w = n + 1u, m = magic[w];
 	const uint32_t r = __umulhi((prev & 0xffu) * n + (c & 0xffu), m);
 	const uint32_t g = __umulhi(((prev >> 8u) & 0xffu) * n + ((c >> 8u) & 0xffu), m);
 	const uint32_t b = __umulhi(((prev >> 16u) & 0xffu) * n + ((c >> 16u) & 0xffu), m);
 	return (umin32(w, 0x3fu) << 24u) | (r & 0xffu) | ((g & 0xffu) << 8u) | ((b & 0xffu) << 16u) | 0xC0000000u;
+}
+
+// Stages the division table in LDS; every thread of the block must call it.
+__device__ __forceinline__ void stage_magic(uint32_t *s_magic) {
+	for (uint32_t i = threadIdx.x; i < 65u; i += blockDim.x)
+		s_magic[i] = kDivMagic[i];
+	__syncthreads();
 }
 
 // The leaf a fragment key belongs to inside its block (child slot at depth L).
@@ -337,22 +344,18 @@ __device__ __forceinline__ uint32_t key_slot(uint64_t key, uint32_t res) {
 	return (x & 1u) | ((y & 1u) << 1u) | ((z & 1u) << 2u);
 }
 
-// Folds one key (in ascending key order) into the 8 leaves of a block: w[s] words, c[s] counts.
-// The slot is selected without dynamic register indexing.
-__device__ __forceinline__ void slot_push(uint32_t (&w)[8], uint32_t (&c)[8], uint64_t key, uint32_t res) {
+// Folds one key (in ascending key order) into the 8 leaf words of a block.  The slot is selected
+// without dynamic register indexing.
+__device__ __forceinline__ void slot_push(uint32_t (&w)[8], uint64_t key, uint32_t res, const uint32_t *magic) {
 	const uint32_t s = key_slot(key, res);
-	uint32_t cw = w[0], cc = c[0];
+	uint32_t cw = w[0];
 #pragma unroll
-	for (uint32_t t = 1; t < 8u; ++t) {
+	for (uint32_t t = 1; t < 8u; ++t)
 		cw = s == t ? w[t] : cw;
-		cc = s == t ? c[t] : cc;
-	}
-	const uint32_t nw = fold_rt(cw, (uint32_t)key & 0xffffffu, cc);
+	const uint32_t nw = fold_word(cw, (uint32_t)key & 0xffffffu, magic);
 #pragma unroll
-	for (uint32_t t = 0; t < 8u; ++t) {
+	for (uint32_t t = 0; t < 8u; ++t)
 		w[t] = s == t ? nw : w[t];
-		c[t] = s == t ? cc + 1u : c[t];
-	}
 }
 
 __device__ __forceinline__ void store_block(uint32_t *oct, uint64_t block, const uint32_t (&w)[8]) {
@@ -363,11 +366,13 @@ __device__ __forceinline__ void store_block(uint32_t *oct, uint64_t block, const
 
 // Work lists of leaf blocks too large for build_fold: list[0] = mid count, list[1] = big count,
 // mid blocks at list[2 ..], big blocks at list[2 + cap ..] (each list holds at most Ftot / 17).
-__device__ __forceinline__ void push_bin(uint32_t *list, uint64_t cap, uint32_t j, uint32_t n) {
-	const uint32_t which = n > kWaveBucket ? 1u : 0u;
-	const uint32_t q = atomicAdd(&list[which], 1u);
-	if (q < cap)
-		list[2u + which * cap + q] = j;
+// Every lane of the wave calls it (the appends are aggregated per wave).
+__device__ __forceinline__ void push_bins(uint32_t *list, uint64_t cap, uint32_t j, bool mid, bool big) {
+	const uint32_t qm = wave_append(&list[0], mid), qb = wave_append(&list[1], big);
+	if (mid && qm < cap)
+		list[2u + qm] = j;
+	if (big && qb < cap)
+		list[2u + cap + qb] = j;
 }
 
 // Octree block of global leaf block j (scene s): the scene's first leaf block + j's rank.
@@ -381,63 +386,110 @@ __device__ __forceinline__ LeafTarget leaf_target(const BuildSeg &g, const uint6
 	return LeafTarget{g.words, (uint64_t)(L == 1u ? 0u : g.fb[L - 1u]) + (j - bbeg[s]), 1u << L};
 }
 
-// One thread per leaf block: its fragments (<= 16) are sorted by the 64-bit fragment in registers
-// (19- or 63-comparator networks, +inf padding), then folded in that order into the 8 leaves; the
-// whole block is written, empty leaves as 0.
+// One thread per leaf block with at most 16 fragments: sorted by the 64-bit fragment in registers
+// (5-, 19- or 63-comparator networks, +inf padding), then folded in that order into the 8 leaves;
+// the whole block is written, empty leaves as 0.  Each block of threads first orders its 256 leaf
+// blocks by size class in LDS, so that most waves run a single network.
+template <uint32_t kN>
+__device__ __forceinline__ void fold_small(const uint2 *__restrict__ b, uint32_t n, uint32_t res, const uint32_t *magic,
+                                           uint32_t (&w)[8]) {
+	uint64_t k[kN];
+#pragma unroll
+	for (uint32_t a = 0; a < kN; ++a)
+		k[a] = a < n ? frag_key(b[a]) : ~0ull;
+	if (kN == 4) {
+		cmp_swap(k[0], k[1]), cmp_swap(k[2], k[3]), cmp_swap(k[0], k[2]), cmp_swap(k[1], k[3]), cmp_swap(k[1], k[2]);
+	} else if (kN == 8) {
+		cmp_swap(k[0], k[1]), cmp_swap(k[2], k[3]), cmp_swap(k[4], k[5]), cmp_swap(k[6], k[7]);
+		cmp_swap(k[0], k[2]), cmp_swap(k[1], k[3]), cmp_swap(k[4], k[6]), cmp_swap(k[5], k[7]);
+		cmp_swap(k[1], k[2]), cmp_swap(k[5], k[6]), cmp_swap(k[0], k[4]), cmp_swap(k[3], k[7]);
+		cmp_swap(k[1], k[5]), cmp_swap(k[2], k[6]);
+		cmp_swap(k[1], k[4]), cmp_swap(k[3], k[6]);
+		cmp_swap(k[2], k[4]), cmp_swap(k[3], k[5]);
+		cmp_swap(k[3], k[4]);
+	} else { // Batcher odd-even merge sort of 16
+		cmp_swap(k[0], k[1]), cmp_swap(k[2], k[3]), cmp_swap(k[0], k[2]), cmp_swap(k[1], k[3]);
+		cmp_swap(k[1], k[2]), cmp_swap(k[4], k[5]), cmp_swap(k[6], k[7]), cmp_swap(k[4], k[6]);
+		cmp_swap(k[5], k[7]), cmp_swap(k[5], k[6]), cmp_swap(k[0], k[4]), cmp_swap(k[2], k[6]);
+		cmp_swap(k[2], k[4]), cmp_swap(k[1], k[5]), cmp_swap(k[3], k[7]), cmp_swap(k[3], k[5]);
+		cmp_swap(k[1], k[2]), cmp_swap(k[3], k[4]), cmp_swap(k[5], k[6]), cmp_swap(k[8], k[9]);
+		cmp_swap(k[10], k[11]), cmp_swap(k[8], k[10]), cmp_swap(k[9], k[11]), cmp_swap(k[9], k[10]);
+		cmp_swap(k[12], k[13]), cmp_swap(k[14], k[15]), cmp_swap(k[12], k[14]), cmp_swap(k[13], k[15]);
+		cmp_swap(k[13], k[14]), cmp_swap(k[8], k[12]), cmp_swap(k[10], k[14]), cmp_swap(k[10], k[12]);
+		cmp_swap(k[9], k[13]), cmp_swap(k[11], k[15]), cmp_swap(k[11], k[13]), cmp_swap(k[9], k[10]);
+		cmp_swap(k[11], k[12]), cmp_swap(k[13], k[14]), cmp_swap(k[0], k[8]), cmp_swap(k[4], k[12]);
+		cmp_swap(k[4], k[8]), cmp_swap(k[2], k[10]), cmp_swap(k[6], k[14]), cmp_swap(k[6], k[10]);
+		cmp_swap(k[2], k[4]), cmp_swap(k[6], k[8]), cmp_swap(k[10], k[12]), cmp_swap(k[1], k[9]);
+		cmp_swap(k[5], k[13]), cmp_swap(k[5], k[9]), cmp_swap(k[3], k[11]), cmp_swap(k[7], k[15]);
+		cmp_swap(k[7], k[11]), cmp_swap(k[3], k[5]), cmp_swap(k[7], k[9]), cmp_swap(k[11], k[13]);
+		cmp_swap(k[1], k[2]), cmp_swap(k[3], k[4]), cmp_swap(k[5], k[6]), cmp_swap(k[7], k[8]);
+		cmp_swap(k[9], k[10]), cmp_swap(k[11], k[12]), cmp_swap(k[13], k[14]);
+	}
+#pragma unroll
+	for (uint32_t a = 0; a < kN; ++a)
+		if (a < n)
+			slot_push(w, k[a], res, magic);
+}
+
 __global__ void __launch_bounds__(kThreads)
     build_fold(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S, uint64_t nbins,
                const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
                uint32_t *list, uint64_t cap) {
+	constexpr uint32_t kWaves = kThreads / 64;
+	__shared__ uint32_t s_magic[65];
+	__shared__ uint32_t s_bin[kThreads], s_n[kThreads];
+	__shared__ uint32_t s_cls[kWaves][3];
+	stage_magic(s_magic);
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	const uint64_t below = (1ull << lane) - 1ull;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < nbins; base += stride) {
 		const uint64_t jj = base + threadIdx.x;
-		if (jj >= nbins)
-			continue;
-		const uint32_t j = (uint32_t)jj;
-		const uint32_t n = cnt[j];
-		if (n > kRegBucket) {
-			push_bin(list, cap, j, n);
-			continue;
+		uint32_t n = 0, cls = 3; // classes: <= 4, <= 8, <= 16 fragments; 3 = not folded here
+		if (jj < nbins) {
+			n = cnt[jj];
+			cls = n <= 4u ? 0u : n <= 8u ? 1u : n <= kRegBucket ? 2u : 3u;
 		}
-		const uint32_t s = item_seg(bbeg, S, base, jj);
-		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
-		const uint2 *b = bucket + aux[j];
-		uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u}, c[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
-		uint64_t k[16];
-#pragma unroll
-		for (uint32_t a = 0; a < 16u; ++a)
-			k[a] = a < n ? frag_key(b[a]) : ~0ull;
-		if (n <= 8u) {
-			cmp_swap(k[0], k[1]), cmp_swap(k[2], k[3]), cmp_swap(k[4], k[5]), cmp_swap(k[6], k[7]);
-			cmp_swap(k[0], k[2]), cmp_swap(k[1], k[3]), cmp_swap(k[4], k[6]), cmp_swap(k[5], k[7]);
-			cmp_swap(k[1], k[2]), cmp_swap(k[5], k[6]), cmp_swap(k[0], k[4]), cmp_swap(k[3], k[7]);
-			cmp_swap(k[1], k[5]), cmp_swap(k[2], k[6]);
-			cmp_swap(k[1], k[4]), cmp_swap(k[3], k[6]);
-			cmp_swap(k[2], k[4]), cmp_swap(k[3], k[5]);
-			cmp_swap(k[3], k[4]);
-		} else { // Batcher odd-even merge sort of 16
-			cmp_swap(k[0], k[1]), cmp_swap(k[2], k[3]), cmp_swap(k[0], k[2]), cmp_swap(k[1], k[3]);
-			cmp_swap(k[1], k[2]), cmp_swap(k[4], k[5]), cmp_swap(k[6], k[7]), cmp_swap(k[4], k[6]);
-			cmp_swap(k[5], k[7]), cmp_swap(k[5], k[6]), cmp_swap(k[0], k[4]), cmp_swap(k[2], k[6]);
-			cmp_swap(k[2], k[4]), cmp_swap(k[1], k[5]), cmp_swap(k[3], k[7]), cmp_swap(k[3], k[5]);
-			cmp_swap(k[1], k[2]), cmp_swap(k[3], k[4]), cmp_swap(k[5], k[6]), cmp_swap(k[8], k[9]);
-			cmp_swap(k[10], k[11]), cmp_swap(k[8], k[10]), cmp_swap(k[9], k[11]), cmp_swap(k[9], k[10]);
-			cmp_swap(k[12], k[13]), cmp_swap(k[14], k[15]), cmp_swap(k[12], k[14]), cmp_swap(k[13], k[15]);
-			cmp_swap(k[13], k[14]), cmp_swap(k[8], k[12]), cmp_swap(k[10], k[14]), cmp_swap(k[10], k[12]);
-			cmp_swap(k[9], k[13]), cmp_swap(k[11], k[15]), cmp_swap(k[11], k[13]), cmp_swap(k[9], k[10]);
-			cmp_swap(k[11], k[12]), cmp_swap(k[13], k[14]), cmp_swap(k[0], k[8]), cmp_swap(k[4], k[12]);
-			cmp_swap(k[4], k[8]), cmp_swap(k[2], k[10]), cmp_swap(k[6], k[14]), cmp_swap(k[6], k[10]);
-			cmp_swap(k[2], k[4]), cmp_swap(k[6], k[8]), cmp_swap(k[10], k[12]), cmp_swap(k[1], k[9]);
-			cmp_swap(k[5], k[13]), cmp_swap(k[5], k[9]), cmp_swap(k[3], k[11]), cmp_swap(k[7], k[15]);
-			cmp_swap(k[7], k[11]), cmp_swap(k[3], k[5]), cmp_swap(k[7], k[9]), cmp_swap(k[11], k[13]);
-			cmp_swap(k[1], k[2]), cmp_swap(k[3], k[4]), cmp_swap(k[5], k[6]), cmp_swap(k[7], k[8]);
-			cmp_swap(k[9], k[10]), cmp_swap(k[11], k[12]), cmp_swap(k[13], k[14]);
+		push_bins(list, cap, (uint32_t)jj, cls == 3u && jj < nbins && n <= kWaveBucket, jj < nbins && n > kWaveBucket);
+		const uint64_t m0 = __ballot(cls == 0u), m1 = __ballot(cls == 1u), m2 = __ballot(cls == 2u);
+		if (lane == 0u) {
+			s_cls[wave][0] = (uint32_t)__popcll(m0);
+			s_cls[wave][1] = (uint32_t)__popcll(m1);
+			s_cls[wave][2] = (uint32_t)__popcll(m2);
 		}
+		__syncthreads();
+		uint32_t pos = 0, total = 0;
 #pragma unroll
-		for (uint32_t a = 0; a < 16u; ++a)
-			if (a < n)
-				slot_push(w, c, k[a], t.res);
-		store_block(t.oct, t.block, w);
+		for (uint32_t c = 0; c < 3u; ++c)
+#pragma unroll
+			for (uint32_t v = 0; v < kWaves; ++v) {
+				const uint32_t k = s_cls[v][c];
+				pos += (c < cls || (c == cls && v < wave)) ? k : 0u;
+				total += k;
+			}
+		if (cls < 3u) {
+			const uint64_t mine = cls == 0u ? m0 : cls == 1u ? m1 : m2;
+			pos += (uint32_t)__popcll(mine & below);
+			s_bin[pos] = (uint32_t)jj;
+			s_n[pos] = n;
+		}
+		__syncthreads();
+		if (threadIdx.x < total) {
+			const uint32_t j = s_bin[threadIdx.x], nj = s_n[threadIdx.x];
+			LeafTarget t;
+			with_seg<kThreads>(bbeg, S, hint, base, j, [&](uint32_t s) { t = leaf_target(segs[s], bbeg, s, j); });
+			const uint2 *b = bucket + aux[j];
+			uint32_t w[8] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u};
+			if (nj <= 4u)
+				fold_small<4>(b, nj, t.res, s_magic, w);
+			else if (nj <= 8u)
+				fold_small<8>(b, nj, t.res, s_magic, w);
+			else
+				fold_small<16>(b, nj, t.res, s_magic, w);
+			store_block(t.oct, t.block, w);
+		}
+		__syncthreads();
 	}
 }
 
@@ -469,94 +521,163 @@ template <class Get, class Put> __device__ void block_bitonic(uint32_t n, uint32
 	}
 }
 
-// Leaf blocks of 17..64 fragments, one wave each: lane i holds key i (+inf padding), a bitonic sort
-// over lane shuffles orders them, the sorted keys go through LDS, and lane s < 8 folds the keys of
-// leaf slot s in ascending order.
+// Leaf blocks of 17..64 fragments, one wave each: lane i holds key i and its leaf slot (padding:
+// slot 8, key +inf); a bitonic sort over lane shuffles orders them by (slot, key), so every leaf's
+// fragments are consecutive and ascending.  Lane s < 8 then folds the run of slot s from LDS.
 __global__ void __launch_bounds__(kThreads)
     build_fold_mid(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
                    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
                    const uint32_t *list, uint64_t cap) {
 	constexpr uint32_t kWaves = kThreads / 64;
-	__shared__ uint64_t s_keys[kWaves][64];
+	__shared__ uint32_t s_magic[65];
+	__shared__ uint32_t s_col[kWaves][64];
+	stage_magic(s_magic);
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 	uint64_t count = list[0];
 	if (count > cap)
 		count = cap;
 	for (uint64_t base = (uint64_t)blockIdx.x * kWaves; base < count; base += (uint64_t)gridDim.x * kWaves) {
 		const uint64_t item = base + wave;
-		uint32_t j = 0u, n = 0u;
-		if (item < count) {
-			j = list[2u + item];
-			n = cnt[j];
+		if (item >= count)
+			break; // wave-uniform; no block barrier follows
+		const uint32_t j = list[2u + item];
+		const uint32_t n = cnt[j];
+		const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
+		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
+		uint64_t key = ~0ull;
+		uint32_t slot = 8u;
+		if (lane < n) {
+			key = frag_key(bucket[aux[j] + lane]);
+			slot = key_slot(key, t.res);
 		}
-		uint64_t key = lane < n ? frag_key(bucket[aux[j] + lane]) : ~0ull;
 #pragma unroll
 		for (uint32_t k = 2; k <= 64u; k <<= 1) {
 #pragma unroll
 			for (uint32_t d = k >> 1; d > 0u; d >>= 1) {
-				const uint64_t other = __shfl_xor(key, (int)d, 64);
-				const bool up = (lane & k) == 0u, lower = (lane & d) == 0u;
-				const bool take_min = up == lower;
-				key = take_min ? (other < key ? other : key) : (other < key ? key : other);
+				const uint64_t ok = __shfl_xor(key, (int)d, 64);
+				const uint32_t os = __shfl_xor(slot, (int)d, 64);
+				const bool other_less = os < slot || (os == slot && ok < key);
+				const bool take_min = ((lane & k) == 0u) == ((lane & d) == 0u);
+				const bool take_other = take_min == other_less;
+				key = take_other ? ok : key;
+				slot = take_other ? os : slot;
 			}
 		}
-		s_keys[wave][lane] = key;
-		__syncthreads();
-		if (item < count && lane < 8u) {
-			const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
-			const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
-			uint32_t word = 0u, a = 0u;
-			for (uint32_t i = 0; i < n; ++i) {
-				const uint64_t kk = s_keys[wave][i];
-				if (key_slot(kk, t.res) == lane)
-					word = fold_rt(word, (uint32_t)kk & 0xffffffu, a++);
+		s_col[wave][lane] = (uint32_t)key & 0xffffffu;
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+		uint64_t run = 0;
+#pragma unroll
+		for (uint32_t u = 0; u < 8u; ++u) {
+			const uint64_t mu = __ballot(slot == u);
+			run = lane == u ? mu : run;
+		}
+		if (lane < 8u) {
+			uint32_t word = 0u;
+			if (run) {
+				const uint32_t beg = (uint32_t)__ffsll((unsigned long long)run) - 1u, len = (uint32_t)__popcll(run);
+				for (uint32_t i = beg; i < beg + len; ++i)
+					word = fold_word(word, s_col[wave][i], s_magic);
 			}
 			t.oct[8u * t.block + lane] = word;
 		}
-		__syncthreads();
+		__builtin_amdgcn_wave_barrier();
 	}
 }
 
-// Leaf blocks of more than 64 fragments, one block each: bitonic sort of the keys (in LDS up to
-// 4096, in place in the bucket beyond), then 8 lanes fold the 8 leaves, each lane walking the
-// sorted keys and taking those of its slot.
+// Bitonic sort of n (<= m = 2^p) (slot, key) pairs held in LDS by one block, ascending; pairs past
+// n behave as +infinity padding.
+__device__ void block_bitonic_kv(uint32_t n, uint32_t m, uint64_t *keys, uint8_t *slots) {
+	for (uint32_t k = 2; k <= m; k <<= 1) {
+		for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+			for (uint32_t t = threadIdx.x; t < m / 2; t += blockDim.x) {
+				const uint32_t blk = t / jj, off = t % jj;
+				uint32_t a, b;
+				if (jj == (k >> 1)) { // first step of a merge: flip
+					a = blk * k + off;
+					b = blk * k + k - 1u - off;
+				} else {
+					a = blk * 2u * jj + off;
+					b = a + jj;
+				}
+				if (b < n) {
+					const uint64_t ka = keys[a], kb = keys[b];
+					const uint8_t sa = slots[a], sb = slots[b];
+					if (sb < sa || (sb == sa && kb < ka)) {
+						keys[a] = kb;
+						keys[b] = ka;
+						slots[a] = sb;
+						slots[b] = sa;
+					}
+				}
+			}
+			__syncthreads();
+		}
+	}
+}
+
+// Leaf blocks of more than 64 fragments, one block each.  Up to 4096 fragments: bitonic sort of the
+// (slot, key) pairs in LDS, then lane s < 8 folds the run of slot s.  Beyond (a global-memory
+// sort, rare): keys sorted in place in the bucket and each of the 8 lanes walks all of them,
+// taking those of its slot.
 __global__ void __launch_bounds__(kThreads)
     build_fold_big(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
                    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, uint2 *bucket,
                    const uint32_t *list, uint64_t cap) {
+	__shared__ uint32_t s_magic[65];
 	__shared__ uint64_t s_keys[kLdsBucket];
+	__shared__ uint8_t s_slots[kLdsBucket];
+	__shared__ uint32_t s_beg[8], s_end[8];
+	stage_magic(s_magic);
 	uint64_t count = list[1];
 	if (count > cap)
 		count = cap;
 	for (uint64_t item = blockIdx.x; item < count; item += gridDim.x) {
 		const uint32_t j = list[2u + cap + item];
 		const uint32_t n = cnt[j];
+		const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
+		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
 		uint2 *b = bucket + aux[j];
 		uint32_t m = 1;
 		while (m < n)
 			m <<= 1;
-		const bool lds = n <= kLdsBucket;
-		if (lds) {
-			for (uint32_t t = threadIdx.x; t < n; t += blockDim.x)
-				s_keys[t] = frag_key(b[t]);
+		if (n <= kLdsBucket) {
+			if (threadIdx.x < 8u)
+				s_beg[threadIdx.x] = s_end[threadIdx.x] = 0u;
+			for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+				const uint64_t key = frag_key(b[i]);
+				s_keys[i] = key;
+				s_slots[i] = (uint8_t)key_slot(key, t.res);
+			}
 			__syncthreads();
-			block_bitonic(
-			    n, m, [&](uint32_t i) { return s_keys[i]; }, [&](uint32_t i, uint64_t v) { s_keys[i] = v; });
+			block_bitonic_kv(n, m, s_keys, s_slots);
+			for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+				const uint32_t sl = s_slots[i];
+				if (i == 0u || s_slots[i - 1u] != sl)
+					s_beg[sl] = i;
+				if (i + 1u == n || s_slots[i + 1u] != sl)
+					s_end[sl] = i + 1u;
+			}
+			__syncthreads();
+			if (threadIdx.x < 8u) {
+				uint32_t word = 0u;
+				for (uint32_t i = s_beg[threadIdx.x]; i < s_end[threadIdx.x]; ++i)
+					word = fold_word(word, (uint32_t)s_keys[i] & 0xffffffu, s_magic);
+				t.oct[8u * t.block + threadIdx.x] = word;
+			}
 		} else {
 			block_bitonic(
 			    n, m, [&](uint32_t i) { return frag_key(b[i]); },
 			    [&](uint32_t i, uint64_t v) { b[i] = make_uint2((uint32_t)(v >> 32), (uint32_t)v); });
-		}
-		if (threadIdx.x < 8u) {
-			const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
-			const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
-			uint32_t word = 0u, a = 0u;
-			for (uint32_t i = 0; i < n; ++i) {
-				const uint64_t key = lds ? s_keys[i] : frag_key(b[i]);
-				if (key_slot(key, t.res) == threadIdx.x)
-					word = fold_rt(word, (uint32_t)key & 0xffffffu, a++);
+			if (threadIdx.x < 8u) {
+				uint32_t word = 0u;
+				for (uint32_t i = 0; i < n; ++i) {
+					const uint64_t key = frag_key(b[i]);
+					if (key_slot(key, t.res) == threadIdx.x)
+						word = fold_word(word, (uint32_t)key & 0xffffffu, s_magic);
+				}
+				t.oct[8u * t.block + threadIdx.x] = word;
 			}
-			t.oct[8u * t.block + threadIdx.x] = word;
 		}
 		__syncthreads();
 	}

@@ -44,4 +44,20 @@ __device__ __forceinline__ f3 normalize3(f3 v) {
 	return {v.x * s, v.y * s, v.z * s};
 }
 
+// Slot of this lane in a list appended to by the lanes whose pred holds: one atomicAdd on *counter
+// per wave (a shared counter hit once per item serialises in one L2 channel).  Call it from every
+// active lane; the result is meaningful where pred holds.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+	const uint64_t m = __ballot(pred);
+	const uint32_t lane = threadIdx.x & 63u;
+	uint32_t base = 0u;
+	if (m) {
+		const int leader = __ffsll((unsigned long long)m) - 1;
+		if ((int)lane == leader)
+			base = atomicAdd(counter, (uint32_t)__popcll(m));
+		base = __shfl(base, leader, 64);
+	}
+	return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 } // namespace svo

@@ -149,13 +149,6 @@ __device__ __forceinline__ void row_span(const Setup &s, int32_t j, int32_t &i0,
 
 __global__ void vox_zero_counter(uint32_t *c) { c[0] = 0u; }
 
-// Scene of triangle t; base is the block-uniform first item of the loop step.
-__device__ __forceinline__ uint32_t tri_seg(const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t base, uint64_t t) {
-	if (S == 1u)
-		return 0u;
-	return seg_walk(tbeg, S, seg_find(tbeg, S, base), t);
-}
-
 // Count pass.  Triangles with more than kBigTri fragments are also appended to `big` (big[0] is
 // the count) for vox_fill_big; the per-lane fill skips them.
 constexpr uint32_t kBigTri = 128;
@@ -164,14 +157,21 @@ __global__ void __launch_bounds__(kThreads)
     vox_count(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
               uint64_t *__restrict__ counts, uint32_t *__restrict__ big) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < T; base += stride) {
 		const uint64_t t = base + threadIdx.x;
 		if (t >= T)
 			continue;
-		const uint32_t sg = tri_seg(tbeg, S, base, t);
-		const VoxSeg &g = segs[sg];
+		const float *xyz;
+		uint64_t tl;
+		uint32_t level;
+		with_seg<kThreads>(tbeg, S, hint, base, t, [&](uint32_t sg) {
+			xyz = segs[sg].xyz;
+			level = segs[sg].level;
+			tl = t - tbeg[sg];
+		});
 		Setup s;
-		tri_setup(g.xyz, t - tbeg[sg], 1u << g.level, s);
+		tri_setup(xyz, tl, 1u << level, s);
 		uint32_t c = 0;
 		if (!s.empty)
 			for (int32_t j = s.jlo; j <= s.jhi; ++j) {
@@ -181,8 +181,9 @@ __global__ void __launch_bounds__(kThreads)
 					c += (uint32_t)(i1 - i0 + 1);
 			}
 		counts[t] = c;
+		const uint32_t q = wave_append(&big[0], c > kBigTri);
 		if (c > kBigTri)
-			big[1 + atomicAdd(&big[0], 1u)] = (uint32_t)t;
+			big[1 + q] = (uint32_t)t;
 	}
 }
 
@@ -223,6 +224,7 @@ __global__ void __launch_bounds__(kThreads)
     vox_fill(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
              const uint64_t *__restrict__ offsets, uint64_t Ftot) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < T; base += stride) {
 		const uint64_t t = base + threadIdx.x;
 		if (t >= T)
@@ -231,15 +233,24 @@ __global__ void __launch_bounds__(kThreads)
 		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : Ftot;
 		if (k == kend || kend - k > kBigTri)
 			continue;
-		const uint32_t sg = tri_seg(tbeg, S, base, t);
-		const VoxSeg &g = segs[sg];
-		const uint64_t tl = t - tbeg[sg];
-		const uint32_t res = 1u << g.level;
-		uint2 *frags = g.frags;
-		const uint64_t fb = g.frag_begin; // offsets are global over the batch
+		const float *xyz;
+		const uint32_t *rgb;
+		uint2 *frags;
+		uint64_t tl, fb; // fb: offsets are global over the batch
+		uint32_t level;
+		with_seg<kThreads>(tbeg, S, hint, base, t, [&](uint32_t sg) {
+			const VoxSeg &g = segs[sg];
+			xyz = g.xyz;
+			rgb = g.rgb;
+			frags = g.frags;
+			fb = g.frag_begin;
+			level = g.level;
+			tl = t - tbeg[sg];
+		});
+		const uint32_t res = 1u << level;
 		Setup s;
-		tri_setup(g.xyz, tl, res, s);
-		const uint32_t colour = g.rgb[tl] & 0x00ffffffu;
+		tri_setup(xyz, tl, res, s);
+		const uint32_t colour = rgb[tl] & 0x00ffffffu;
 		for (int32_t j = s.jlo; j <= s.jhi && k < kend; ++j) {
 			int32_t i0, i1;
 			row_span(s, j, i0, i1);
