@@ -55,6 +55,13 @@ CONFIGS = {
         "256x256 primary rays each; batched launches vs one scene per launch",
         kind=4, seed=0x5EED0004, scenes=256, width=256, height=256,
     ),
+    # serving: Poisson arrivals, 10% builds of sphere+sliver scenes, 90% ray queries (4K..2M rays)
+    5: dict(
+        workload="c5: serving loop, Poisson arrivals at 1000 QPS for 5 s; 10% builds of sphere+sliver scenes "
+        "(32K..516K tris, level U{8..11}), 90% ray queries of log-U[4K,2M] rays (half camera, half "
+        "uniform-sphere directions from in-volume origins) against up to 512 resident octrees",
+        seed=0x5EED0005, qps=1000.0, seconds=5.0, initial=64, pool_scenes=24, budget_gib=160, max_octrees=512,
+    ),
 }
 
 
@@ -308,6 +315,147 @@ def bench_batch(args, cfg):
     ctx.close()
 
 
+def bench_serve(args, cfg):
+    """c5: a client submits requests at Poisson arrival times to the serving loop (svo_server_*) and a
+    collector waits for them; the server batches them on the GPU.  Latencies are submit -> completion
+    on the host clock, as the server measures them."""
+    import queue
+    import threading
+
+    import sparsevoxeloctree_amd as S
+
+    rng = np.random.default_rng(cfg["seed"])
+    ctx = S.Context(0)
+    # scene pool: sphere + sliver scenes for build requests, uploaded once
+    pool = []
+    for i in range(cfg["pool_scenes"]):
+        n = int(rng.integers(4, 65))
+        xyz, rgb = S.gen_scene(3, cfg["seed"] + (i << 16), n)
+        pool.append(S.Scene.Create(ctx, xyz, rgb))
+    # ray pools in HBM: camera fans and uniform-sphere rays, 2M rays each
+    RMAX = 2 * 1024 * 1024
+    ray_sets = []
+    for i in range(8):
+        if i % 2 == 0:
+            v = rng.normal(size=3)
+            eye = 1.5 + 1.6 * v / np.linalg.norm(v)
+            look = 1.5 - eye + rng.uniform(-0.2, 0.2, 3)
+            look /= np.linalg.norm(look)
+            side = np.cross(look, [0.0, 1.0, 0.0])
+            side /= np.linalg.norm(side)
+            up = np.cross(side, look)
+            g = int(np.ceil(np.sqrt(RMAX)))
+            u, v = np.meshgrid(np.linspace(-0.6, 0.6, g), np.linspace(-0.6, 0.6, g))
+            d = look[None, :] + u.reshape(-1, 1) * side[None, :] + v.reshape(-1, 1) * up[None, :]
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            o = np.broadcast_to(eye, d.shape)
+        else:
+            o = rng.uniform(1.0, 2.0, (RMAX, 3))
+            d = rng.normal(size=(RMAX, 3))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays = np.concatenate([o, d], axis=1).astype(np.float32)
+        ray_sets.append(ctx.upload(rays[:RMAX]))
+    free_hits = queue.Queue()
+    for _ in range(48):
+        free_hits.put(ctx.alloc(RMAX * 32))
+    srv = S.Server(0, budget_bytes=cfg["budget_gib"] << 30, max_octrees=cfg["max_octrees"])
+    levels = list(range(8, 12))
+    # initial resident octrees
+    resident = []
+    init = [srv.submit_build(pool[i % len(pool)], levels[i % len(levels)]) for i in range(cfg["initial"])]
+    for rid in init:
+        st, _ = srv.wait(rid)
+        if st == 0:
+            resident.append(rid)
+    srv.reset_stats()
+    # arrivals
+    T, qps = cfg["seconds"], cfg["qps"]
+    t_arr = np.cumsum(rng.exponential(1.0 / qps, int(qps * T * 1.2)))
+    t_arr = t_arr[t_arr < T]
+    kinds = rng.uniform(size=t_arr.size) < 0.1
+    Rs = np.exp(rng.uniform(np.log(4096), np.log(RMAX), t_arr.size)).astype(np.int64)
+    pending = queue.Queue()
+    done = {"lat_b": [], "lat_r": [], "fail": 0, "rej": 0}
+    lock = threading.Lock()
+
+    def collector():
+        while True:
+            item = pending.get()
+            if item is None:
+                return
+            rid, is_build, buf = item
+            st, lat = srv.wait(rid)
+            with lock:
+                if is_build:
+                    if st == 0:
+                        resident.append(rid)
+                    else:
+                        done["rej"] += 1
+                elif st != 0:
+                    done["fail"] += 1
+            if buf is not None:
+                free_hits.put(buf)
+
+    col = threading.Thread(target=collector)
+    col.start()
+    t0 = time.perf_counter()
+    for k in range(t_arr.size):
+        while time.perf_counter() - t0 < t_arr[k]:
+            time.sleep(0.0002)
+        if kinds[k]:
+            rid = srv.submit_build(pool[int(rng.integers(len(pool)))], int(levels[int(rng.integers(4))]))
+            pending.put((rid, True, None))
+        else:
+            with lock:
+                target = resident[int(rng.integers(len(resident)))]
+            buf = free_hits.get()
+            rs = ray_sets[int(rng.integers(len(ray_sets)))]
+            rid = srv.submit_rays(target, rs, int(Rs[k]), buf, 0)
+            pending.put((rid, False, buf))
+    pending.put(None)
+    col.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    nb, nr = int(kinds.sum()), int((~kinds).sum())
+    line = {
+        "metric": METRIC,
+        "value": round(st["rays_traced"] / wall / 1e6, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": 0,
+        "ms_per_step": round(wall * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generators; scenes and ray pools resident in HBM)",
+        "config": {"workload": cfg["workload"], "qps": qps, "seconds": T, "requests": int(t_arr.size),
+                   "build_requests": nb, "ray_requests": nr, "initial_octrees": cfg["initial"],
+                   "budget_gib": cfg["budget_gib"], "max_octrees": cfg["max_octrees"]},
+        "ray_p50_ms": round(st["rays_p50_ms"], 4),
+        "ray_p99_ms": round(st["rays_p99_ms"], 4),
+        "build_p50_ms": round(st["build_p50_ms"], 4),
+        "build_p99_ms": round(st["build_p99_ms"], 4),
+        "builds_per_s": round(st["builds_done"] / wall, 2),
+        "ray_queries_per_s": round(st["ray_queries_done"] / wall, 2),
+        "builds_rejected": st["builds_rejected"],
+        "ray_queries_failed": st["ray_queries_failed"],
+        "evictions": st["evictions"],
+        "resident_octrees": st["resident_octrees"],
+        "resident_gib": round(st["resident_bytes"] / 2**30, 3),
+        "gpu_busy_build_s": round(st["busy_build_ms"] / 1e3, 3),
+        "gpu_busy_rays_s": round(st["busy_rays_ms"] / 1e3, 3),
+        "server_batches": st["batches"],
+        "cpu_baseline": None,
+        "roofline": None,
+    }
+    srv.close()
+    print(json.dumps(line))
+    sys.stdout.flush()
+    ctx.close()
+
+
 def copy_calibration(mib):
     import sparsevoxeloctree_amd as S
     from sparsevoxeloctree_amd._capi import lib
@@ -353,6 +501,9 @@ def main():
     cfg = CONFIGS[args.config]
     if args.config == 4:
         bench_batch(args, cfg)
+        return
+    if args.config == 5:
+        bench_serve(args, cfg)
         return
     xyz, rgb = S.gen_scene(cfg["kind"], cfg["seed"], cfg["param"])
     T = rgb.shape[0]

@@ -230,6 +230,39 @@ int svo_autotune_report(svo_ctx *ctx, char *buf, uint64_t cap, uint64_t *len);
  * per ray index.  rays_dev = R x 6 floats; a ray whose hit bit is clear gets o = 0, d = -1. */
 int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t seed, float *rays_dev);
 
+/* ---- serving loop (new; the reference's nearest analogue is its loader thread, which builds one
+ * octree on a side queue and hands it to the renderer, src/LoaderThread.cpp:15-116,
+ * src/Octree.cpp:22-35).  Build and ray-query requests for many scenes are accepted from any thread
+ * at any time.  Two worker threads, each with its own context (HIP stream) on the device, batch
+ * them: every pending ray query is marched in one launch per variant, pending builds are voxelized
+ * and built together, and every built octree stays resident, admitted by its exact size against
+ * budget_bytes and max_octrees (least-recently-used idle octrees are evicted to make room; a
+ * build that still does not fit completes with SVO_ERR_OUT_OF_MEMORY).
+ *   submit_build: the octree of `scene` at `level`; its id is the request id.  The scene must stay
+ *                 alive until the request completes.
+ *   submit_rays:  R rays (rays_dev, 6 floats each) against resident octree `octree_id`; hits_dev
+ *                 receives R svo_hit.  Both buffers must stay valid until completion.  A query
+ *                 for an octree that is not resident completes with SVO_ERR_INVALID_ARGUMENT.
+ *   wait:         blocks until the request completes; status = its result; latency_ms = submit to
+ *                 completion on the host clock.  Each request id is waited for at most once. */
+typedef struct svo_server svo_server;
+typedef struct svo_server_stats {
+	uint64_t builds_done, builds_rejected, ray_queries_done, ray_queries_failed, rays_traced, evictions, batches;
+	uint64_t resident_octrees, resident_bytes, budget_bytes;
+	double build_p50_ms, build_p99_ms, rays_p50_ms, rays_p99_ms;
+	double busy_build_ms, busy_rays_ms, elapsed_ms;
+} svo_server_stats;
+int svo_server_create(int device, uint64_t budget_bytes, uint32_t max_octrees, svo_server **out);
+int svo_server_destroy(svo_server *srv); /* drains pending requests, frees the resident octrees */
+int svo_server_submit_build(svo_server *srv, const svo_scene *scene, uint32_t level, uint64_t *req_id);
+int svo_server_submit_rays(svo_server *srv, uint64_t octree_id, const float *rays_dev, uint64_t R, int variant,
+                           svo_hit *hits_dev, uint64_t *req_id);
+int svo_server_wait(svo_server *srv, uint64_t req_id, int *status, double *latency_ms);
+/* A resident octree's word count and handle (for inspection; owned by the server). */
+int svo_server_resident(svo_server *srv, uint64_t octree_id, uint64_t *W, const svo_octree **oct);
+int svo_server_get_stats(svo_server *srv, svo_server_stats *out);
+int svo_server_reset_stats(svo_server *srv);
+
 /* ---- camera helper (Camera::fetch_uniform_data, src/Camera.cpp:77-95). */
 int svo_camera_from_yaw_pitch(const float position[3], float yaw, float pitch, float fov, float aspect,
                               svo_camera *out);

@@ -15,6 +15,7 @@ from .api import (  # noqa: F401
     OctreeBuilder,
     OctreeTracer,
     Scene,
+    Server,
     SvoError,
     Voxelizer,
     autotune,

@@ -56,6 +56,35 @@ class svo_timing(C.Structure):
     ]
 
 
+class svo_server_stats(C.Structure):
+    _fields_ = [
+        (n, C.c_uint64)
+        for n in (
+            "builds_done",
+            "builds_rejected",
+            "ray_queries_done",
+            "ray_queries_failed",
+            "rays_traced",
+            "evictions",
+            "batches",
+            "resident_octrees",
+            "resident_bytes",
+            "budget_bytes",
+        )
+    ] + [
+        (n, C.c_double)
+        for n in (
+            "build_p50_ms",
+            "build_p99_ms",
+            "rays_p50_ms",
+            "rays_p99_ms",
+            "busy_build_ms",
+            "busy_rays_ms",
+            "elapsed_ms",
+        )
+    ]
+
+
 class SvoError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
@@ -129,6 +158,14 @@ _SIGS = {
     "svo_march_plan_destroy": (i32, [P]),
     "svo_march_rays_batch": (i32, [P, PP, u32, C.POINTER(u64), P, i32, P]),
     "svo_autotune": (i32, [P, i32]),
+    "svo_server_create": (i32, [i32, u64, u32, PP]),
+    "svo_server_destroy": (i32, [P]),
+    "svo_server_submit_build": (i32, [P, P, u32, C.POINTER(u64)]),
+    "svo_server_submit_rays": (i32, [P, u64, P, u64, i32, P, C.POINTER(u64)]),
+    "svo_server_wait": (i32, [P, u64, C.POINTER(i32), C.POINTER(C.c_double)]),
+    "svo_server_resident": (i32, [P, u64, C.POINTER(u64), PP]),
+    "svo_server_get_stats": (i32, [P, C.POINTER(svo_server_stats)]),
+    "svo_server_reset_stats": (i32, [P]),
     "svo_autotune_report": (i32, [P, C.c_char_p, u64, C.POINTER(u64)]),
     "svo_camera_from_yaw_pitch": (i32, [C.POINTER(C.c_float), f32, f32, f32, f32, C.POINTER(svo_camera)]),
     "svo_gen_scene": (i32, [i32, u64, u64, P, P, C.POINTER(u64)]),

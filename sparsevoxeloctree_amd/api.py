@@ -544,6 +544,67 @@ def autotune(ctx, on=True):
     return buf.value.decode()
 
 
+class Server:
+    """svo_server_*: a serving loop with resident octrees, admission by memory and p50/p99 latency.
+
+    Requests may be submitted from any thread; wait() returns (status, latency_ms).  Scenes, ray and
+    hit buffers passed in must stay alive until their request completes."""
+
+    def __init__(self, device=0, budget_bytes=64 << 30, max_octrees=512):
+        h = C.c_void_p()
+        rc = lib.svo_server_create(device, int(budget_bytes), int(max_octrees), C.byref(h))
+        if rc:
+            raise SvoError(rc, "server create")
+        self.h = h
+
+    def submit_build(self, scene, level):
+        rid = C.c_uint64()
+        rc = lib.svo_server_submit_build(self.h, scene.h, int(level), C.byref(rid))
+        if rc:
+            raise SvoError(rc, "submit build")
+        return rid.value
+
+    def submit_rays(self, octree_id, rays, R, hits, variant=_capi.SVO_MARCH_LEAF_ITER):
+        """rays, hits: DeviceArray (or raw device pointers) of R rays / R hit records."""
+        rid = C.c_uint64()
+        rp = rays.ptr if isinstance(rays, DeviceArray) else rays
+        hp = hits.ptr if isinstance(hits, DeviceArray) else hits
+        rc = lib.svo_server_submit_rays(self.h, int(octree_id), rp, int(R), int(variant), hp, C.byref(rid))
+        if rc:
+            raise SvoError(rc, "submit rays")
+        return rid.value
+
+    def wait(self, req_id):
+        st, lat = C.c_int(), C.c_double()
+        rc = lib.svo_server_wait(self.h, int(req_id), C.byref(st), C.byref(lat))
+        if rc:
+            raise SvoError(rc, "wait")
+        return st.value, lat.value
+
+    def resident(self, octree_id):
+        """(W, octree handle) of a resident octree, or None."""
+        W, h = C.c_uint64(), C.c_void_p()
+        if lib.svo_server_resident(self.h, int(octree_id), C.byref(W), C.byref(h)):
+            return None
+        return W.value, h
+
+    def stats(self):
+        st = _capi.svo_server_stats()
+        lib.svo_server_get_stats(self.h, C.byref(st))
+        return {n: getattr(st, n) for n, _ in st._fields_}
+
+    def reset_stats(self):
+        lib.svo_server_reset_stats(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.svo_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
 def gen_scene(kind, seed, param=0):
     """Seeded synthetic scene: (xyz (T, 9) float32, rgb (T,) uint32)."""
     T = C.c_uint64()

@@ -18,7 +18,15 @@ BUILD = os.path.join(HERE, "_build")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libsvo.so")
 
-SOURCES = ["svo_capi.cpp", "svo_voxelize.hip", "svo_build.hip", "svo_march.hip", "svo_util.hip", "scenegen.cpp"]
+SOURCES = [
+    "svo_capi.cpp",
+    "svo_voxelize.hip",
+    "svo_build.hip",
+    "svo_march.hip",
+    "svo_util.hip",
+    "svo_server.cpp",
+    "scenegen.cpp",
+]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SVO_OFFLOAD_ARCH", "gfx950")

@@ -76,13 +76,15 @@ struct CamArgs {
 	float px, py, pz, lx, ly, lz, sx, sy, sz, ux, uy, uz;
 };
 
-// One octree of a march batch.
+// One octree of a march batch.  Explicit-ray batches: the segment's rays and hit records.
 struct MarchSeg {
 	const uint32_t *oct;
 	uint64_t W;
 	CamArgs cam;
 	float orig_sz; // exp2(1 - level), the beam's cone origin size (src/OctreeTracer.cpp:206)
 	uint32_t level;
+	const float *rays; // 6 floats per ray
+	uint4 *hits;       // 2 uint4 per ray
 };
 
 // One scene of a voxelize batch.  frag_begin / F are written by the count pass.
@@ -178,6 +180,9 @@ void march_plan_release(svo_ctx *ctx, MarchPlan *p);
 // Explicit rays against S octrees in one launch; ray_begin (host, S + 1 entries) splits rays.
 int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, const uint64_t *ray_begin,
                      const float *rays, int variant, svo_hit *hits);
+// Same with separate buffers per octree: rays[s] (count[s] rays) -> hits[s].
+int march_rays_segments(svo_ctx *ctx, const svo_octree *const *octs, const float *const *rays, svo_hit *const *hits,
+                        const uint64_t *count, uint32_t S, int variant);
 
 } // namespace svo
 

@@ -412,24 +412,26 @@ __global__ void __launch_bounds__(64 * WX * WY)
 		atomicOr(err_out, err);
 }
 
-// Explicit rays against S octrees: rays [rbeg[s], rbeg[s+1]) go to octree s.
+// Explicit rays against S octrees: rays [rbeg[s], rbeg[s+1]) go to octree s, read from and written
+// to the segment's own buffers.
 template <int kVariant, uint32_t kT>
 __global__ void __launch_bounds__(kT)
-    march_rays_batch(const MarchSeg *__restrict__ segs, const uint64_t *__restrict__ rbeg, uint32_t S,
-                     const float *__restrict__ rays, uint64_t R, uint4 *__restrict__ hits, uint32_t *err_out) {
+    march_rays_batch(const MarchSeg *__restrict__ segs, const uint64_t *__restrict__ rbeg, uint32_t S, uint64_t R,
+                     uint32_t *err_out) {
 	__shared__ uint32_t s_stack[kStackSlots * kT];
 	const uint64_t base = (uint64_t)blockIdx.x * kT, k = base + threadIdx.x;
 	if (k >= R)
 		return;
 	const uint32_t s = S == 1u ? 0u : seg_walk(rbeg, S, seg_find(rbeg, S, base), k);
 	const MarchSeg &g = segs[s];
-	const float *r = rays + 6 * k;
+	const uint64_t local = k - rbeg[s];
+	const float *r = g.rays + 6 * local;
 	f3 o{r[0], r[1], r[2]}, d{r[3], r[4], r[5]};
 	uint32_t err = 0u;
 	uint4 r0, r1;
 	march_leaf<kVariant, kT>(g.oct, g.W, o, d, s_stack + threadIdx.x, err, r0, r1);
-	hits[2 * k] = r0;
-	hits[2 * k + 1] = r1;
+	g.hits[2 * local] = r0;
+	g.hits[2 * local + 1] = r1;
 	if (err)
 		atomicOr(err_out, err);
 }
@@ -810,14 +812,21 @@ void march_plan_release(svo_ctx *ctx, MarchPlan *p) {
 	p->beam_buf = nullptr;
 }
 
-int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, const uint64_t *ray_begin,
-                     const float *rays, int variant, svo_hit *hits) {
-	const uint64_t R = ray_begin[S];
-	if (S == 0 || R == 0)
+int march_rays_segments(svo_ctx *ctx, const svo_octree *const *octs, const float *const *rays, svo_hit *const *hits,
+                        const uint64_t *count, uint32_t S, int variant) {
+	if (S == 0)
 		return SVO_OK;
 	std::vector<MarchSeg> h(S);
-	for (uint32_t s = 0; s < S; ++s)
+	std::vector<uint64_t> rb(S + 1, 0);
+	for (uint32_t s = 0; s < S; ++s) {
 		h[s] = march_seg(octs[s], nullptr);
+		h[s].rays = rays[s];
+		h[s].hits = reinterpret_cast<uint4 *>(hits[s]);
+		rb[s + 1] = rb[s] + count[s];
+	}
+	const uint64_t R = rb[S];
+	if (R == 0)
+		return SVO_OK;
 	MarchSeg *segs = nullptr;
 	uint64_t *rbeg = nullptr;
 	if (ctx->pool->alloc((void **)&segs, sizeof(MarchSeg) * S) != hipSuccess ||
@@ -827,14 +836,13 @@ int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, co
 	}
 	int rc = upload_table(ctx, segs, h.data(), sizeof(MarchSeg) * S);
 	if (!rc)
-		rc = upload_table(ctx, rbeg, ray_begin, sizeof(uint64_t) * (S + 1));
-	uint4 *out = reinterpret_cast<uint4 *>(hits);
+		rc = upload_table(ctx, rbeg, rb.data(), sizeof(uint64_t) * (S + 1));
 	auto run = [&](int sh) -> hipError_t {
 		const uint32_t bs = kRayBlock[sh];
 		const dim3 grid(div_up(R, bs)), block(bs);
 #define SVO_RAYS(V, T)                                                                                                 \
 	launch(ctx, "march_rays_batch", march_rays_batch<V, T>, grid, block, 0, (const MarchSeg *)segs,                    \
-	       (const uint64_t *)rbeg, S, rays, R, out, ctx->d_err)
+	       (const uint64_t *)rbeg, S, R, ctx->d_err)
 		if (variant == SVO_MARCH_LEAF_ITER)
 			return bs == 256 ? SVO_RAYS(0, 256) : bs == 128 ? SVO_RAYS(0, 128) : SVO_RAYS(0, 64);
 		if (variant == SVO_MARCH_LEAF)
@@ -861,6 +869,19 @@ int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, co
 	ctx->pool->free(segs);
 	ctx->pool->free(rbeg);
 	return rc;
+}
+
+int march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t S, const uint64_t *ray_begin,
+                     const float *rays, int variant, svo_hit *hits) {
+	std::vector<const float *> r(S);
+	std::vector<svo_hit *> h(S);
+	std::vector<uint64_t> c(S);
+	for (uint32_t s = 0; s < S; ++s) {
+		r[s] = rays + 6 * ray_begin[s];
+		h[s] = hits + ray_begin[s];
+		c[s] = ray_begin[s + 1] - ray_begin[s];
+	}
+	return march_rays_segments(ctx, octs, r.data(), h.data(), c.data(), S, variant);
 }
 
 } // namespace svo

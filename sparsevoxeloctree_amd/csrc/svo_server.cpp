@@ -1,0 +1,461 @@
+// Serving loop: build and ray-query requests for many scenes arrive at any time and are batched by
+// two worker threads, each with its own context (HIP stream) on the same device, so ray queries
+// never wait behind a build:
+//   ray worker    drains every pending ray query, pins the target octrees and marches all of them
+//                 with one launch per march variant (per-request ray and hit buffers, no copies);
+//   build worker  drains pending builds (up to kMaxBuildBatch), voxelizes them with one count and
+//                 one fill launch, builds them with one launch per pass, then admits each octree
+//                 into the resident set by its exact size: least-recently-used unpinned octrees are
+//                 evicted until it fits the byte budget and the octree-count limit, else the build
+//                 is rejected (SVO_ERR_OUT_OF_MEMORY).
+// Latency is measured on the host from submission to the completion of the request's batch.
+// The reference's nearest analogue is its loader thread, which builds one octree on a side queue
+// and hands it to the renderer (src/LoaderThread.cpp:15-116, src/Octree.cpp:22-35).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "svo_batch.hpp"
+#include "svo_runtime.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+constexpr uint32_t kMaxBuildBatch = 64;
+
+struct Request {
+	uint64_t id = 0;
+	bool build = false;
+	const svo_scene *scene = nullptr;
+	uint32_t level = 0;
+	uint64_t octree = 0;
+	const float *rays = nullptr;
+	uint64_t R = 0;
+	int variant = 0;
+	svo_hit *hits = nullptr;
+	Clock::time_point t_submit, t_done;
+	int status = SVO_OK;
+	bool done = false;
+};
+
+struct Resident {
+	svo_octree *oct = nullptr;
+	uint64_t bytes = 0;
+	uint64_t last_use = 0;
+	uint32_t pins = 0;
+};
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+	return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+double percentile(std::vector<double> v, double p) {
+	if (v.empty())
+		return 0.0;
+	const size_t k = std::min(v.size() - 1, (size_t)(p * (double)(v.size() - 1) + 0.5));
+	std::nth_element(v.begin(), v.begin() + (long)k, v.end());
+	return v[k];
+}
+
+} // namespace
+
+struct svo_server {
+	int device = 0;
+	svo_ctx *cb = nullptr; // build worker's context
+	svo_ctx *cr = nullptr; // ray worker's context
+	uint64_t budget = 0;
+	uint32_t max_octrees = 0;
+	std::mutex mu;
+	std::condition_variable cv_build, cv_rays, cv_done;
+	std::deque<Request *> qb, qr;
+	std::unordered_map<uint64_t, Request *> reqs;
+	std::unordered_map<uint64_t, Resident> res;
+	uint64_t resident_bytes = 0, use_clock = 0, next_id = 1;
+	bool stop = false;
+	std::thread tb, tr;
+	// statistics
+	std::vector<double> lat_build, lat_rays;
+	uint64_t builds_rejected = 0, rays_failed = 0, rays_traced = 0, evictions = 0, batches = 0;
+	double busy_build_ms = 0.0, busy_rays_ms = 0.0;
+	Clock::time_point t_stats = Clock::now();
+
+	void finish(Request *r, int status, Clock::time_point t) { // mu held
+		r->status = status;
+		r->t_done = t;
+		r->done = true;
+		const double ms = ms_between(r->t_submit, t);
+		if (r->build) {
+			if (status == SVO_OK)
+				lat_build.push_back(ms);
+			else
+				++builds_rejected;
+		} else {
+			if (status == SVO_OK) {
+				lat_rays.push_back(ms);
+				rays_traced += r->R;
+			} else {
+				++rays_failed;
+			}
+		}
+	}
+
+	// Evicts least-recently-used unpinned octrees until `need` more bytes (and, with slot, one more
+	// octree) fit.  mu held; only the build worker destroys octrees (they belong to its context).
+	bool make_room(uint64_t need, bool slot) {
+		while (resident_bytes + need > budget || (slot && res.size() + 1 > max_octrees)) {
+			auto victim = res.end();
+			for (auto it = res.begin(); it != res.end(); ++it)
+				if (it->second.pins == 0 && (victim == res.end() || it->second.last_use < victim->second.last_use))
+					victim = it;
+			if (victim == res.end())
+				return false;
+			resident_bytes -= victim->second.bytes;
+			svo_octree_destroy(victim->second.oct);
+			res.erase(victim);
+			++evictions;
+		}
+		return true;
+	}
+
+	void ray_loop();
+	void build_loop();
+	void build_batch_requests(std::vector<Request *> &batch);
+};
+
+void svo_server::ray_loop() {
+	(void)hipSetDevice(device);
+	std::unique_lock<std::mutex> lk(mu);
+	for (;;) {
+		cv_rays.wait(lk, [&] { return stop || !qr.empty(); });
+		if (qr.empty())
+			break; // stop requested and drained
+		std::vector<Request *> batch(qr.begin(), qr.end());
+		qr.clear();
+		const Clock::time_point t0 = Clock::now();
+		// resolve and pin the target octrees
+		std::vector<Request *> run;
+		std::vector<const svo_octree *> octs;
+		for (Request *r : batch) {
+			auto it = res.find(r->octree);
+			if (it == res.end()) {
+				finish(r, SVO_ERR_INVALID_ARGUMENT, t0); // not resident (never built, rejected or evicted)
+				continue;
+			}
+			it->second.pins++;
+			it->second.last_use = ++use_clock;
+			run.push_back(r);
+			octs.push_back(it->second.oct);
+		}
+		if (!run.empty())
+			++batches;
+		lk.unlock();
+		int rc = SVO_OK;
+		for (int variant = 0; variant < 3 && !rc; ++variant) {
+			std::vector<const svo_octree *> o;
+			std::vector<const float *> rays;
+			std::vector<svo_hit *> hits;
+			std::vector<uint64_t> count;
+			for (size_t i = 0; i < run.size(); ++i)
+				if (run[i]->variant == variant) {
+					o.push_back(octs[i]);
+					rays.push_back(run[i]->rays);
+					hits.push_back(run[i]->hits);
+					count.push_back(run[i]->R);
+				}
+			for (size_t b = 0; b < o.size() && !rc; b += svo::kMaxBatch) {
+				const uint32_t n = (uint32_t)std::min<size_t>(svo::kMaxBatch, o.size() - b);
+				rc = svo::march_rays_segments(cr, o.data() + b, rays.data() + b, hits.data() + b, count.data() + b, n,
+				                              variant);
+			}
+		}
+		if (!run.empty() && !rc && hipStreamSynchronize(cr->stream) != hipSuccess)
+			rc = SVO_ERR_HIP;
+		const Clock::time_point t1 = Clock::now();
+		lk.lock();
+		busy_rays_ms += ms_between(t0, t1);
+		for (size_t i = 0; i < run.size(); ++i) {
+			auto it = res.find(run[i]->octree);
+			if (it != res.end())
+				it->second.pins--;
+			finish(run[i], rc, t1);
+		}
+		cv_done.notify_all();
+	}
+}
+
+// Voxelize + build of a batch of build requests; on a batch failure each request is retried alone
+// so that one bad scene cannot fail the others.
+void svo_server::build_batch_requests(std::vector<Request *> &batch) {
+	const uint32_t n = (uint32_t)batch.size();
+	std::vector<const svo_scene *> scenes(n);
+	std::vector<uint32_t> levels(n);
+	for (uint32_t i = 0; i < n; ++i) {
+		scenes[i] = batch[i]->scene;
+		levels[i] = batch[i]->level;
+	}
+	std::vector<svo_fraglist *> fls(n, nullptr);
+	std::vector<svo_octree *> octs(n, nullptr);
+	int rc = svo_voxelize_batch(cb, scenes.data(), levels.data(), n, fls.data());
+	if (!rc) {
+		// transient build scratch: node + bucket (16 B per fragment) plus bitmaps and ranks
+		uint64_t scratch = 0;
+		for (uint32_t i = 0; i < n; ++i) {
+			uint64_t F = 0;
+			svo_fraglist_info(fls[i], nullptr, &F, nullptr);
+			scratch += 20 * F + 8 * svo::build_bitmap_words(levels[i]);
+		}
+		std::unique_lock<std::mutex> lk(mu);
+		if (!make_room(scratch, false))
+			rc = SVO_ERR_OUT_OF_MEMORY;
+	}
+	if (!rc)
+		rc = svo_build_batch(cb, (const svo_fraglist *const *)fls.data(), n, octs.data());
+	if (!rc && hipStreamSynchronize(cb->stream) != hipSuccess)
+		rc = SVO_ERR_HIP;
+	for (uint32_t i = 0; i < n; ++i)
+		if (fls[i])
+			svo_fraglist_destroy(fls[i]);
+	if (rc && n > 1) {
+		for (uint32_t i = 0; i < n; ++i)
+			if (octs[i])
+				svo_octree_destroy(octs[i]);
+		for (uint32_t i = 0; i < n; ++i) {
+			std::vector<Request *> one{batch[i]};
+			build_batch_requests(one);
+		}
+		return;
+	}
+	const Clock::time_point t1 = Clock::now();
+	std::unique_lock<std::mutex> lk(mu);
+	for (uint32_t i = 0; i < n; ++i) {
+		Request *r = batch[i];
+		if (rc) {
+			if (octs[i])
+				svo_octree_destroy(octs[i]);
+			finish(r, rc, t1);
+			continue;
+		}
+		svo_build_finish(cb, octs[i], nullptr);
+		uint64_t W = 0;
+		svo_octree_info(octs[i], nullptr, &W, nullptr);
+		const uint64_t bytes = 4 * W;
+		if (!make_room(bytes, true)) {
+			svo_octree_destroy(octs[i]);
+			finish(r, SVO_ERR_OUT_OF_MEMORY, t1); // rejected by admission
+			continue;
+		}
+		Resident e;
+		e.oct = octs[i];
+		e.bytes = bytes;
+		e.last_use = ++use_clock;
+		res[r->id] = e;
+		resident_bytes += bytes;
+		finish(r, SVO_OK, t1);
+	}
+}
+
+void svo_server::build_loop() {
+	(void)hipSetDevice(device);
+	std::unique_lock<std::mutex> lk(mu);
+	for (;;) {
+		cv_build.wait(lk, [&] { return stop || !qb.empty(); });
+		if (qb.empty())
+			break;
+		std::vector<Request *> batch;
+		while (!qb.empty() && batch.size() < kMaxBuildBatch) {
+			batch.push_back(qb.front());
+			qb.pop_front();
+		}
+		++batches;
+		lk.unlock();
+		const Clock::time_point t0 = Clock::now();
+		build_batch_requests(batch);
+		const Clock::time_point t1 = Clock::now();
+		lk.lock();
+		busy_build_ms += ms_between(t0, t1);
+		cv_done.notify_all();
+	}
+}
+
+extern "C" {
+
+int svo_server_create(int device, uint64_t budget_bytes, uint32_t max_octrees, svo_server **out) {
+	if (!out || max_octrees == 0)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	svo_server *s = new (std::nothrow) svo_server();
+	if (!s)
+		return SVO_ERR_OUT_OF_MEMORY;
+	s->device = device;
+	s->budget = budget_bytes;
+	s->max_octrees = max_octrees;
+	int rc = svo_ctx_create(device, &s->cb);
+	if (!rc)
+		rc = svo_ctx_create(device, &s->cr);
+	if (rc) {
+		if (s->cb)
+			svo_ctx_destroy(s->cb);
+		delete s;
+		return rc;
+	}
+	s->tb = std::thread([s] { s->build_loop(); });
+	s->tr = std::thread([s] { s->ray_loop(); });
+	*out = s;
+	return SVO_OK;
+}
+
+int svo_server_destroy(svo_server *s) {
+	if (!s)
+		return SVO_ERR_INVALID_ARGUMENT;
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		s->stop = true;
+	}
+	s->cv_build.notify_all();
+	s->cv_rays.notify_all();
+	s->tb.join();
+	s->tr.join();
+	for (auto &kv : s->res)
+		svo_octree_destroy(kv.second.oct);
+	for (auto &kv : s->reqs)
+		delete kv.second;
+	svo_ctx_destroy(s->cr);
+	svo_ctx_destroy(s->cb);
+	delete s;
+	return SVO_OK;
+}
+
+int svo_server_submit_build(svo_server *s, const svo_scene *scene, uint32_t level, uint64_t *req_id) {
+	if (!s || !scene || !req_id)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (level < 1 || level > 12)
+		return SVO_ERR_BAD_LEVEL;
+	Request *r = new (std::nothrow) Request();
+	if (!r)
+		return SVO_ERR_OUT_OF_MEMORY;
+	r->build = true;
+	r->scene = scene;
+	r->level = level;
+	r->t_submit = Clock::now();
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		if (s->stop) {
+			delete r;
+			return SVO_ERR_INVALID_ARGUMENT;
+		}
+		r->id = s->next_id++;
+		s->reqs[r->id] = r;
+		s->qb.push_back(r);
+		*req_id = r->id;
+	}
+	s->cv_build.notify_one();
+	return SVO_OK;
+}
+
+int svo_server_submit_rays(svo_server *s, uint64_t octree_id, const float *rays_dev, uint64_t R, int variant,
+                           svo_hit *hits_dev, uint64_t *req_id) {
+	if (!s || !req_id || (R && (!rays_dev || !hits_dev)) || variant < 0 || variant > 2)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (R > 0xFFFFFFFFull)
+		return SVO_ERR_INVALID_ARGUMENT;
+	Request *r = new (std::nothrow) Request();
+	if (!r)
+		return SVO_ERR_OUT_OF_MEMORY;
+	r->octree = octree_id;
+	r->rays = rays_dev;
+	r->R = R;
+	r->variant = variant;
+	r->hits = hits_dev;
+	r->t_submit = Clock::now();
+	{
+		std::lock_guard<std::mutex> lk(s->mu);
+		if (s->stop) {
+			delete r;
+			return SVO_ERR_INVALID_ARGUMENT;
+		}
+		r->id = s->next_id++;
+		s->reqs[r->id] = r;
+		s->qr.push_back(r);
+		*req_id = r->id;
+	}
+	s->cv_rays.notify_one();
+	return SVO_OK;
+}
+
+int svo_server_wait(svo_server *s, uint64_t req_id, int *status, double *latency_ms) {
+	if (!s)
+		return SVO_ERR_INVALID_ARGUMENT;
+	std::unique_lock<std::mutex> lk(s->mu);
+	auto it = s->reqs.find(req_id);
+	if (it == s->reqs.end())
+		return SVO_ERR_INVALID_ARGUMENT;
+	Request *r = it->second;
+	s->cv_done.wait(lk, [&] { return r->done; });
+	if (status)
+		*status = r->status;
+	if (latency_ms)
+		*latency_ms = ms_between(r->t_submit, r->t_done);
+	s->reqs.erase(it);
+	delete r;
+	return SVO_OK;
+}
+
+int svo_server_resident(svo_server *s, uint64_t octree_id, uint64_t *W, const svo_octree **oct) {
+	if (!s)
+		return SVO_ERR_INVALID_ARGUMENT;
+	std::lock_guard<std::mutex> lk(s->mu);
+	auto it = s->res.find(octree_id);
+	if (it == s->res.end())
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (W)
+		*W = it->second.bytes / 4;
+	if (oct)
+		*oct = it->second.oct;
+	return SVO_OK;
+}
+
+int svo_server_get_stats(svo_server *s, svo_server_stats *out) {
+	if (!s || !out)
+		return SVO_ERR_INVALID_ARGUMENT;
+	std::lock_guard<std::mutex> lk(s->mu);
+	out->builds_done = s->lat_build.size();
+	out->builds_rejected = s->builds_rejected;
+	out->ray_queries_done = s->lat_rays.size();
+	out->ray_queries_failed = s->rays_failed;
+	out->rays_traced = s->rays_traced;
+	out->evictions = s->evictions;
+	out->batches = s->batches;
+	out->resident_octrees = s->res.size();
+	out->resident_bytes = s->resident_bytes;
+	out->budget_bytes = s->budget;
+	out->build_p50_ms = percentile(s->lat_build, 0.50);
+	out->build_p99_ms = percentile(s->lat_build, 0.99);
+	out->rays_p50_ms = percentile(s->lat_rays, 0.50);
+	out->rays_p99_ms = percentile(s->lat_rays, 0.99);
+	out->busy_build_ms = s->busy_build_ms;
+	out->busy_rays_ms = s->busy_rays_ms;
+	out->elapsed_ms = ms_between(s->t_stats, Clock::now());
+	return SVO_OK;
+}
+
+int svo_server_reset_stats(svo_server *s) {
+	if (!s)
+		return SVO_ERR_INVALID_ARGUMENT;
+	std::lock_guard<std::mutex> lk(s->mu);
+	s->lat_build.clear();
+	s->lat_rays.clear();
+	s->builds_rejected = s->rays_failed = s->rays_traced = s->evictions = s->batches = 0;
+	s->busy_build_ms = s->busy_rays_ms = 0.0;
+	s->t_stats = Clock::now();
+	return SVO_OK;
+}
+
+} // extern "C"
