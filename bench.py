@@ -9,7 +9,9 @@ The triangles are uploaded and normalised before timing (the reference loads and
 the host once, src/Scene.cpp:36-147).
 
 Prints one JSON line.  `value` is the primary-ray throughput (Mrays/s, beam pre-pass included)
-of the timed steps; the SVO build time is reported in the reference's scope (fill voxelize + all
+of the timed steps; with the default configuration the line also carries `batched` (c4: 256
+scenes through the batched API against one scene per launch) and `served` (c5: the serving loop
+under Poisson arrivals, p50/p99 latency) summaries (skip them with --no-extra); the SVO build time is reported in the reference's scope (fill voxelize + all
 level passes, device time, src/LoaderThread.cpp:57-75) and end to end (count + allocation + fill +
 build, wall clock).  Per-kernel device times come from HIP events on the library's stream
 (svo_prof_*); `roofline` prices the dominant kernel's algorithmic bytes against 8 TB/s.
@@ -310,9 +312,8 @@ def bench_batch(args, cfg):
         "cpu_baseline": None,
         "roofline": None,
     }
-    print(json.dumps(line))
-    sys.stdout.flush()
     ctx.close()
+    return line
 
 
 def bench_serve(args, cfg):
@@ -451,9 +452,8 @@ def bench_serve(args, cfg):
         "roofline": None,
     }
     srv.close()
-    print(json.dumps(line))
-    sys.stdout.flush()
     ctx.close()
+    return line
 
 
 def copy_calibration(mib):
@@ -480,6 +480,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument(
+        "--no-extra", action="store_true", help="with --config 2: skip the batched (c4) and served (c5) summaries"
+    )
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument(
         "--copy-calibration", type=int, default=0, metavar="MIB",
@@ -499,16 +502,18 @@ def main():
     import sparsevoxeloctree_amd as S
 
     cfg = CONFIGS[args.config]
-    if args.config == 4:
-        bench_batch(args, cfg)
-        return
-    if args.config == 5:
-        bench_serve(args, cfg)
+    if args.config in (4, 5):
+        line = (bench_batch if args.config == 4 else bench_serve)(args, cfg)
+        print(json.dumps(line))
+        sys.stdout.flush()
         return
     xyz, rgb = S.gen_scene(cfg["kind"], cfg["seed"], cfg["param"])
     T = rgb.shape[0]
     cam = make_camera(S, cfg, xyz)
     ctx = S.Context(0)
+    # launch schedules (one ray per lane / persistent refill) picked per shape class by the
+    # library's autotuner during the warm-up steps
+    S.autotune(ctx, True)
     scene = S.Scene.Create(ctx, xyz, rgb, normalize=True)
     R = cfg["width"] * cfg["height"]
     bw, bh = (cfg["width"] + 7) // 8 + 1, (cfg["height"] + 7) // 8 + 1
@@ -635,15 +640,39 @@ def main():
         line["bounce_mrays_per_s"] = round(nb / (st["rays_ms"] * 1e-3) / 1e6, 3)
     mem = ctx.memory()
     line["peak_pool_bytes"] = mem["peak_in_use"]
+    line["autotune"] = S.autotune(ctx, True).strip().splitlines()
+    line["reference_lavapipe"] = "unavailable: no Vulkan loader, lavapipe, Xvfb or glslc on the host"
     if not args.no_cpu:
         words = octree.words()
         line["cpu_baseline"] = cpu_baseline(S, cfg, cam, words, args.cpu_budget)
     else:
         line["cpu_baseline"] = None
     del keep, octree, _vox, _builder
+    ctx.close()
+    if args.config == 2 and not args.no_extra:
+        # the metric's batched and served figures: c4 and c5 at their own configurations
+        line["batched"] = summarise_extra(bench_batch, args, CONFIGS[4], (
+            "value", "batched_builds_per_s", "sequential_builds_per_s", "batched_march_ms", "sequential_march_ms",
+            "speedup_build", "speedup_march", "speedup_total", "hits_identical_to_sequential", "fragments"))
+        line["served"] = summarise_extra(bench_serve, args, CONFIGS[5], (
+            "value", "ray_p50_ms", "ray_p99_ms", "build_p50_ms", "build_p99_ms", "builds_per_s",
+            "ray_queries_per_s", "builds_rejected", "ray_queries_failed", "evictions"))
     print(json.dumps(line))
     sys.stdout.flush()
-    ctx.close()
+
+
+def summarise_extra(fn, args, cfg, keys):
+    """Runs the c4 / c5 benchmark and keeps its headline fields; an error is recorded, not raised,
+    so the primary line is always printed."""
+    try:
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = 2, 1
+        full = fn(sub, cfg)
+        out = {"workload": cfg["workload"], "unit": full["unit"]}
+        out.update({k: full[k] for k in keys})
+        return out
+    except Exception as exc:  # noqa: BLE001
+        return {"workload": cfg["workload"], "error": "%s: %s" % (type(exc).__name__, exc)}
 
 
 if __name__ == "__main__":

@@ -223,6 +223,12 @@ int svo_march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t n
  * new shape class (march batch: n, width, height, beam; ray batch: variant, log2 R) times every
  * candidate workgroup/tile shape once and keeps the fastest; the report lists the decisions. */
 int svo_autotune(svo_ctx *ctx, int on);
+/* Schedule of svo_march, svo_march_rays, svo_march_rays_batch and the server's ray batches:
+ * 1 = a fixed grid of waves whose lanes take a new ray as soon as theirs has ended (rays dealt to
+ * the waves in 64-ray chunks; wins when ray lengths diverge); 0 = one ray per lane (wins
+ * for coherent rays); -1 (default) = chosen per shape class by the autotuner when svo_autotune is
+ * on, else 0.  Results are identical. */
+int svo_march_mode(svo_ctx *ctx, int mode);
 int svo_autotune_report(svo_ctx *ctx, char *buf, uint64_t cap, uint64_t *len);
 
 /* One cosine-weighted diffuse bounce per hit (Diffuse_Sample + AlignDirection,

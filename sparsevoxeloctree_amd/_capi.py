@@ -158,6 +158,7 @@ _SIGS = {
     "svo_march_plan_destroy": (i32, [P]),
     "svo_march_rays_batch": (i32, [P, PP, u32, C.POINTER(u64), P, i32, P]),
     "svo_autotune": (i32, [P, i32]),
+    "svo_march_mode": (i32, [P, i32]),
     "svo_server_create": (i32, [i32, u64, u32, PP]),
     "svo_server_destroy": (i32, [P]),
     "svo_server_submit_build": (i32, [P, P, u32, C.POINTER(u64)]),

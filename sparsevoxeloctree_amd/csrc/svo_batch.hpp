@@ -146,6 +146,8 @@ struct BuildBatch {
 	const uint64_t *bbeg() const { return index + (S + 1); }
 	const uint64_t *dw(uint32_t d) const { return index + (2 + d) * (uint64_t)(S + 1); }
 	uint64_t h_dw(uint32_t d, uint32_t s) const { return h_index[(2 + d) * (uint64_t)(S + 1) + s]; }
+	const uint64_t *dr(uint32_t d) const { return index + (3 + kMaxLevel + d) * (uint64_t)(S + 1); }
+	uint64_t h_dr(uint32_t d, uint32_t s) const { return h_index[(3 + kMaxLevel + d) * (uint64_t)(S + 1) + s]; }
 };
 
 void vox_batch_release(svo_ctx *ctx, VoxBatch *vb);

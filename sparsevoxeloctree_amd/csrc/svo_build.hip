@@ -203,11 +203,35 @@ __global__ void bm_levels(BuildSeg *segs, uint32_t S, const uint32_t *__restrict
 	g.nbins = nlast;
 }
 
-// Blocks of depth d (the children of depth-(d-1) nodes), one thread per word of B[d] = 4 parents;
-// items are the words of B[d] of the scenes with L > d (begin table dw).  Every block is written
-// whole: pointer words for occupied children, zeros otherwise.
+// The root block of every scene with L > 1 (one thread per scene): its 8 words point to the depth-1
+// blocks of the occupied children (bits of B[1]) or are 0.
+__global__ void bm_write_root(const BuildSeg *__restrict__ segs, uint32_t S, const uint32_t *__restrict__ B) {
+	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
+		return;
+	const BuildSeg &g = segs[s];
+	if (g.L <= 1u)
+		return; // the root block holds the leaves (build_fold)
+	const uint32_t byte = B[g.bm + g.off[1]] & 0xffu, fbd = g.fb[1];
+	uint32_t wd[8], r = 0u;
+#pragma unroll
+	for (uint32_t t = 0; t < 8u; ++t) {
+		const bool set = (byte >> t) & 1u;
+		wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
+		r += set ? 1u : 0u;
+	}
+	uint4 *dst = reinterpret_cast<uint4 *>(g.words);
+	dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+	dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+}
+
+// Blocks of depth d >= 2 (the children of depth-(d-1) nodes), one thread per word of B[d-1]: for
+// every occupied parent q of the word, in order, its block (the parent's rank among depth d-1)
+// is written whole: pointer words for the occupied children (bits of byte q of B[d]), zeros
+// otherwise.  Items are the words of B[d-1] of the scenes with L > d (begin table dr[d-1]), so
+// the scan covers the parents' bitmap, 8x smaller than the children's.
 __global__ void __launch_bounds__(kThreads)
-    bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dw, uint32_t S, uint64_t n,
+    bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dp, uint32_t S, uint64_t n,
                     uint32_t d, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	uint32_t hint = 0;
@@ -215,23 +239,26 @@ __global__ void __launch_bounds__(kThreads)
 		const uint64_t i = base + threadIdx.x;
 		if (i >= n)
 			continue;
-		with_seg<kThreads>(dw, S, hint, base, i, [&](uint32_t s) {
+		with_seg<kThreads>(dp, S, hint, base, i, [&](uint32_t s) {
 			const BuildSeg &g = segs[s];
-			const uint64_t w = i - dw[s];
+			const uint64_t w = i - dp[s];
 			const uint32_t *Bs = B + g.bm, *Rs = R + g.bm;
-			const uint64_t offd = g.off[d];
-			const uint32_t v = Bs[offd + w];
-			const uint32_t base_rank = Rs[offd + w] - Rs[offd];
-			const uint32_t fbd = g.fb[d], fbp = g.fb[d - 1u];
+			const uint64_t offp = g.off[d - 1u], offc = g.off[d];
+			uint32_t v = Bs[offp + w];
+			if (v == 0u)
+				return;
+			const uint32_t fbd = g.fb[d];
+			uint32_t block = g.fb[d - 1u] + (Rs[offp + w] - Rs[offp]); // first parent's block
+			const uint32_t rc0 = Rs[offc];
 			uint32_t *oct = g.words;
-#pragma unroll
-			for (uint32_t k = 0; k < 4u; ++k) {
-				const uint32_t byte = (v >> (8u * k)) & 0xffu;
-				const uint64_t q = 4u * w + k; // parent at depth d-1
-				if (d == 1u ? q != 0u : byte == 0u)
-					continue; // the root block always exists
-				const uint32_t block = d == 1u ? 0u : fbp + node_rank(Bs, Rs, g.off[d - 1u], q);
-				uint32_t r = base_rank + (uint32_t)__popc(v & ((1u << (8u * k)) - 1u));
+			while (v) {
+				const uint32_t b = (uint32_t)__ffs(v) - 1u;
+				v &= v - 1u;
+				const uint64_t q = 32u * w + b; // parent node index at depth d-1
+				const uint64_t cw = offc + (q >> 2);
+				const uint32_t cv = Bs[cw], sh = 8u * (uint32_t)(q & 3u);
+				const uint32_t byte = (cv >> sh) & 0xffu;
+				uint32_t r = (Rs[cw] - rc0) + (uint32_t)__popc(cv & ((1u << sh) - 1u));
 				uint32_t wd[8];
 #pragma unroll
 				for (uint32_t t = 0; t < 8u; ++t) {
@@ -242,6 +269,7 @@ __global__ void __launch_bounds__(kThreads)
 				uint4 *dst = reinterpret_cast<uint4 *>(oct + 8u * (uint64_t)block);
 				dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
 				dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+				++block;
 			}
 		});
 	}
@@ -765,10 +793,14 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	const uint32_t S = bb.S;
 	const BuildSeg *segs = bb.segs;
 	hipError_t e;
-	for (uint32_t d = 1; d + 1 <= bb.maxL; ++d) {
-		const uint64_t n = bb.h_dw(d, S);
+	if (bb.maxL > 1u &&
+	    (e = launch(ctx, "bm_write_root", bm_write_root, dim3(div_up(S, 64)), dim3(64), 0, segs, S,
+	                (const uint32_t *)bb.bm)) != hipSuccess)
+		return hip_check(ctx, e, "bm_write_root");
+	for (uint32_t d = 2; d + 1 <= bb.maxL; ++d) {
+		const uint64_t n = bb.h_dr(d - 1, S); // words of B[d-1], scenes with L > d
 		if (n > 0 && (e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(n, ctx)), dim3(kThreads), 0,
-		                         segs, bb.dw(d), S, n, d, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk)) !=
+		                         segs, bb.dr(d - 1), S, n, d, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk)) !=
 		                 hipSuccess)
 			return hip_check(ctx, e, "bm_write_blocks");
 	}

@@ -1109,6 +1109,13 @@ int svo_march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t n
 	return march_rays_batch(ctx, octs, n, ray_begin, rays_dev, variant, hits_dev);
 }
 
+int svo_march_mode(svo_ctx *ctx, int mode) {
+	if (!ctx || mode < -1 || mode > 1)
+		return SVO_ERR_INVALID_ARGUMENT;
+	ctx->march_mode = mode;
+	return SVO_OK;
+}
+
 int svo_autotune(svo_ctx *ctx, int on) {
 	if (!ctx)
 		return SVO_ERR_INVALID_ARGUMENT;

@@ -193,23 +193,11 @@ __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uin
 	return true;
 }
 
-// Octree_RayMarchLeaf (variant 0: 6-arg with iteration count; 1: 5-arg) and
-// Octree_RayMarchOcclude (variant 2), with the outputs of octree.glsl:307-339.
-template <int kVariant, uint32_t kStride = kBlock>
-__device__ __forceinline__ void march_leaf(const uint32_t *__restrict__ oct, uint64_t W, f3 o, f3 d, uint32_t *stk,
-                                           uint32_t &err, uint4 &out0, uint4 &out1) {
-	March m;
-	uint32_t nfetch = 0u, iter = 0u;
-	march_init<kVariant != 0>(o, d, m);
-	while (m.scale < kStackSize) {
-		++iter;
-		if (iter > kMaxIter) {
-			err |= kErrIterCap;
-			break;
-		}
-		if (!march_step<false, kStride>(oct, W, m, stk, nfetch, err, 0.0f, 0.0f))
-			break;
-	}
+// Outputs of Octree_RayMarchLeaf / Octree_RayMarchOcclude (octree.glsl:307-339) from the final
+// traversal state; d is the direction as march_init left it.
+template <int kVariant>
+__device__ __forceinline__ void march_finish(const March &m, f3 o, f3 d, uint32_t iter, uint32_t nfetch, uint4 &out0,
+                                             uint4 &out1) {
 	const bool hit = m.scale < kStackSize && m.t_min <= m.t_max;
 	if (kVariant == 2) {
 		out0 = make_uint4(0u, 0u, 0u, 0u);
@@ -255,6 +243,33 @@ __device__ __forceinline__ void march_leaf(const uint32_t *__restrict__ oct, uin
 	}
 	out0 = make_uint4(fbits(op.x), fbits(op.y), fbits(op.z), kVariant == 0 ? iter : 0u);
 	out1 = make_uint4(vxy, vzf, m.cur, nfetch);
+}
+
+// One trip of the leaf march loop (octree.glsl:219-305 `while (scale < STACK_SIZE)`); returns false
+// when the loop has ended.
+template <uint32_t kStride>
+__device__ __forceinline__ bool march_trip(const uint32_t *__restrict__ oct, uint64_t W, March &m, uint32_t *stk,
+                                           uint32_t &iter, uint32_t &nfetch, uint32_t &err) {
+	if (m.scale >= kStackSize)
+		return false;
+	if (++iter > kMaxIter) {
+		err |= kErrIterCap;
+		return false;
+	}
+	return march_step<false, kStride>(oct, W, m, stk, nfetch, err, 0.0f, 0.0f);
+}
+
+// Octree_RayMarchLeaf (variant 0: 6-arg with iteration count; 1: 5-arg) and
+// Octree_RayMarchOcclude (variant 2), with the outputs of octree.glsl:307-339.
+template <int kVariant, uint32_t kStride = kBlock>
+__device__ __forceinline__ void march_leaf(const uint32_t *__restrict__ oct, uint64_t W, f3 o, f3 d, uint32_t *stk,
+                                           uint32_t &err, uint4 &out0, uint4 &out1) {
+	March m;
+	uint32_t nfetch = 0u, iter = 0u;
+	march_init<kVariant != 0>(o, d, m);
+	while (march_trip<kStride>(oct, W, m, stk, iter, nfetch, err)) {
+	}
+	march_finish<kVariant>(m, o, d, iter, nfetch, out0, out1);
 }
 
 // Camera_GenRay (shader/camera.glsl:11-14)
@@ -310,19 +325,29 @@ __device__ __forceinline__ float beam_cell(const uint32_t *__restrict__ oct, uin
 
 // One pixel of the primary march (octree_tracer.frag:23-37): ray through the pixel corner, start
 // moved to the minimum of the 4 beam texels around it, 6-argument leaf march.
-template <uint32_t kStride>
-__device__ __forceinline__ void frame_pixel(const uint32_t *__restrict__ oct, uint64_t W, const CamArgs &cam,
-                                            uint32_t width, uint32_t height, const float *__restrict__ beam, uint32_t bw,
-                                            uint32_t px, uint32_t py, uint32_t *stk, uint32_t &err, uint4 &r0,
-                                            uint4 &r1) {
-	f3 o{cam.px, cam.py, cam.pz};
-	f3 d = gen_ray(cam, (float)px / (float)width, (float)py / (float)height);
+// Primary ray through pixel (px, py) (octree_tracer.frag:23-32): through the pixel corner, start
+// moved to the minimum of the 4 beam texels around it.
+__device__ __forceinline__ void frame_ray(const CamArgs &cam, uint32_t width, uint32_t height,
+                                          const float *__restrict__ beam, uint32_t bw, uint32_t px, uint32_t py, f3 &o,
+                                          f3 &d) {
+	o = f3{cam.px, cam.py, cam.pz};
+	d = gen_ray(cam, (float)px / (float)width, (float)py / (float)height);
 	if (beam) {
 		const uint32_t bx = px >> 3, by = py >> 3;
 		const float *b0 = beam + by * bw + bx, *b1 = b0 + bw;
 		float bm = gmin(gmin(b0[0], b0[1]), gmin(b1[0], b1[1]));
 		o = f3{o.x + d.x * bm, o.y + d.y * bm, o.z + d.z * bm};
 	}
+}
+
+// One pixel of the primary march (octree_tracer.frag:23-37), 6-argument leaf march.
+template <uint32_t kStride>
+__device__ __forceinline__ void frame_pixel(const uint32_t *__restrict__ oct, uint64_t W, const CamArgs &cam,
+                                            uint32_t width, uint32_t height, const float *__restrict__ beam, uint32_t bw,
+                                            uint32_t px, uint32_t py, uint32_t *stk, uint32_t &err, uint4 &r0,
+                                            uint4 &r1) {
+	f3 o, d;
+	frame_ray(cam, width, height, beam, bw, px, py, o, d);
 	march_leaf<0, kStride>(oct, W, o, d, stk, err, r0, r1);
 }
 
@@ -455,6 +480,142 @@ __global__ void __launch_bounds__(kBlock)
 		atomicOr(err_out, err);
 }
 
+// ---- persistent march with per-lane ray refill ------------------------------------------------
+// Rays of one march diverge: a wave lasts as long as its longest ray while finished lanes idle.
+// Here a fixed grid of waves marches all rays; a lane whose ray has ended writes its record and,
+// once kRefill lanes of the wave are idle, they take the next rays of the wave's pool.  The rays
+// are dealt to the waves in chunks of 64 (an 8x8-pixel tile of a frame), chunk c to wave
+// c mod (number of waves), so costly image regions spread over all waves and no atomics are
+// needed.  Every ray runs the same per-ray arithmetic as the one-ray-per-lane kernels, so the
+// records are identical; only which lane marches which ray changes.
+constexpr uint32_t kRefill = 16; // idle lanes that trigger a refill
+constexpr uint32_t kTrips = 4;   // loop trips between refill checks
+constexpr uint32_t kChunk = 64;  // rays per chunk
+
+// Ray sources: ray(k, ...) sets up ray k (false: no ray, e.g. a pixel outside the frame) and the
+// address of its 32-byte record.
+struct FrameRays { // primary rays of one frame, numbered in 8x8-pixel tiles
+	const uint32_t *oct;
+	uint64_t W;
+	CamArgs cam;
+	uint32_t width, height, bw, tiles_x;
+	const float *beam;
+	uint4 *hits;
+	__device__ bool ray(uint64_t k, f3 &o, f3 &d, const uint32_t *&oc, uint64_t &w, uint4 *&out) const {
+		const uint32_t tile = (uint32_t)(k >> 6), q = (uint32_t)k & 63u;
+		const uint32_t px = (tile % tiles_x) * 8u + (q & 7u), py = (tile / tiles_x) * 8u + (q >> 3);
+		if (px >= width || py >= height)
+			return false;
+		frame_ray(cam, width, height, beam, bw, px, py, o, d);
+		oc = oct;
+		w = W;
+		out = hits + 2 * ((uint64_t)py * width + px);
+		return true;
+	}
+};
+struct ExplicitRays { // R x {o, d} against one octree
+	const uint32_t *oct;
+	uint64_t W;
+	const float *rays;
+	uint4 *hits;
+	__device__ bool ray(uint64_t k, f3 &o, f3 &d, const uint32_t *&oc, uint64_t &w, uint4 *&out) const {
+		const float *r = rays + 6 * k;
+		o = f3{r[0], r[1], r[2]};
+		d = f3{r[3], r[4], r[5]};
+		oc = oct;
+		w = W;
+		out = hits + 2 * k;
+		return true;
+	}
+};
+struct SegmentRays { // rays [rbeg[s], rbeg[s+1]) against octree s, per-segment buffers
+	const MarchSeg *segs;
+	const uint64_t *rbeg;
+	uint32_t S;
+	__device__ bool ray(uint64_t k, f3 &o, f3 &d, const uint32_t *&oc, uint64_t &w, uint4 *&out) const {
+		const uint32_t s = S == 1u ? 0u : seg_find(rbeg, S, k);
+		const MarchSeg &g = segs[s];
+		const uint64_t local = k - rbeg[s];
+		const float *r = g.rays + 6 * local;
+		o = f3{r[0], r[1], r[2]};
+		d = f3{r[3], r[4], r[5]};
+		oc = g.oct;
+		w = g.W;
+		out = g.hits + 2 * local;
+		return true;
+	}
+};
+
+template <int kVariant, uint32_t kT, class Src>
+__global__ void __launch_bounds__(kT)
+    march_persist(Src src, uint64_t n, uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * kT];
+	uint32_t *stk = s_stack + threadIdx.x;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t wave = (uint64_t)blockIdx.x * (kT / 64u) + (threadIdx.x >> 6);
+	const uint64_t nwaves = (uint64_t)gridDim.x * (kT / 64u);
+	const uint64_t below = (1ull << lane) - 1ull;
+	March m;
+	f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f};
+	const uint32_t *oct = nullptr;
+	uint64_t W = 0;
+	uint4 *out = nullptr;
+	uint32_t iter = 0u, nfetch = 0u, err = 0u;
+	bool active = false;
+	// wave-uniform: the pool [pool_next, pool_end) and the wave's next chunk
+	uint64_t pool_next = 0, pool_end = 0, chunk = wave;
+	bool exhausted = false;
+	for (;;) {
+		const uint64_t idle = __ballot(!active);
+		const uint32_t nidle = (uint32_t)__popcll(idle);
+		if (nidle >= kRefill && (!exhausted || pool_next < pool_end)) {
+			if (pool_next == pool_end) {
+				const uint64_t lo = chunk * kChunk;
+				if (lo < n) {
+					pool_next = lo;
+					pool_end = lo + kChunk < n ? lo + kChunk : n;
+					chunk += nwaves;
+				} else {
+					exhausted = true;
+				}
+			}
+			if (pool_next < pool_end) {
+				if (!active) {
+					const uint64_t k = pool_next + (uint32_t)__popcll(idle & below);
+					if (k < pool_end && src.ray(k, o, d, oct, W, out)) {
+						march_init<kVariant != 0>(o, d, m);
+						iter = 0u;
+						nfetch = 0u;
+						active = true;
+					}
+				}
+				pool_next = pool_next + nidle < pool_end ? pool_next + nidle : pool_end;
+			}
+		}
+		if (__ballot(active) == 0ull) {
+			if (exhausted && pool_next == pool_end)
+				break;
+			continue;
+		}
+		// kTrips trips; a lane whose ray ends stops stepping and writes its record after the trips,
+		// together with the other lanes that ended in them
+		bool marching = active;
+#pragma unroll 1
+		for (uint32_t t = 0; t < kTrips; ++t)
+			if (marching)
+				marching = march_trip<kT>(oct, W, m, stk, iter, nfetch, err);
+		if (active && !marching) {
+			uint4 r0, r1;
+			march_finish<kVariant>(m, o, d, iter, nfetch, r0, r1);
+			out[0] = r0;
+			out[1] = r1;
+			active = false;
+		}
+	}
+	if (err)
+		atomicOr(err_out, err);
+}
+
 // ---- bounce sampling ---------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t sobol0(uint32_t n) { return __brev(n); }
 __device__ __forceinline__ uint32_t sobol1(uint32_t n) { // primitive polynomial x + 1
@@ -536,6 +697,66 @@ uint64_t splitmix64(uint64_t &s) {
 	return z ^ (z >> 31);
 }
 
+// Times fn(shape) for every candidate (one warm-up, best of two timed runs) and returns the fastest;
+// the decision is kept per key and logged.
+template <class Fn>
+int tune(svo_ctx *ctx, const std::string &key, int nshapes, const char *const *names, Fn fn, int &choice) {
+	auto it = ctx->tune_choice.find(key);
+	if (it != ctx->tune_choice.end()) {
+		choice = it->second;
+		return SVO_OK;
+	}
+	hipEvent_t a = nullptr, b = nullptr;
+	hipError_t e = hipEventCreate(&a);
+	if (e == hipSuccess)
+		e = hipEventCreate(&b);
+	float best = 3.4e38f;
+	int bi = 0;
+	std::string line = key + ":";
+	for (int sh = 0; sh < nshapes && e == hipSuccess; ++sh) {
+		if ((e = fn(sh)) != hipSuccess)
+			break;
+		float tmin = 3.4e38f;
+		for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+			if ((e = hipEventRecord(a, ctx->stream)) != hipSuccess || (e = fn(sh)) != hipSuccess ||
+			    (e = hipEventRecord(b, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess)
+				break;
+			float ms = 0.0f;
+			if ((e = hipEventElapsedTime(&ms, a, b)) == hipSuccess)
+				tmin = std::min(tmin, ms);
+		}
+		char buf[64];
+		std::snprintf(buf, sizeof buf, " %s %.4f ms", names[sh], (double)tmin);
+		line += buf;
+		if (tmin < best) {
+			best = tmin;
+			bi = sh;
+		}
+	}
+	if (a)
+		(void)hipEventDestroy(a);
+	if (b)
+		(void)hipEventDestroy(b);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "autotune");
+	ctx->tune_choice[key] = bi;
+	ctx->tune_log += line + " -> " + names[bi] + "\n";
+	choice = bi;
+	return SVO_OK;
+}
+
+const char *const kScheduleName[2] = {"one-ray-per-lane", "persistent-refill"};
+
+// Persistent march of n rays from src; name is the profiler's name of the march.
+template <int kVariant, class Src>
+hipError_t launch_persist(svo_ctx *ctx, const char *name, const Src &src, uint64_t n) {
+	uint64_t grid = (n + kBlock - 1) / kBlock;
+	const uint64_t cap = (uint64_t)ctx->cu_count * 6u; // 6 blocks of 4 waves per CU: the occupancy limit
+	grid = grid < cap ? grid : cap;
+	return launch(ctx, name, march_persist<kVariant, kBlock, Src>, dim3((uint32_t)(grid ? grid : 1)), dim3(kBlock), 0,
+	              src, n, ctx->d_err);
+}
+
 } // namespace
 
 int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam_img,
@@ -570,11 +791,31 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 		if (rc)
 			return rc;
 	}
-	const uint32_t tiles_x = (w + 15u) / 16u, tiles_y = (h + 15u) / 16u;
+	auto run = [&](int mode) -> hipError_t {
+		if (mode == 1) {
+			const uint32_t tx8 = (w + 7u) / 8u, ty8 = (h + 7u) / 8u;
+			FrameRays src{(const uint32_t *)oct->words, oct->W, ca, w, h, bw, tx8, beam_img,
+			              reinterpret_cast<uint4 *>(hits)};
+			return launch_persist<0>(ctx, "march_frame", src, (uint64_t)tx8 * ty8 * 64u);
+		}
+		const uint32_t tiles_x = (w + 15u) / 16u, tiles_y = (h + 15u) / 16u;
+		return launch(ctx, "march_frame", march_frame_kernel, dim3(tiles_x * tiles_y), dim3(kBlock), 0,
+		              (const uint32_t *)oct->words, oct->W, ca, w, h, (const float *)beam_img, bw, tiles_x,
+		              reinterpret_cast<uint4 *>(hits), ctx->d_err);
+	};
+	int mode = ctx->march_mode;
+	if (mode < 0) {
+		mode = 0;
+		if (ctx->autotune) {
+			char key[96];
+			std::snprintf(key, sizeof key, "march_frame %ux%u level=%u beam=%d", w, h, oct->level, beam ? 1 : 0);
+			const int rc = tune(ctx, key, 2, kScheduleName, run, mode);
+			if (rc)
+				return rc;
+		}
+	}
 	ctx->t_march.start(ctx->stream);
-	e = launch(ctx, "march_frame", march_frame_kernel, dim3(tiles_x * tiles_y), dim3(kBlock), 0,
-	           (const uint32_t *)oct->words, oct->W, ca, w, h, (const float *)beam_img, bw, tiles_x,
-	           reinterpret_cast<uint4 *>(hits), ctx->d_err);
+	e = run(mode);
 	ctx->t_march.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "march_frame");
@@ -589,16 +830,37 @@ int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t 
 	hipError_t e;
 	const dim3 grid(div_up(R, kBlock));
 	uint4 *out = reinterpret_cast<uint4 *>(hits);
+	const ExplicitRays src{(const uint32_t *)oct->words, oct->W, rays, out};
+	auto run = [&](int mode) -> hipError_t {
+		if (mode == 1)
+			return variant == SVO_MARCH_LEAF_ITER ? launch_persist<0>(ctx, "march_rays6", src, R)
+			       : variant == SVO_MARCH_LEAF    ? launch_persist<1>(ctx, "march_rays5", src, R)
+			                                      : launch_persist<2>(ctx, "march_occlude", src, R);
+		if (variant == SVO_MARCH_LEAF_ITER)
+			return launch(ctx, "march_rays6", march_rays_kernel<0>, grid, dim3(kBlock), 0,
+			              (const uint32_t *)oct->words, oct->W, rays, R, out, ctx->d_err);
+		if (variant == SVO_MARCH_LEAF)
+			return launch(ctx, "march_rays5", march_rays_kernel<1>, grid, dim3(kBlock), 0,
+			              (const uint32_t *)oct->words, oct->W, rays, R, out, ctx->d_err);
+		return launch(ctx, "march_occlude", march_rays_kernel<2>, grid, dim3(kBlock), 0, (const uint32_t *)oct->words,
+		              oct->W, rays, R, out, ctx->d_err);
+	};
+	int mode = ctx->march_mode;
+	if (mode < 0) {
+		mode = 0;
+		if (ctx->autotune) {
+			uint32_t lg = 0;
+			while ((1ull << (lg + 1)) <= R)
+				++lg;
+			char key[96];
+			std::snprintf(key, sizeof key, "march_rays variant=%d level=%u R~2^%u", variant, oct->level, lg);
+			const int rc = tune(ctx, key, 2, kScheduleName, run, mode);
+			if (rc)
+				return rc;
+		}
+	}
 	ctx->t_rays.start(ctx->stream);
-	if (variant == SVO_MARCH_LEAF_ITER)
-		e = launch(ctx, "march_rays6", march_rays_kernel<0>, grid, dim3(kBlock), 0, (const uint32_t *)oct->words,
-		           oct->W, rays, R, out, ctx->d_err);
-	else if (variant == SVO_MARCH_LEAF)
-		e = launch(ctx, "march_rays5", march_rays_kernel<1>, grid, dim3(kBlock), 0, (const uint32_t *)oct->words,
-		           oct->W, rays, R, out, ctx->d_err);
-	else
-		e = launch(ctx, "march_occlude", march_rays_kernel<2>, grid, dim3(kBlock), 0, (const uint32_t *)oct->words,
-		           oct->W, rays, R, out, ctx->d_err);
+	e = run(mode);
 	ctx->t_rays.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "march_rays");
@@ -681,54 +943,6 @@ hipError_t plan_enqueue(svo_ctx *ctx, const MarchPlan &p, int shape, svo_hit *hi
 	}
 	ctx->prof.enabled = saved;
 	return e;
-}
-
-// Times fn(shape) for every candidate (one warm-up, best of two timed runs) and returns the fastest;
-// the decision is kept per key and logged.
-template <class Fn>
-int tune(svo_ctx *ctx, const std::string &key, int nshapes, const char *const *names, Fn fn, int &choice) {
-	auto it = ctx->tune_choice.find(key);
-	if (it != ctx->tune_choice.end()) {
-		choice = it->second;
-		return SVO_OK;
-	}
-	hipEvent_t a = nullptr, b = nullptr;
-	hipError_t e = hipEventCreate(&a);
-	if (e == hipSuccess)
-		e = hipEventCreate(&b);
-	float best = 3.4e38f;
-	int bi = 0;
-	std::string line = key + ":";
-	for (int sh = 0; sh < nshapes && e == hipSuccess; ++sh) {
-		if ((e = fn(sh)) != hipSuccess)
-			break;
-		float tmin = 3.4e38f;
-		for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-			if ((e = hipEventRecord(a, ctx->stream)) != hipSuccess || (e = fn(sh)) != hipSuccess ||
-			    (e = hipEventRecord(b, ctx->stream)) != hipSuccess || (e = hipEventSynchronize(b)) != hipSuccess)
-				break;
-			float ms = 0.0f;
-			if ((e = hipEventElapsedTime(&ms, a, b)) == hipSuccess)
-				tmin = std::min(tmin, ms);
-		}
-		char buf[64];
-		std::snprintf(buf, sizeof buf, " %s %.4f ms", names[sh], (double)tmin);
-		line += buf;
-		if (tmin < best) {
-			best = tmin;
-			bi = sh;
-		}
-	}
-	if (a)
-		(void)hipEventDestroy(a);
-	if (b)
-		(void)hipEventDestroy(b);
-	if (e != hipSuccess)
-		return hip_check(ctx, e, "autotune");
-	ctx->tune_choice[key] = bi;
-	ctx->tune_log += line + " -> " + names[bi] + "\n";
-	choice = bi;
-	return SVO_OK;
 }
 
 } // namespace
@@ -850,6 +1064,17 @@ int march_rays_segments(svo_ctx *ctx, const svo_octree *const *octs, const float
 		return bs == 256 ? SVO_RAYS(2, 256) : bs == 128 ? SVO_RAYS(2, 128) : SVO_RAYS(2, 64);
 #undef SVO_RAYS
 	};
+	if (!rc && ctx->march_mode == 1) { // explicit choice; the block-size autotuner covers the default
+		const SegmentRays src{(const MarchSeg *)segs, (const uint64_t *)rbeg, S};
+		ctx->t_rays.start(ctx->stream);
+		const hipError_t e = variant == SVO_MARCH_LEAF_ITER ? launch_persist<0>(ctx, "march_rays_batch", src, R)
+		                     : variant == SVO_MARCH_LEAF    ? launch_persist<1>(ctx, "march_rays_batch", src, R)
+		                                                    : launch_persist<2>(ctx, "march_rays_batch", src, R);
+		ctx->t_rays.stop(ctx->stream);
+		ctx->pool->free(segs);
+		ctx->pool->free(rbeg);
+		return hip_check(ctx, e, "march_rays_batch");
+	}
 	int shape = 0;
 	if (!rc && ctx->autotune) {
 		uint32_t lg = 0;

@@ -131,6 +131,9 @@ struct svo_ctx {
 	// launch-shape autotuner: off by default; when on, a batched launch of a new shape class times
 	// its candidate shapes once and keeps the fastest
 	bool autotune = false;
+	// march schedule: 1 = persistent waves with per-lane ray refill, 0 = one ray per lane,
+	// -1 = chosen per shape class by the autotuner when it is on, else one ray per lane
+	int march_mode = -1;
 	std::map<std::string, int> tune_choice;
 	std::string tune_log;
 };

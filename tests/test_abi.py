@@ -98,3 +98,27 @@ def test_no_device_is_reported():
     h = ctypes.c_void_p()
     rc = lib.svo_ctx_create(1 << 20, ctypes.byref(h))
     assert rc == 6 and not h.value
+
+
+def test_batch_and_server_argument_checks():
+    # argument errors are reported before any device work (no GPU needed)
+    import ctypes
+
+    from sparsevoxeloctree_amd._capi import lib, svo_server_stats
+
+    assert ctypes.sizeof(svo_server_stats) == 10 * 8 + 7 * 8
+    h = ctypes.c_void_p()
+    assert lib.svo_server_create(1 << 20, 1 << 30, 0, ctypes.byref(h)) == 1  # max_octrees 0
+    assert lib.svo_server_create(1 << 20, 1 << 30, 8, ctypes.byref(h)) == 6 and not h.value  # no such device
+    rid = ctypes.c_uint64()
+    assert lib.svo_server_submit_build(None, None, 5, ctypes.byref(rid)) == 1
+    assert lib.svo_server_submit_rays(None, 1, None, 0, 0, None, ctypes.byref(rid)) == 1
+    assert lib.svo_server_wait(None, 1, None, None) == 1
+    assert lib.svo_server_destroy(None) == 1
+    assert lib.svo_march_mode(None, 0) == 1
+    assert lib.svo_autotune(None, 1) == 1
+    assert lib.svo_voxelize_batch(None, None, None, 0, None) == 1
+    assert lib.svo_build_batch(None, None, 0, None) == 1
+    assert lib.svo_march_batch(None, None, None, 0, 8, 8, 0, None) == 1
+    assert lib.svo_march_plan_destroy(None) == 1
+    assert lib.svo_march_plan_set_shape(None, 0) == 1

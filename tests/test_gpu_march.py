@@ -169,3 +169,37 @@ def _shuffle_blocks(words, level, seed):
         blk[internal] = 0x80000000 | (perm[ptrs[internal]] * 8).astype(np.uint32)
         out[8 * perm[b] : 8 * perm[b] + 8] = blk
     return out
+
+
+@pytest.mark.parametrize("beam", [False, True])
+def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
+    # persistent waves with per-lane refill (mode 1) and one ray per lane (mode 0): same records
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    octree, words, level, _ = scene_octree
+    W, H = 203, 117  # partial 8x8 tiles on both edges
+    outs = []
+    try:
+        for mode in (0, 1):
+            ctx.check(lib.svo_march_mode(ctx.h, mode))
+            per = []
+            for cam_idx in range(4):
+                tr = S.OctreeTracer.Create(octree, _camera_struct(cameras()[cam_idx]))
+                tr.Resize(W, H)
+                tr.m_beam_enable = beam
+                tr.CmdDrawPipeline()
+                per.append(tr.GetHits())
+            rng = np.random.default_rng(5)
+            o = rng.uniform(0.5, 2.5, (3001, 3))
+            d = rng.normal(size=(3001, 3))
+            d /= np.linalg.norm(d, axis=1, keepdims=True)
+            rays = np.concatenate([o, d], axis=1).astype(np.float32)
+            for variant in (0, 1, 2):
+                per.append(S.march_rays(ctx, octree, rays, variant))
+            outs.append(per)
+    finally:
+        ctx.check(lib.svo_march_mode(ctx.h, -1))
+    for k, (a, b) in enumerate(zip(outs[0], outs[1])):
+        ok, msg = hits_equal(a, b)
+        assert ok, "case %d: %s" % (k, msg)
