@@ -226,8 +226,9 @@ int svo_autotune(svo_ctx *ctx, int on);
 /* Schedule of svo_march, svo_march_rays, svo_march_rays_batch and the server's ray batches:
  * 1 = a fixed grid of waves whose lanes take a new ray as soon as theirs has ended (rays dealt to
  * the waves in 64-ray chunks; wins when ray lengths diverge); 0 = one ray per lane (wins
- * for coherent rays); -1 (default) = chosen per shape class by the autotuner when svo_autotune is
- * on, else 0.  Results are identical. */
+ * for coherent rays), frames in 16x16-pixel blocks; 2 = as 0 with one-wave 8x8-pixel blocks for
+ * frames; -1 (default) = chosen per shape class by the autotuner when svo_autotune is on, else 0.
+ * Results are identical. */
 int svo_march_mode(svo_ctx *ctx, int mode);
 int svo_autotune_report(svo_ctx *ctx, char *buf, uint64_t cap, uint64_t *len);
 

@@ -1110,7 +1110,7 @@ int svo_march_rays_batch(svo_ctx *ctx, const svo_octree *const *octs, uint32_t n
 }
 
 int svo_march_mode(svo_ctx *ctx, int mode) {
-	if (!ctx || mode < -1 || mode > 1)
+	if (!ctx || mode < -1 || mode > 2)
 		return SVO_ERR_INVALID_ARGUMENT;
 	ctx->march_mode = mode;
 	return SVO_OK;

@@ -351,38 +351,45 @@ __device__ __forceinline__ void frame_pixel(const uint32_t *__restrict__ oct, ui
 	march_leaf<0, kStride>(oct, W, o, d, stk, err, r0, r1);
 }
 
-__global__ void __launch_bounds__(kBlock)
+// One wave per block: a frame has only (W/8+1)(H/8+1) beam cells (33K at 1920x1080), so small
+// blocks spread them over every CU instead of filling half of them.
+constexpr uint32_t kBeamBlock = 64;
+
+__global__ void __launch_bounds__(kBeamBlock)
     march_beam_kernel(const uint32_t *__restrict__ oct, uint64_t W, CamArgs cam, uint32_t width, uint32_t height,
                       uint32_t bw, uint32_t bh, float orig_sz, float *__restrict__ beam, uint32_t *__restrict__ nf_out,
                       uint32_t *err_out) {
-	__shared__ uint32_t s_stack[kStackSlots * kBlock];
-	const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+	__shared__ uint32_t s_stack[kStackSlots * kBeamBlock];
+	const uint32_t k = blockIdx.x * kBeamBlock + threadIdx.x;
 	if (k >= bw * bh)
 		return;
 	uint32_t err = 0u, nfetch = 0u;
-	beam[k] = beam_cell<kBlock>(oct, W, cam, width, height, k % bw, k / bw, orig_sz, s_stack + threadIdx.x, err, nfetch);
+	beam[k] = beam_cell<kBeamBlock>(oct, W, cam, width, height, k % bw, k / bw, orig_sz, s_stack + threadIdx.x, err,
+	                                nfetch);
 	if (nf_out)
 		nf_out[k] = nfetch;
 	if (err)
 		atomicOr(err_out, err);
 }
 
-__global__ void __launch_bounds__(kBlock)
+// (8 WX) x (8 WY)-pixel tile per block, one 8x8 quad per wave, XCD-banded tile order.
+template <uint32_t WX, uint32_t WY>
+__global__ void __launch_bounds__(64 * WX * WY)
     march_frame_kernel(const uint32_t *__restrict__ oct, uint64_t W, CamArgs cam, uint32_t width, uint32_t height,
                        const float *__restrict__ beam, uint32_t bw, uint32_t tiles_x, uint4 *__restrict__ hits,
                        uint32_t *err_out) {
-	__shared__ uint32_t s_stack[kStackSlots * kBlock];
-	// 16x16-pixel tile per block, one 8x8 quad per wave
+	constexpr uint32_t kT = 64 * WX * WY;
+	__shared__ uint32_t s_stack[kStackSlots * kT];
 	const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x);
 	const uint32_t tx = tile % tiles_x, ty = tile / tiles_x;
 	const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-	const uint32_t px = tx * 16u + (wave & 1u) * 8u + (lane & 7u);
-	const uint32_t py = ty * 16u + (wave >> 1) * 8u + (lane >> 3);
+	const uint32_t px = tx * (8u * WX) + (wave % WX) * 8u + (lane & 7u);
+	const uint32_t py = ty * (8u * WY) + (wave / WX) * 8u + (lane >> 3);
 	if (px >= width || py >= height)
 		return;
 	uint32_t err = 0u;
 	uint4 r0, r1;
-	frame_pixel<kBlock>(oct, W, cam, width, height, beam, bw, px, py, s_stack + threadIdx.x, err, r0, r1);
+	frame_pixel<kT>(oct, W, cam, width, height, beam, bw, px, py, s_stack + threadIdx.x, err, r0, r1);
 	const uint64_t k = (uint64_t)py * width + px;
 	hits[2 * k] = r0;
 	hits[2 * k + 1] = r1;
@@ -745,7 +752,7 @@ int tune(svo_ctx *ctx, const std::string &key, int nshapes, const char *const *n
 	return SVO_OK;
 }
 
-const char *const kScheduleName[2] = {"one-ray-per-lane", "persistent-refill"};
+const char *const kScheduleName[3] = {"one-ray-per-lane", "persistent-refill", "one-ray-per-lane-8x8"};
 
 // Persistent march of n rays from src; name is the profiler's name of the march.
 template <int kVariant, class Src>
@@ -766,7 +773,7 @@ int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
 	// exp2f(1.0f - level) (src/OctreeTracer.cpp:206): an exact power of two
 	const float orig_sz = ldexpf(1.0f, 1 - (int)oct->level);
 	ctx->t_beam.start(ctx->stream);
-	hipError_t e = launch(ctx, "march_beam", march_beam_kernel, dim3(div_up((uint64_t)bw * bh, kBlock)), dim3(kBlock),
+	hipError_t e = launch(ctx, "march_beam", march_beam_kernel, dim3(div_up((uint64_t)bw * bh, kBeamBlock)), dim3(kBeamBlock),
 	                      0, (const uint32_t *)oct->words, oct->W, ca, w, h, bw, bh, orig_sz, beam_img, nfetch,
 	                      ctx->d_err);
 	ctx->t_beam.stop(ctx->stream);
@@ -791,6 +798,8 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 		if (rc)
 			return rc;
 	}
+	// schedules: 0 = one ray per lane in 16x16 tiles, 1 = persistent refill, 2 = one ray per lane in
+	// one-wave 8x8 tiles
 	auto run = [&](int mode) -> hipError_t {
 		if (mode == 1) {
 			const uint32_t tx8 = (w + 7u) / 8u, ty8 = (h + 7u) / 8u;
@@ -798,8 +807,13 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 			              reinterpret_cast<uint4 *>(hits)};
 			return launch_persist<0>(ctx, "march_frame", src, (uint64_t)tx8 * ty8 * 64u);
 		}
-		const uint32_t tiles_x = (w + 15u) / 16u, tiles_y = (h + 15u) / 16u;
-		return launch(ctx, "march_frame", march_frame_kernel, dim3(tiles_x * tiles_y), dim3(kBlock), 0,
+		const uint32_t WX = mode == 0 ? 2u : 1u, WY = mode == 0 ? 2u : 1u; // 16x16 or 8x8 tiles
+		const uint32_t tiles_x = (w + 8u * WX - 1u) / (8u * WX), tiles_y = (h + 8u * WY - 1u) / (8u * WY);
+		if (mode == 0)
+			return launch(ctx, "march_frame", march_frame_kernel<2, 2>, dim3(tiles_x * tiles_y), dim3(256), 0,
+			              (const uint32_t *)oct->words, oct->W, ca, w, h, (const float *)beam_img, bw, tiles_x,
+			              reinterpret_cast<uint4 *>(hits), ctx->d_err);
+		return launch(ctx, "march_frame", march_frame_kernel<1, 1>, dim3(tiles_x * tiles_y), dim3(64), 0,
 		              (const uint32_t *)oct->words, oct->W, ca, w, h, (const float *)beam_img, bw, tiles_x,
 		              reinterpret_cast<uint4 *>(hits), ctx->d_err);
 	};
@@ -809,7 +823,7 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 		if (ctx->autotune) {
 			char key[96];
 			std::snprintf(key, sizeof key, "march_frame %ux%u level=%u beam=%d", w, h, oct->level, beam ? 1 : 0);
-			const int rc = tune(ctx, key, 2, kScheduleName, run, mode);
+			const int rc = tune(ctx, key, 3, kScheduleName, run, mode);
 			if (rc)
 				return rc;
 		}

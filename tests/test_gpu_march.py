@@ -173,7 +173,8 @@ def _shuffle_blocks(words, level, seed):
 
 @pytest.mark.parametrize("beam", [False, True])
 def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
-    # persistent waves with per-lane refill (mode 1) and one ray per lane (mode 0): same records
+    # one ray per lane (mode 0, 16x16 frame tiles; mode 2, 8x8 tiles) and persistent waves with
+    # per-lane refill (mode 1): the same records
     import sparsevoxeloctree_amd as S
     from sparsevoxeloctree_amd._capi import lib
 
@@ -181,7 +182,7 @@ def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
     W, H = 203, 117  # partial 8x8 tiles on both edges
     outs = []
     try:
-        for mode in (0, 1):
+        for mode in (0, 1, 2):
             ctx.check(lib.svo_march_mode(ctx.h, mode))
             per = []
             for cam_idx in range(4):
@@ -200,6 +201,7 @@ def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
             outs.append(per)
     finally:
         ctx.check(lib.svo_march_mode(ctx.h, -1))
-    for k, (a, b) in enumerate(zip(outs[0], outs[1])):
-        ok, msg = hits_equal(a, b)
-        assert ok, "case %d: %s" % (k, msg)
+    for other in (1, 2):
+        for k, (a, b) in enumerate(zip(outs[0], outs[other])):
+            ok, msg = hits_equal(a, b)
+            assert ok, "mode %d case %d: %s" % (other, k, msg)
