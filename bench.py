@@ -138,18 +138,25 @@ def cpu_baseline(S, cfg, cam, words, budget_s=20.0):
     oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + 8, beam, threads=cores)
     per_row = (time.perf_counter() - t0) / 8
     rows = int(max(8, min(h, budget_s / max(per_row, 1e-6))))
+    reps = 1
+    if rows == h:
+        # the whole frame fits the budget: march it (beam pre-pass included) as many times as fit
+        reps = int(max(1, min(1000, budget_s / max(per_row * h + t_beam, 1e-6))))
     y0 = max(0, h // 2 - rows // 2)
     t0 = time.perf_counter()
-    oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + rows, beam, threads=cores)
+    for _ in range(reps):
+        if reps > 1:
+            beam, _ = oracle.beam(words, level, cam_arr, w, h, threads=cores)
+        oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + rows, beam, threads=cores)
     t_rows = time.perf_counter() - t0
-    t = t_rows + t_beam * rows / h  # the beam pass's share for those rows
+    t = t_rows if reps > 1 else t_rows + t_beam * rows / h  # the beam pass's share for those rows
     return {
-        "value": round(rows * w / t / 1e6, 4),
+        "value": round(reps * rows * w / t / 1e6, 4),
         "unit": "Mrays/s",
         "cores": cores,
         "kind": "port",
-        "sample": "CPU oracle, beam pre-pass + Octree_RayMarchLeaf for rows %d..%d of the %dx%d frame "
-        "(%d rays, %.1f s)" % (y0, y0 + rows, w, h, rows * w, t),
+        "sample": "CPU oracle, beam pre-pass + Octree_RayMarchLeaf for rows %d..%d of the %dx%d frame, "
+        "%d pass(es) (%d rays, %.1f s)" % (y0, y0 + rows, w, h, reps, reps * rows * w, t),
     }
 
 
