@@ -137,7 +137,7 @@ struct BuildBatch {
 	std::vector<uint64_t> h_index;
 	BuildSeg *segs = nullptr;
 	uint64_t *index = nullptr;
-	uint32_t *bm = nullptr, *rk = nullptr; // bitmaps (TW words), ranks (TW + 1)
+	uint32_t *bm = nullptr, *rk = nullptr; // bitmaps (TW words), ranks of 8-word chunks (TW / 8 + 1)
 	uint32_t *cnt = nullptr, *aux = nullptr; // per leaf block (nbins)
 	uint32_t *node = nullptr;   // 2 * Ftot: {leaf block, place in its bucket}
 	uint32_t *bucket = nullptr; // 2 * Ftot

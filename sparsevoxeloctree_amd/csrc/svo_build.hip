@@ -11,9 +11,10 @@
 //   octree_tag_node.comp      -> bm_mark: each fragment sets the bit of its depth-(L-1) node in
 //   (levels 1..L-1)              bitmap B[L-1]; bm_reduce ORs each byte of B[d+1] (8 children) into
 //                                one bit of B[d], d = L-2..1
-//   octree_alloc_node.comp    -> one exclusive popcount scan over all bitmaps (rank of every
-//                                occupied node among its depth = its block among the allocated
-//                                children blocks); child pointer = 8 * (first block of depth + rank)
+//   octree_alloc_node.comp    -> one exclusive popcount scan over all bitmaps in 8-word chunks
+//                                (rank of every occupied node among its depth = its block among
+//                                the allocated children blocks); child pointer = 8 * (first block
+//                                of depth + rank)
 //   octree_modify_arg.comp    -> bm_levels: per-depth counts -> first block of every depth, the
 //                                leaf blocks and W = (counter + 1) * 8
 //   octree_init_node.comp     -> bm_write_blocks writes every block whole (pointers and zeros)
@@ -25,9 +26,11 @@
 // The node words are identical to the reference's pass structure run with a serial allocator.
 //
 // Batch layout: the scenes' fragments, bitmap words and leaf blocks each form one index space
-// (begin tables fbeg, dw/dr per depth, bbeg); the rank array R is one prefix over all bitmaps, and
-// every rank is a difference of two R entries of the same scene, so modular uint32 arithmetic
-// stays exact.
+// (begin tables fbeg, dw/dr per depth, bbeg).  Every depth's bitmap starts on an 8-word boundary;
+// the rank array R holds one entry per 8-word chunk (set bits of all bitmaps before the chunk), so
+// it is 1/8 of the bitmaps' size, and a node's rank is its chunk's entry plus the popcounts of the
+// chunk's words before it.  Every rank is a difference of two R-based counts of the same scene,
+// so modular uint32 arithmetic stays exact.
 #include <algorithm>
 #include <string>
 
@@ -78,17 +81,66 @@ __device__ __forceinline__ void cmp_swap(uint64_t &x, uint64_t &y) {
 
 __device__ __forceinline__ uint64_t frag_key(uint2 f) { return ((uint64_t)f.x << 32) | f.y; }
 
-// Rank of node P among the occupied nodes of depth d.
-__device__ __forceinline__ uint32_t node_rank(const uint32_t *__restrict__ B, const uint32_t *__restrict__ R,
-                                              uint64_t off, uint64_t P) {
-	const uint64_t w = off + (P >> 5);
-	return (R[w] - R[off]) + (uint32_t)__popc(B[w] & ((1u << (uint32_t)(P & 31u)) - 1u));
+// Set bits of the bitmaps before word w (a global word index) and the word itself: the rank of
+// w's 8-word chunk plus the popcounts of the chunk's words before w (one 32-byte load).
+__device__ __forceinline__ uint32_t bits_before(const uint32_t *__restrict__ B, const uint32_t *__restrict__ R,
+                                                uint64_t w, uint32_t &word) {
+	const uint4 *c = reinterpret_cast<const uint4 *>(B + (w & ~7ull));
+	const uint4 a = c[0], b = c[1];
+	const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+	const uint32_t k = (uint32_t)(w & 7u);
+	uint32_t r = R[w >> 3], cur = v[0];
+#pragma unroll
+	for (uint32_t j = 0; j < 8u; ++j) {
+		r += j < k ? (uint32_t)__popc(v[j]) : 0u;
+		cur = j == k ? v[j] : cur;
+	}
+	word = cur;
+	return r;
 }
 
 __global__ void __launch_bounds__(kThreads) bm_zero(uint32_t *B, uint64_t n) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += stride)
 		B[i] = 0u;
+}
+
+// Fragments per thread and loop trip in the per-fragment passes: their loads are issued together,
+// so a wave waits on one chain of dependent memory round trips for kU fragments.
+constexpr uint32_t kU = 4;
+
+// Where a fragment's scene keeps what the per-fragment passes read: the fragment itself, the first
+// word of the scene's B[L-1] (and of its ranks), the level and the scene's first bin.
+struct FragSrc {
+	const uint2 *frag;
+	uint64_t w0;
+	uint32_t L, bin_begin;
+};
+
+// Descriptor lookups for the kU fragments base + u * kThreads + threadIdx.x of one loop trip
+// (no fragment loads); frag = nullptr past the end.
+__device__ __forceinline__ void frag_sources(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg,
+                                             uint32_t S, uint64_t Ftot, uint64_t base, uint32_t &hint,
+                                             FragSrc (&src)[kU]) {
+#pragma unroll
+	for (uint32_t u = 0; u < kU; ++u) {
+		const uint64_t b = base + (uint64_t)u * kThreads, f = b + threadIdx.x;
+		src[u] = FragSrc{nullptr, 0u, 1u, 0u};
+		if (f < Ftot)
+			with_seg<kThreads>(fbeg, S, hint, b, f, [&](uint32_t s) {
+				const BuildSeg &g = segs[s];
+				const uint32_t L = g.L;
+				src[u] = FragSrc{g.frags + (f - fbeg[s]), g.bm + (L > 1u ? g.off[L - 1u] : 0u), L,
+				                 (uint32_t)g.bin_begin};
+			});
+	}
+}
+
+// Depth-(L-1) node of a fragment (its leaf block's parent), as a Morton index.
+__device__ __forceinline__ uint64_t frag_parent(uint2 fr, uint32_t L) {
+	uint32_t x, y, z;
+	decode_frag(fr, 1u << L, x, y, z);
+	return morton(x, y, z) >> 3;
 }
 
 // Each fragment marks its depth-(L-1) node.  Neighbouring fragments mostly fall into the same
@@ -99,36 +151,44 @@ __global__ void __launch_bounds__(kThreads)
     bm_mark(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
             uint32_t *B) {
 	const uint32_t lane = threadIdx.x & 63u;
-	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
 	uint32_t hint = 0;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
-		const uint64_t f = base + threadIdx.x;
-		uint64_t word = ~0ull; // global bitmap word
-		uint32_t bits = 0u;
-		if (f < Ftot)
-			with_seg<kThreads>(fbeg, S, hint, base, f, [&](uint32_t s) {
-				const BuildSeg &g = segs[s];
-				const uint32_t L = g.L;
-				if (L > 1u) {
-					uint32_t x, y, z;
-					decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
-					const uint64_t P = morton(x, y, z) >> 3;
-					word = g.bm + g.off[L - 1u] + (P >> 5);
-					bits = 1u << (uint32_t)(P & 31u);
-				}
-			});
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
+		FragSrc src[kU];
+		frag_sources(segs, fbeg, S, Ftot, base, hint, src);
+		uint2 fr[kU];
 #pragma unroll
-		for (uint32_t d = 1; d < 64u; d <<= 1) {
-			const uint64_t ow = __shfl_up(word, d, 64);
-			const uint32_t ob = __shfl_up(bits, d, 64);
-			bits |= (lane >= d && ow == word) ? ob : 0u; // any lane with the same word may contribute
+		for (uint32_t u = 0; u < kU; ++u)
+			fr[u] = src[u].frag ? *src[u].frag : make_uint2(0u, 0u);
+		uint64_t word[kU]; // global bitmap word
+		uint32_t bits[kU];
+		bool last[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u) {
+			word[u] = ~0ull;
+			bits[u] = 0u;
+			if (src[u].frag && src[u].L > 1u) {
+				const uint64_t P = frag_parent(fr[u], src[u].L);
+				word[u] = src[u].w0 + (P >> 5);
+				bits[u] = 1u << (uint32_t)(P & 31u);
+			}
+#pragma unroll
+			for (uint32_t d = 1; d < 64u; d <<= 1) {
+				const uint64_t ow = __shfl_up(word[u], d, 64);
+				const uint32_t ob = __shfl_up(bits[u], d, 64);
+				bits[u] |= (lane >= d && ow == word[u]) ? ob : 0u; // any lane with the same word may contribute
+			}
+			const uint64_t next = __shfl_down(word[u], 1u, 64);
+			last[u] = word[u] != ~0ull && (lane == 63u || next != word[u]);
 		}
-		const uint64_t next = __shfl_down(word, 1u, 64);
-		if (word != ~0ull && (lane == 63u || next != word)) {
-			uint32_t *w = B + word;
-			if ((*w & bits) != bits) // a stale cached word only costs a redundant atomic
-				atomicOr(w, bits);
-		}
+		uint32_t have[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u)
+			have[u] = last[u] ? B[word[u]] : ~0u;
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u)
+			if ((have[u] & bits[u]) != bits[u]) // a stale cached word only costs a redundant atomic
+				atomicOr(B + word[u], bits[u]);
 	}
 }
 
@@ -162,11 +222,16 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
+// Set bits of 8-word chunk i.
 struct PopSrc {
-	const uint32_t *B;
-	uint64_t n;
+	const uint4 *B;
+	uint64_t n; // chunks
 	__device__ uint64_t size() const { return n; }
-	__device__ uint32_t operator()(uint64_t i) const { return (uint32_t)__popc(B[i]); }
+	__device__ uint32_t operator()(uint64_t i) const {
+		const uint4 a = B[2u * i], b = B[2u * i + 1u];
+		return (uint32_t)(__popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w) + __popc(b.x) + __popc(b.y) +
+		                  __popc(b.z) + __popc(b.w));
+	}
 };
 struct RankTotal {
 	uint32_t *R;
@@ -187,13 +252,13 @@ __global__ void bm_levels(BuildSeg *segs, uint32_t S, const uint32_t *__restrict
 		return;
 	BuildSeg &g = segs[s];
 	const uint32_t L = g.L;
-	const uint32_t *Rs = R + g.bm;
+	const uint32_t *Rs = R + (g.bm >> 3); // every depth starts a chunk
 	uint64_t first = 1;
 	g.fb[0] = 0u;
 	g.fb[1] = 1u;
 	uint64_t nlast = 1; // nodes at depth L-1 (the root when L == 1)
 	for (uint32_t d = 1; d < L; ++d) {
-		const uint64_t nd = (uint32_t)(Rs[g.off[d + 1u]] - Rs[g.off[d]]);
+		const uint64_t nd = (uint32_t)(Rs[g.off[d + 1u] >> 3] - Rs[g.off[d] >> 3]);
 		first += nd;
 		g.fb[d + 1u] = (uint32_t)(first < 0xFFFFFFFFull ? first : 0xFFFFFFFFull);
 		nlast = nd;
@@ -242,23 +307,31 @@ __global__ void __launch_bounds__(kThreads)
 		with_seg<kThreads>(dp, S, hint, base, i, [&](uint32_t s) {
 			const BuildSeg &g = segs[s];
 			const uint64_t w = i - dp[s];
-			const uint32_t *Bs = B + g.bm, *Rs = R + g.bm;
-			const uint64_t offp = g.off[d - 1u], offc = g.off[d];
-			uint32_t v = Bs[offp + w];
+			const uint64_t offp = g.bm + g.off[d - 1u], offc = g.bm + g.off[d]; // both start chunks
+			uint32_t v;
+			const uint32_t rp = bits_before(B, R, offp + w, v);
 			if (v == 0u)
 				return;
 			const uint32_t fbd = g.fb[d];
-			uint32_t block = g.fb[d - 1u] + (Rs[offp + w] - Rs[offp]); // first parent's block
-			const uint32_t rc0 = Rs[offc];
+			uint32_t block = g.fb[d - 1u] + (rp - R[offp >> 3]); // first parent's block
+			// the 32 parents' children bytes are the 8-word chunk offc + 8w; parents not in v have
+			// zero bytes, so a running count gives every child its rank
+			const uint4 *cc = reinterpret_cast<const uint4 *>(B + offc + 8u * w);
+			const uint4 c0 = cc[0], c1 = cc[1];
+			uint32_t r = R[(offc >> 3) + w] - R[offc >> 3];
 			uint32_t *oct = g.words;
 			while (v) {
-				const uint32_t b = (uint32_t)__ffs(v) - 1u;
+				const uint32_t b = (uint32_t)__ffs(v) - 1u, j = b >> 2u;
 				v &= v - 1u;
-				const uint64_t q = 32u * w + b; // parent node index at depth d-1
-				const uint64_t cw = offc + (q >> 2);
-				const uint32_t cv = Bs[cw], sh = 8u * (uint32_t)(q & 3u);
-				const uint32_t byte = (cv >> sh) & 0xffu;
-				uint32_t r = (Rs[cw] - rc0) + (uint32_t)__popc(cv & ((1u << sh) - 1u));
+				uint32_t cv = c0.x;
+				cv = j == 1u ? c0.y : cv;
+				cv = j == 2u ? c0.z : cv;
+				cv = j == 3u ? c0.w : cv;
+				cv = j == 4u ? c1.x : cv;
+				cv = j == 5u ? c1.y : cv;
+				cv = j == 6u ? c1.z : cv;
+				cv = j == 7u ? c1.w : cv;
+				const uint32_t byte = (cv >> (8u * (b & 3u))) & 0xffu;
 				uint32_t wd[8];
 #pragma unroll
 				for (uint32_t t = 0; t < 8u; ++t) {
@@ -284,35 +357,54 @@ __global__ void __launch_bounds__(kThreads)
                    const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint2 *__restrict__ node,
                    uint32_t *cnt) {
 	const uint32_t lane = threadIdx.x & 63u;
-	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
 	uint32_t hint = 0;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
-		const uint64_t f = base + threadIdx.x;
-		uint32_t bin = 0xFFFFFFFFu;
-		if (f < Ftot)
-			with_seg<kThreads>(fbeg, S, hint, base, f, [&](uint32_t s) {
-				const BuildSeg &g = segs[s];
-				const uint32_t L = g.L;
-				uint32_t x, y, z;
-				decode_frag(g.frags[f - fbeg[s]], 1u << L, x, y, z);
-				const uint32_t local =
-				    L == 1u ? 0u : node_rank(B + g.bm, R + g.bm, g.off[L - 1u], morton(x, y, z) >> 3);
-				bin = (uint32_t)g.bin_begin + local;
-			});
-		const uint32_t prev = __shfl_up(bin, 1u, 64);
-		const bool head = (lane == 0u) || prev != bin;
-		const uint64_t heads = __ballot(head);
-		const uint64_t after = (lane == 63u) ? 0ull : (heads >> (lane + 1u));
-		const uint32_t run = after ? (uint32_t)__ffsll((unsigned long long)after) : 64u - lane;
-		uint32_t pos = 0u;
-		if (head && bin != 0xFFFFFFFFu)
-			pos = atomicAdd(&cnt[bin], run);
-		// lane of this lane's run head = highest head at or below this lane
-		const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
-		const uint32_t head_lane = 63u - (uint32_t)__clzll((long long)upto);
-		pos = __shfl(pos, (int)head_lane, 64) + (lane - head_lane);
-		if (f < Ftot)
-			node[f] = make_uint2(bin, pos);
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
+		FragSrc src[kU];
+		frag_sources(segs, fbeg, S, Ftot, base, hint, src);
+		uint2 fr[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u)
+			fr[u] = src[u].frag ? *src[u].frag : make_uint2(0u, 0u);
+		// rank of the fragment's depth-(L-1) node among its scene's occupied nodes of that depth
+		// (w0, the depth's first word, starts a chunk)
+		uint32_t below[kU], r[kU], r0[kU], bin[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u) {
+			r[u] = r0[u] = below[u] = 0u;
+			if (src[u].frag && src[u].L > 1u) {
+				const uint64_t P = frag_parent(fr[u], src[u].L);
+				uint32_t word;
+				r[u] = bits_before(B, R, src[u].w0 + (P >> 5), word);
+				r0[u] = R[src[u].w0 >> 3];
+				below[u] = word & ((1u << (uint32_t)(P & 31u)) - 1u);
+			}
+		}
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u)
+			bin[u] = src[u].frag ? src[u].bin_begin + (r[u] - r0[u]) + (uint32_t)__popc(below[u]) : 0xFFFFFFFFu;
+		uint32_t pos[kU], head_lane[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u) {
+			const uint32_t prev = __shfl_up(bin[u], 1u, 64);
+			const bool head = (lane == 0u) || prev != bin[u];
+			const uint64_t heads = __ballot(head);
+			const uint64_t after = (lane == 63u) ? 0ull : (heads >> (lane + 1u));
+			const uint32_t run = after ? (uint32_t)__ffsll((unsigned long long)after) : 64u - lane;
+			pos[u] = 0u;
+			if (head && bin[u] != 0xFFFFFFFFu)
+				pos[u] = atomicAdd(&cnt[bin[u]], run);
+			// lane of this lane's run head = highest head at or below this lane
+			const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
+			head_lane[u] = 63u - (uint32_t)__clzll((long long)upto);
+		}
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u) {
+			const uint32_t p = __shfl(pos[u], (int)head_lane[u], 64) + (lane - head_lane[u]);
+			const uint64_t f = base + (uint64_t)u * kThreads + threadIdx.x;
+			if (f < Ftot)
+				node[f] = make_uint2(bin[u], p);
+		}
 	}
 }
 
@@ -320,15 +412,29 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     build_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
                   const uint2 *__restrict__ node, const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
-	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
 	uint32_t hint = 0;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < Ftot; base += stride) {
-		const uint64_t f = base + threadIdx.x;
-		if (f >= Ftot)
-			continue;
-		const uint2 np = node[f];
-		with_seg<kThreads>(fbeg, S, hint, base, f,
-		                   [&](uint32_t s) { bucket[aux[np.x] + np.y] = segs[s].frags[f - fbeg[s]]; });
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
+		FragSrc src[kU];
+		frag_sources(segs, fbeg, S, Ftot, base, hint, src);
+		uint2 fr[kU], np[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u) {
+			const uint64_t f = base + (uint64_t)u * kThreads + threadIdx.x;
+			fr[u] = np[u] = make_uint2(0u, 0u);
+			if (src[u].frag) {
+				fr[u] = *src[u].frag;
+				np[u] = node[f];
+			}
+		}
+		uint32_t first[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u)
+			first[u] = src[u].frag ? aux[np[u].x] : 0u;
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u)
+			if (src[u].frag)
+				bucket[first[u] + np[u].y] = fr[u];
 	}
 }
 
@@ -740,14 +846,15 @@ uint32_t grid_for(uint64_t items, svo_ctx *ctx) {
 	return (uint32_t)(g ? g : 1);
 }
 
-// Word offsets of B[1..L-1] inside one scene's bitmaps; off[L] = the scene's bitmap words.
+// Word offsets of B[1..L-1] inside one scene's bitmaps; off[L] = the scene's bitmap words.  Every
+// depth starts an 8-word rank chunk (the padding words stay zero).
 void levels_for(uint32_t L, uint64_t *off) {
 	uint64_t o = 0;
 	for (uint32_t d = 1; d <= L; ++d) {
 		off[d] = o;
 		if (d < L) {
 			const uint64_t bits = 1ull << (3u * d);
-			o += bits < 32u ? 1u : bits / 32u;
+			o += ((bits < 32u ? 1u : bits / 32u) + 7u) & ~7ull;
 		}
 	}
 }
@@ -762,7 +869,7 @@ int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
 		    hipSuccess)
 			return hip_check(ctx, e, "bm_zero");
 		if (bb.Ftot > 0 &&
-		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_for(bb.Ftot, ctx)), dim3(kThreads), 0,
+		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_for(div_up(bb.Ftot, kU), ctx)), dim3(kThreads), 0,
 		                (const BuildSeg *)bb.segs, bb.fbeg(), S, bb.Ftot, bb.bm)) != hipSuccess)
 			return hip_check(ctx, e, "bm_mark");
 		for (uint32_t d = bb.maxL >= 3 ? bb.maxL - 2 : 0; d >= 1; --d) {
@@ -772,12 +879,12 @@ int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
 			                         d, bb.bm)) != hipSuccess)
 				return hip_check(ctx, e, "bm_reduce");
 		}
-		PopSrc src{bb.bm, bb.TW};
+		PopSrc src{reinterpret_cast<const uint4 *>(bb.bm), bb.TW / 8u};
 		if ((e = launch(ctx, "rank_reduce", scan_reduce<PopSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_reduce");
 		if ((e = launch(ctx, "rank_parts", scan_parts<RankTotal>, dim3(1), dim3(1024), 0, ctx->d_scan_parts,
-		                RankTotal{bb.rk, bb.TW})) != hipSuccess)
+		                RankTotal{bb.rk, bb.TW / 8u})) != hipSuccess)
 			return hip_check(ctx, e, "rank_parts");
 		if ((e = launch(ctx, "rank_down", scan_down<PopSrc, RankDst>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                RankDst{bb.rk}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
@@ -807,7 +914,7 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	if ((e = launch(ctx, "bin_zero", bm_zero, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, bb.cnt, bb.nbins)) !=
 	    hipSuccess)
 		return hip_check(ctx, e, "bin_zero");
-	const uint32_t gF = grid_for(bb.Ftot, ctx);
+	const uint32_t gF = grid_for(div_up(bb.Ftot, kU), ctx); // kU fragments per thread and trip
 	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
 	if (bb.Ftot > 0 && (e = launch(ctx, "build_tag_leaf", build_tag_leaf, dim3(gF), dim3(kThreads), 0, segs,
 	                               bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk, node,
@@ -914,7 +1021,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	if (pool->alloc((void **)&bb.segs, sizeof(BuildSeg) * S) != hipSuccess ||
 	    pool->alloc((void **)&bb.index, sizeof(uint64_t) * bb.h_index.size()) != hipSuccess ||
 	    pool->alloc((void **)&bb.bm, sizeof(uint32_t) * (bb.TW ? bb.TW : 1)) != hipSuccess ||
-	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW + 1)) != hipSuccess ||
+	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW / 8u + 1)) != hipSuccess ||
 	    pool->alloc((void **)&bb.node, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
 	    pool->alloc((void **)&bb.bucket, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
 	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (2 + 2 * cap)) != hipSuccess)

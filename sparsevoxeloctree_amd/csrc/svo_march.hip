@@ -107,14 +107,20 @@ template <bool kGeEps> __device__ __forceinline__ void march_init(f3 o, f3 &d, M
 
 // One trip of the traversal loop (octree.glsl:86-172 / 221-304).  Returns false on break.
 // stk: this lane's LDS column (slot s at stk[s * kStride], kStride = threads per block).
+//
+// Node words are addressed with 32-bit offsets: octrees hold at most 2^30 words (30-bit child
+// pointers; svo_octree_create and the builder reject larger ones), so an index below W is below
+// 2^30 and its byte offset fits in 32 bits.
 template <bool kCoarse, uint32_t kStride = kBlock>
 __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uint64_t W, March &m, uint32_t *stk,
                                            uint32_t &nfetch, uint32_t &err, float orig_sz, float dir_sz) {
+	const uint32_t W32 = W < (1ull << 30) ? (uint32_t)W : (1u << 30);
+	uint32_t *stk_s = stk - kMinScale * kStride; // slot of scale s at stk_s[s * kStride]
 	if (m.cur == 0u) {
-		uint64_t a = (uint64_t)m.parent + (m.idx ^ m.oct_mask);
+		const uint32_t a = m.parent + (m.idx ^ m.oct_mask); // parent < 2^30: no wrap-around
 		++nfetch;
-		if (a < W)
-			m.cur = oct[a];
+		if (a < W32)
+			m.cur = *reinterpret_cast<const uint32_t *>(reinterpret_cast<const char *>(oct) + (a << 2u));
 		else
 			err |= kErrOutOfRange;
 	}
@@ -126,62 +132,53 @@ __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uin
 		if (kCoarse && orig_sz + tc_max * dir_sz >= m.scale_exp2)
 			return false;
 		// INTERSECT
-		float half_scale_exp2 = m.scale_exp2 * 0.5f;
-		f3 t_center{half_scale_exp2 * m.t_coef.x + t_corner.x, half_scale_exp2 * m.t_coef.y + t_corner.y,
-		            half_scale_exp2 * m.t_coef.z + t_corner.z};
-		if ((m.cur & 0x40000000u) != 0u)
-			return false; // leaf
-		// PUSH
-		if (m.scale < kMinScale) {
-			err |= kErrTooDeep;
-			m.scale = kStackSize; // ends the ray as a miss
+		const bool leaf = (m.cur & 0x40000000u) != 0u;
+		if (leaf || m.scale < kMinScale) {
+			if (!leaf) { // a push below the stack's last slot
+				err |= kErrTooDeep;
+				m.scale = kStackSize; // ends the ray as a miss
+			}
 			return false;
 		}
+		// PUSH
+		const float half_scale_exp2 = m.scale_exp2 * 0.5f;
+		f3 t_center{half_scale_exp2 * m.t_coef.x + t_corner.x, half_scale_exp2 * m.t_coef.y + t_corner.y,
+		            half_scale_exp2 * m.t_coef.z + t_corner.z};
 		if (tc_max < m.h)
-			stk[(m.scale - kMinScale) * kStride] = m.parent;
+			stk_s[m.scale * kStride] = m.parent;
 		m.h = tc_max;
 		m.parent = m.cur & 0x3fffffffu;
-		m.idx = 0u;
 		--m.scale;
 		m.scale_exp2 = half_scale_exp2;
-		if (t_center.x > m.t_min)
-			m.idx ^= 1u, m.pos.x += m.scale_exp2;
-		if (t_center.y > m.t_min)
-			m.idx ^= 2u, m.pos.y += m.scale_exp2;
-		if (t_center.z > m.t_min)
-			m.idx ^= 4u, m.pos.z += m.scale_exp2;
+		const bool cx = t_center.x > m.t_min, cy = t_center.y > m.t_min, cz = t_center.z > m.t_min;
+		m.idx = (cx ? 1u : 0u) | (cy ? 2u : 0u) | (cz ? 4u : 0u);
+		m.pos.x = cx ? m.pos.x + half_scale_exp2 : m.pos.x;
+		m.pos.y = cy ? m.pos.y + half_scale_exp2 : m.pos.y;
+		m.pos.z = cz ? m.pos.z + half_scale_exp2 : m.pos.z;
 		m.cur = 0u;
 		return true;
 	}
-	// ADVANCE
-	uint32_t step_mask = 0u;
-	if (t_corner.x <= tc_max)
-		step_mask ^= 1u, m.pos.x -= m.scale_exp2;
-	if (t_corner.y <= tc_max)
-		step_mask ^= 2u, m.pos.y -= m.scale_exp2;
-	if (t_corner.z <= tc_max)
-		step_mask ^= 4u, m.pos.z -= m.scale_exp2;
+	// ADVANCE.  pos is a multiple of scale_exp2 in [1, 2), so pos - scale_exp2 is exact and the
+	// POP's pos + scale_exp2 of a stepped axis (octree.glsl:275-280) is pos before the step; an axis
+	// that does not step contributes no differing bits.
+	const bool sx = t_corner.x <= tc_max, sy = t_corner.y <= tc_max, sz = t_corner.z <= tc_max;
+	const uint32_t step_mask = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
+	const float nx = sx ? m.pos.x - m.scale_exp2 : m.pos.x, ny = sy ? m.pos.y - m.scale_exp2 : m.pos.y,
+	            nz = sz ? m.pos.z - m.scale_exp2 : m.pos.z;
+	const uint32_t differing_bits =
+	    (fbits(nx) ^ fbits(m.pos.x)) | (fbits(ny) ^ fbits(m.pos.y)) | (fbits(nz) ^ fbits(m.pos.z));
+	m.pos = f3{nx, ny, nz};
 	m.t_min = tc_max;
 	m.idx ^= step_mask;
 	if ((m.idx & step_mask) != 0u) {
-		// POP
-		uint32_t differing_bits = 0u;
-		if ((step_mask & 1u) != 0u)
-			differing_bits |= fbits(m.pos.x) ^ fbits(m.pos.x + m.scale_exp2);
-		if ((step_mask & 2u) != 0u)
-			differing_bits |= fbits(m.pos.y) ^ fbits(m.pos.y + m.scale_exp2);
-		if ((step_mask & 4u) != 0u)
-			differing_bits |= fbits(m.pos.z) ^ fbits(m.pos.z + m.scale_exp2);
-		m.scale = find_msb(differing_bits);
+		// POP.  differing_bits != 0 and its highest bit lies above the bit of the old scale (the
+		// step borrowed from it), so the new scale exceeds the old one (>= kMinScale - 1) and its
+		// stack slot exists.
+		m.scale = 31u - (uint32_t)__builtin_clz(differing_bits);
 		if (m.scale >= kStackSize)
 			return false;
-		if (m.scale < kMinScale) {
-			err |= kErrTooDeep;
-			m.scale = kStackSize;
-			return false;
-		}
 		m.scale_exp2 = bitsf((m.scale - kStackSize + 127u) << 23u);
-		m.parent = stk[(m.scale - kMinScale) * kStride];
+		m.parent = stk_s[m.scale * kStride];
 		uint32_t shx = fbits(m.pos.x) >> m.scale, shy = fbits(m.pos.y) >> m.scale, shz = fbits(m.pos.z) >> m.scale;
 		m.pos.x = bitsf(shx << m.scale);
 		m.pos.y = bitsf(shy << m.scale);
