@@ -143,14 +143,31 @@ __device__ __forceinline__ uint64_t frag_parent(uint2 fr, uint32_t L) {
 	return morton(x, y, z) >> 3;
 }
 
-// Each fragment marks its depth-(L-1) node.  Neighbouring fragments mostly fall into the same
-// bitmap word, so a wave ORs the bits of lanes with equal words together (segmented scan over
-// shuffles) and only the last lane of every run of equal words touches memory: one atomic when a
-// plain load does not already show all of the run's bits.
+// Open-addressing tables in LDS that merge the kU * kThreads fragments of one loop trip by key
+// before anything touches global memory: fragments come in raster order, so lanes next to each
+// other alternate between leaf blocks and a wave's runs of equal keys are short, while a trip's
+// 1024 fragments hold few distinct keys.
+constexpr uint32_t kHash = 2048; // slots, twice the fragments of a trip
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t hash_slot(uint64_t key) {
+	return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - 11)); // 11 bits = kHash
+}
+
+// Each fragment marks its depth-(L-1) node.  The trip's bits are ORed per bitmap word in LDS, then
+// each distinct word costs one load and, only when that load does not already show all of its
+// bits, one atomic.
 __global__ void __launch_bounds__(kThreads)
     bm_mark(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
             uint32_t *B) {
-	const uint32_t lane = threadIdx.x & 63u;
+	static_assert(kHash == 2048u && kHash >= 2u * kU * kThreads, "hash_slot returns 11 bits");
+	__shared__ unsigned long long s_word[kHash];
+	__shared__ uint32_t s_bits[kHash];
+	for (uint32_t h = threadIdx.x; h < kHash; h += kThreads) {
+		s_word[h] = ~0ull;
+		s_bits[h] = 0u;
+	}
+	__syncthreads();
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
@@ -160,35 +177,40 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u)
 			fr[u] = src[u].frag ? *src[u].frag : make_uint2(0u, 0u);
-		uint64_t word[kU]; // global bitmap word
-		uint32_t bits[kU];
-		bool last[kU];
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
-			word[u] = ~0ull;
-			bits[u] = 0u;
 			if (src[u].frag && src[u].L > 1u) {
 				const uint64_t P = frag_parent(fr[u], src[u].L);
-				word[u] = src[u].w0 + (P >> 5);
-				bits[u] = 1u << (uint32_t)(P & 31u);
+				const unsigned long long word = src[u].w0 + (P >> 5); // global bitmap word
+				uint32_t h = hash_slot(word);
+				for (;;) { // at most kU * kThreads keys in kHash slots: a free or equal slot exists
+					const unsigned long long old = atomicCAS(&s_word[h], ~0ull, word);
+					if (old == ~0ull || old == word)
+						break;
+					h = (h + 1u) & (kHash - 1u);
+				}
+				atomicOr(&s_bits[h], 1u << (uint32_t)(P & 31u));
 			}
-#pragma unroll
-			for (uint32_t d = 1; d < 64u; d <<= 1) {
-				const uint64_t ow = __shfl_up(word[u], d, 64);
-				const uint32_t ob = __shfl_up(bits[u], d, 64);
-				bits[u] |= (lane >= d && ow == word[u]) ? ob : 0u; // any lane with the same word may contribute
-			}
-			const uint64_t next = __shfl_down(word[u], 1u, 64);
-			last[u] = word[u] != ~0ull && (lane == 63u || next != word[u]);
 		}
-		uint32_t have[kU];
+		__syncthreads();
+		uint64_t word[kHash / kThreads];
+		uint32_t bits[kHash / kThreads], have[kHash / kThreads];
 #pragma unroll
-		for (uint32_t u = 0; u < kU; ++u)
-			have[u] = last[u] ? B[word[u]] : ~0u;
+		for (uint32_t k = 0; k < kHash / kThreads; ++k) {
+			const uint32_t h = threadIdx.x + k * kThreads;
+			word[k] = s_word[h];
+			bits[k] = s_bits[h];
+			s_word[h] = ~0ull;
+			s_bits[h] = 0u;
+		}
 #pragma unroll
-		for (uint32_t u = 0; u < kU; ++u)
-			if ((have[u] & bits[u]) != bits[u]) // a stale cached word only costs a redundant atomic
-				atomicOr(B + word[u], bits[u]);
+		for (uint32_t k = 0; k < kHash / kThreads; ++k)
+			have[k] = word[k] != ~0ull ? B[word[k]] : ~0u;
+#pragma unroll
+		for (uint32_t k = 0; k < kHash / kThreads; ++k)
+			if ((have[k] & bits[k]) != bits[k]) // a stale cached word only costs a redundant atomic
+				atomicOr(B + word[k], bits[k]);
+		__syncthreads();
 	}
 }
 
@@ -838,6 +860,7 @@ struct BinOffsets {
 	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { aux[i] = (uint32_t)k; }
 };
 
+// Blocks for a grid-stride loop over items (kThreads per block and trip), at most 16 per CU.
 uint32_t grid_for(uint64_t items, svo_ctx *ctx) {
 	uint64_t g = (items + kThreads - 1) / kThreads;
 	const uint64_t cap = (uint64_t)ctx->cu_count * 16u;
@@ -859,6 +882,9 @@ void levels_for(uint32_t L, uint64_t *off) {
 	}
 }
 
+// Blocks for the per-fragment passes (kU fragments per thread and trip).
+uint32_t grid_tiles(uint64_t F, svo_ctx *ctx) { return grid_for(div_up(F, kU), ctx); }
+
 uint64_t big_capacity(uint64_t F) { return F / (kRegBucket + 1u) + 1u; }
 
 int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
@@ -869,7 +895,7 @@ int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
 		    hipSuccess)
 			return hip_check(ctx, e, "bm_zero");
 		if (bb.Ftot > 0 &&
-		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_for(div_up(bb.Ftot, kU), ctx)), dim3(kThreads), 0,
+		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_tiles(bb.Ftot, ctx)), dim3(kThreads), 0,
 		                (const BuildSeg *)bb.segs, bb.fbeg(), S, bb.Ftot, bb.bm)) != hipSuccess)
 			return hip_check(ctx, e, "bm_mark");
 		for (uint32_t d = bb.maxL >= 3 ? bb.maxL - 2 : 0; d >= 1; --d) {
@@ -914,7 +940,7 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	if ((e = launch(ctx, "bin_zero", bm_zero, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, bb.cnt, bb.nbins)) !=
 	    hipSuccess)
 		return hip_check(ctx, e, "bin_zero");
-	const uint32_t gF = grid_for(div_up(bb.Ftot, kU), ctx); // kU fragments per thread and trip
+	const uint32_t gF = grid_tiles(bb.Ftot, ctx);
 	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
 	if (bb.Ftot > 0 && (e = launch(ctx, "build_tag_leaf", build_tag_leaf, dim3(gF), dim3(kThreads), 0, segs,
 	                               bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk, node,

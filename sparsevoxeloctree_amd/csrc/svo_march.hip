@@ -348,11 +348,13 @@ __device__ __forceinline__ void frame_pixel(const uint32_t *__restrict__ oct, ui
 	march_leaf<0, kStride>(oct, W, o, d, stk, err, r0, r1);
 }
 
-// One wave per block: a frame has only (W/8+1)(H/8+1) beam cells (33K at 1920x1080), so small
-// blocks spread them over every CU instead of filling half of them.
-constexpr uint32_t kBeamBlock = 64;
+// 16 beam cells per block (one wave, 16 lanes active): a frame has only (W/8+1)(H/8+1) beam cells
+// (33K at 1920x1080), too few to fill the GPU one per lane.  Spreading them thinly gives every SIMD
+// several waves to switch between while node loads are in flight, and a wave lasts as long as the
+// longest of 16 coarse marches instead of 64.
+constexpr uint32_t kBeamBlock = 16;
 
-__global__ void __launch_bounds__(kBeamBlock)
+__global__ void __launch_bounds__(64)
     march_beam_kernel(const uint32_t *__restrict__ oct, uint64_t W, CamArgs cam, uint32_t width, uint32_t height,
                       uint32_t bw, uint32_t bh, float orig_sz, float *__restrict__ beam, uint32_t *__restrict__ nf_out,
                       uint32_t *err_out) {
