@@ -371,17 +371,32 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Leaf block ("bin") of every fragment: the rank of its depth-(L-1) node, offset by the scene's
-// first bin.  Fragments are counted per bin with one atomic per run of equal neighbours; the
-// atomic's old value gives every fragment its place inside the bin, so node[f] = {bin, place} and
-// the scatter needs no atomics.
+// first bin; node[f] = {bin, place}, where place is unique inside the bin, so the scatter needs no
+// atomics.  Places come from per-bin counters in global memory whose atomics the L2 cannot perform
+// (they are device-coherent, so they go to memory), and each such request stalls the L2 for the
+// reads behind it.  So a loop trip first counts its kU * kThreads fragments in LDS, per 64-byte line
+// of 16 counters (a line table with linear probing; the count's old value is the fragment's place
+// among the trip's fragments of its bin), and then adds each line's counts with 16 neighbouring
+// lanes, which the L1 merges into one request per line: a trip of 1024 fragments touches about
+// 40 lines instead of 200+.  The two tables alternate between trips, so each is cleared while the
+// other is in use.  A fragment that finds the table full takes its place with its own atomic.
+constexpr uint32_t kLineBins = 16; // 4-byte counters per 64-byte line
+constexpr uint32_t kLines = 128;   // line slots of one table
+
 __global__ void __launch_bounds__(kThreads)
     build_tag_leaf(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
                    const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint2 *__restrict__ node,
                    uint32_t *cnt) {
-	const uint32_t lane = threadIdx.x & 63u;
+	__shared__ uint32_t s_line[2][kLines];
+	__shared__ uint32_t s_n[2][kLines * kLineBins];
+	for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
+		s_line[0][i] = s_line[1][i] = kEmpty;
+	for (uint32_t i = threadIdx.x; i < kLines * kLineBins; i += kThreads)
+		s_n[0][i] = s_n[1][i] = 0u;
+	__syncthreads();
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
-	uint32_t hint = 0;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
+	uint32_t hint = 0, buf = 0;
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride, buf ^= 1u) {
 		FragSrc src[kU];
 		frag_sources(segs, fbeg, S, Ftot, base, hint, src);
 		uint2 fr[kU];
@@ -402,30 +417,57 @@ __global__ void __launch_bounds__(kThreads)
 				below[u] = word & ((1u << (uint32_t)(P & 31u)) - 1u);
 			}
 		}
-#pragma unroll
-		for (uint32_t u = 0; u < kU; ++u)
-			bin[u] = src[u].frag ? src[u].bin_begin + (r[u] - r0[u]) + (uint32_t)__popc(below[u]) : 0xFFFFFFFFu;
-		uint32_t pos[kU], head_lane[kU];
+		uint32_t *line = s_line[buf], *n = s_n[buf];
+		uint32_t slot[kU], local[kU];
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
-			const uint32_t prev = __shfl_up(bin[u], 1u, 64);
-			const bool head = (lane == 0u) || prev != bin[u];
-			const uint64_t heads = __ballot(head);
-			const uint64_t after = (lane == 63u) ? 0ull : (heads >> (lane + 1u));
-			const uint32_t run = after ? (uint32_t)__ffsll((unsigned long long)after) : 64u - lane;
-			pos[u] = 0u;
-			if (head && bin[u] != 0xFFFFFFFFu)
-				pos[u] = atomicAdd(&cnt[bin[u]], run);
-			// lane of this lane's run head = highest head at or below this lane
-			const uint64_t upto = heads & ((lane == 63u) ? ~0ull : ((2ull << lane) - 1ull));
-			head_lane[u] = 63u - (uint32_t)__clzll((long long)upto);
+			bin[u] = src[u].frag ? src[u].bin_begin + (r[u] - r0[u]) + (uint32_t)__popc(below[u]) : kEmpty;
+			slot[u] = kEmpty;
+			local[u] = 0u;
+			if (bin[u] != kEmpty) {
+				const uint32_t key = bin[u] / kLineBins;
+				uint32_t h = (key * 2654435761u) >> 25, probes = 0; // 7 bits = kLines
+				for (;;) {
+					const uint32_t old = atomicCAS(&line[h], kEmpty, key);
+					if (old == kEmpty || old == key) {
+						slot[u] = h * kLineBins + bin[u] % kLineBins;
+						break;
+					}
+					if (++probes == kLines)
+						break; // table full
+					h = (h + 1u) & (kLines - 1u);
+				}
+				local[u] = slot[u] != kEmpty ? atomicAdd(&n[slot[u]], 1u) : atomicAdd(&cnt[bin[u]], 1u);
+			}
+		}
+		__syncthreads();
+		{ // the other table was last read before the barrier above: clear it for the next trip
+			uint32_t *ol = s_line[buf ^ 1u], *on = s_n[buf ^ 1u];
+			for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
+				ol[i] = kEmpty;
+			for (uint32_t i = threadIdx.x; i < kLines * kLineBins; i += kThreads)
+				on[i] = 0u;
+		}
+		// one atomic per counter with fragments; lanes 16k..16k+15 hold the 16 counters of one line
+		uint32_t key[kLines * kLineBins / kThreads], c[kLines * kLineBins / kThreads];
+#pragma unroll
+		for (uint32_t k = 0; k < kLines * kLineBins / kThreads; ++k) {
+			const uint32_t p = threadIdx.x + k * kThreads;
+			key[k] = line[p / kLineBins];
+			c[k] = n[p];
 		}
 #pragma unroll
+		for (uint32_t k = 0; k < kLines * kLineBins / kThreads; ++k) {
+			const uint32_t p = threadIdx.x + k * kThreads;
+			if (key[k] != kEmpty && c[k] != 0u)
+				n[p] = atomicAdd(&cnt[key[k] * kLineBins + p % kLineBins], c[k]);
+		}
+		__syncthreads();
+#pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
-			const uint32_t p = __shfl(pos[u], (int)head_lane[u], 64) + (lane - head_lane[u]);
 			const uint64_t f = base + (uint64_t)u * kThreads + threadIdx.x;
-			if (f < Ftot)
-				node[f] = make_uint2(bin[u], p);
+			if (bin[u] != kEmpty)
+				node[f] = make_uint2(bin[u], slot[u] != kEmpty ? n[slot[u]] + local[u] : local[u]);
 		}
 	}
 }
