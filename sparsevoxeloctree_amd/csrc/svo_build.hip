@@ -81,22 +81,26 @@ __device__ __forceinline__ void cmp_swap(uint64_t &x, uint64_t &y) {
 
 __device__ __forceinline__ uint64_t frag_key(uint2 f) { return ((uint64_t)f.x << 32) | f.y; }
 
-// Set bits of the bitmaps before word w (a global word index) and the word itself: the rank of
-// w's 8-word chunk plus the popcounts of the chunk's words before w (one 32-byte load).
-__device__ __forceinline__ uint32_t bits_before(const uint32_t *__restrict__ B, const uint32_t *__restrict__ R,
-                                                uint64_t w, uint32_t &word) {
-	const uint4 *c = reinterpret_cast<const uint4 *>(B + (w & ~7ull));
+// Set bits of the words of an 8-word chunk before word j (< 8), and word j (one 32-byte load).
+__device__ __forceinline__ uint32_t chunk_before(const uint32_t *__restrict__ chunk, uint32_t j, uint32_t &word) {
+	const uint4 *c = reinterpret_cast<const uint4 *>(chunk);
 	const uint4 a = c[0], b = c[1];
 	const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-	const uint32_t k = (uint32_t)(w & 7u);
-	uint32_t r = R[w >> 3], cur = v[0];
+	uint32_t r = 0u, cur = v[0];
 #pragma unroll
-	for (uint32_t j = 0; j < 8u; ++j) {
-		r += j < k ? (uint32_t)__popc(v[j]) : 0u;
-		cur = j == k ? v[j] : cur;
+	for (uint32_t t = 0; t < 8u; ++t) {
+		r += t < j ? (uint32_t)__popc(v[t]) : 0u;
+		cur = t == j ? v[t] : cur;
 	}
 	word = cur;
 	return r;
+}
+
+// Set bits of the bitmaps before word w (a global word index) and the word itself: the rank of
+// w's 8-word chunk plus the popcounts of the chunk's words before w.
+__device__ __forceinline__ uint32_t bits_before(const uint32_t *__restrict__ B, const uint32_t *__restrict__ R,
+                                                uint64_t w, uint32_t &word) {
+	return R[w >> 3] + chunk_before(B + (w & ~7ull), (uint32_t)(w & 7u), word);
 }
 
 __global__ void __launch_bounds__(kThreads) bm_zero(uint32_t *B, uint64_t n) {
@@ -312,61 +316,134 @@ __global__ void bm_write_root(const BuildSeg *__restrict__ segs, uint32_t S, con
 	dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
 }
 
-// Blocks of depth d >= 2 (the children of depth-(d-1) nodes), one thread per word of B[d-1]: for
-// every occupied parent q of the word, in order, its block (the parent's rank among depth d-1)
-// is written whole: pointer words for the occupied children (bits of byte q of B[d]), zeros
-// otherwise.  Items are the words of B[d-1] of the scenes with L > d (begin table dr[d-1]), so
-// the scan covers the parents' bitmap, 8x smaller than the children's.
+// The 8 words of the block of one depth-(d-1) parent: pointers to the blocks of its occupied
+// children (bits of its byte in B[d]), zeros otherwise; r = rank of its first child among depth d.
+__device__ __forceinline__ void store_parent_block(uint32_t *oct, uint64_t block, uint32_t byte, uint32_t fbd,
+                                                   uint32_t r) {
+	uint32_t wd[8];
+#pragma unroll
+	for (uint32_t t = 0; t < 8u; ++t) {
+		const bool set = (byte >> t) & 1u;
+		wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
+		r += set ? 1u : 0u;
+	}
+	uint4 *dst = reinterpret_cast<uint4 *>(oct + 8u * block);
+	dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
+	dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
+}
+
+// Position of the n-th (from 0) set bit of v (n < popc(v)).
+__device__ __forceinline__ uint32_t nth_bit(uint32_t v, uint32_t n) {
+	uint32_t pos = 0u;
+#pragma unroll
+	for (uint32_t half = 16u; half >= 1u; half >>= 1) {
+		const uint32_t c = (uint32_t)__popc(v & ((1u << half) - 1u));
+		if (n >= c) {
+			n -= c;
+			pos += half;
+			v >>= half;
+		}
+	}
+	return pos;
+}
+
+// Blocks of depth d >= 2 (the children of depth-(d-1) nodes): every occupied parent's block (the
+// parent's rank among depth d-1) is written whole.  Items are the words of B[d-1] of the scenes
+// with L > d (begin table dr[d-1]), so the scan covers the parents' bitmap, 8x smaller than the
+// children's.  A wave whose 64 words lie in one scene works cooperatively: the lanes' popcounts
+// are prefix-summed, and lane k of each round writes the wave's k-th occupied parent, so every
+// lane has work and the 64 blocks of a round are contiguous.  A wave across a scene boundary
+// walks each word's set bits in its own lane.
 __global__ void __launch_bounds__(kThreads)
     bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dp, uint32_t S, uint64_t n,
                     uint32_t d, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R) {
+	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
 		const uint64_t i = base + threadIdx.x;
-		if (i >= n)
-			continue;
-		with_seg<kThreads>(dp, S, hint, base, i, [&](uint32_t s) {
-			const BuildSeg &g = segs[s];
-			const uint64_t w = i - dp[s];
+		uint32_t s = kEmpty;
+		if (i < n)
+			with_seg<kThreads>(dp, S, hint, base, i, [&](uint32_t t) { s = t; });
+		const uint32_t s0 = __shfl(s, 0, 64);
+		if (s0 == kEmpty)
+			continue; // the wave is past the end (lane 0 holds its first item)
+		if (__all(s == s0 || s == kEmpty)) {
+			const BuildSeg &g = segs[__builtin_amdgcn_readfirstlane(s0)];
 			const uint64_t offp = g.bm + g.off[d - 1u], offc = g.bm + g.off[d]; // both start chunks
-			uint32_t v;
-			const uint32_t rp = bits_before(B, R, offp + w, v);
-			if (v == 0u)
-				return;
-			const uint32_t fbd = g.fb[d];
-			uint32_t block = g.fb[d - 1u] + (rp - R[offp >> 3]); // first parent's block
-			// the 32 parents' children bytes are the 8-word chunk offc + 8w; parents not in v have
-			// zero bytes, so a running count gives every child its rank
-			const uint4 *cc = reinterpret_cast<const uint4 *>(B + offc + 8u * w);
-			const uint4 c0 = cc[0], c1 = cc[1];
-			uint32_t r = R[(offc >> 3) + w] - R[offc >> 3];
-			uint32_t *oct = g.words;
-			while (v) {
-				const uint32_t b = (uint32_t)__ffs(v) - 1u, j = b >> 2u;
-				v &= v - 1u;
-				uint32_t cv = c0.x;
-				cv = j == 1u ? c0.y : cv;
-				cv = j == 2u ? c0.z : cv;
-				cv = j == 3u ? c0.w : cv;
-				cv = j == 4u ? c1.x : cv;
-				cv = j == 5u ? c1.y : cv;
-				cv = j == 6u ? c1.z : cv;
-				cv = j == 7u ? c1.w : cv;
-				const uint32_t byte = (cv >> (8u * (b & 3u))) & 0xffu;
-				uint32_t wd[8];
+			const uint64_t w0 = (i - lane) - dp[__builtin_amdgcn_readfirstlane(s0)]; // lane 0's word
+			const uint32_t v = s != kEmpty ? B[offp + w0 + lane] : 0u, c = (uint32_t)__popc(v);
+			uint32_t incl = c;
 #pragma unroll
-				for (uint32_t t = 0; t < 8u; ++t) {
-					const bool set = (byte >> t) & 1u;
-					wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
-					r += set ? 1u : 0u;
-				}
-				uint4 *dst = reinterpret_cast<uint4 *>(oct + 8u * (uint64_t)block);
-				dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-				dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
-				++block;
+			for (uint32_t dd = 1; dd < 64u; dd <<= 1) {
+				const uint32_t t = __shfl_up(incl, dd, 64);
+				incl += lane >= dd ? t : 0u;
 			}
-		});
+			const uint32_t excl = incl - c, T = __shfl(incl, 63, 64);
+			if (T == 0u)
+				continue;
+			uint32_t first = 0u; // rank of the wave's first parent among depth d-1
+			if (lane == 0u) {
+				uint32_t dummy;
+				first = bits_before(B, R, offp + w0, dummy) - R[offp >> 3];
+			}
+			const uint64_t blk0 = (uint64_t)g.fb[d - 1u] + __shfl(first, 0, 64);
+			const uint32_t fbd = g.fb[d];
+			// children of depth d before this word's 8-word chunk of B[d] (read only for occupied
+			// words: an empty padding word of a small B[d-1] may have no chunk)
+			const uint32_t rc = v != 0u ? R[(offc >> 3) + w0 + lane] - R[offc >> 3] : 0u;
+			for (uint32_t t0 = 0; t0 < T; t0 += 64u) {
+				const uint32_t k = t0 + lane;
+				uint32_t o = 0u; // owner lane: the last with excl <= k
+#pragma unroll
+				for (uint32_t step = 32u; step >= 1u; step >>= 1) {
+					const uint32_t e = __shfl(excl, (int)(o + step), 64);
+					o = e <= k ? o + step : o;
+				}
+				const uint32_t vo = __shfl(v, (int)o, 64), eo = __shfl(excl, (int)o, 64), ro = __shfl(rc, (int)o, 64);
+				if (k < T) {
+					const uint32_t b = nth_bit(vo, k - eo); // the parent's bit in the owner's word
+					uint32_t cv; // the owner's chunk of B[d]: its word b / 4 holds the parent's byte
+					const uint32_t before = chunk_before(B + offc + 8u * (w0 + o), b >> 2u, cv);
+					const uint32_t sh = 8u * (b & 3u);
+					const uint32_t r = ro + before + (uint32_t)__popc(cv & ((1u << sh) - 1u));
+					store_parent_block(g.words, blk0 + k, (cv >> sh) & 0xffu, fbd, r);
+				}
+			}
+			continue;
+		}
+		if (s == kEmpty)
+			continue;
+		const BuildSeg &g = segs[s];
+		const uint64_t w = i - dp[s];
+		const uint64_t offp = g.bm + g.off[d - 1u], offc = g.bm + g.off[d]; // both start chunks
+		uint32_t v;
+		const uint32_t rp = bits_before(B, R, offp + w, v);
+		if (v == 0u)
+			continue;
+		const uint32_t fbd = g.fb[d];
+		uint64_t block = (uint64_t)g.fb[d - 1u] + (rp - R[offp >> 3]); // first parent's block
+		// the 32 parents' children bytes are the 8-word chunk offc + 8w; parents not in v have zero
+		// bytes, so a running count gives every child its rank
+		const uint4 *cc = reinterpret_cast<const uint4 *>(B + offc + 8u * w);
+		const uint4 c0 = cc[0], c1 = cc[1];
+		uint32_t r = R[(offc >> 3) + w] - R[offc >> 3];
+		while (v) {
+			const uint32_t b = (uint32_t)__ffs(v) - 1u, j = b >> 2u;
+			v &= v - 1u;
+			uint32_t cv = c0.x;
+			cv = j == 1u ? c0.y : cv;
+			cv = j == 2u ? c0.z : cv;
+			cv = j == 3u ? c0.w : cv;
+			cv = j == 4u ? c1.x : cv;
+			cv = j == 5u ? c1.y : cv;
+			cv = j == 6u ? c1.z : cv;
+			cv = j == 7u ? c1.w : cv;
+			const uint32_t byte = (cv >> (8u * (b & 3u))) & 0xffu;
+			store_parent_block(g.words, block, byte, fbd, r);
+			r += (uint32_t)__popc(byte);
+			++block;
+		}
 	}
 }
 

@@ -132,23 +132,21 @@ __device__ __forceinline__ bool march_step(const uint32_t *__restrict__ oct, uin
 		if (kCoarse && orig_sz + tc_max * dir_sz >= m.scale_exp2)
 			return false;
 		// INTERSECT
-		const bool leaf = (m.cur & 0x40000000u) != 0u;
-		if (leaf || m.scale < kMinScale) {
-			if (!leaf) { // a push below the stack's last slot
-				err |= kErrTooDeep;
-				m.scale = kStackSize; // ends the ray as a miss
-			}
-			return false;
-		}
-		// PUSH
+		if ((m.cur & 0x40000000u) != 0u)
+			return false; // leaf
+		// PUSH.  A push below the stack's last slot is an error that ends the ray as a miss: it
+		// writes no slot, and the scale it leaves (kStackSize) stops the loop before the next trip
+		// counts, as an immediate break would.
+		const bool too_deep = m.scale < kMinScale;
+		err |= too_deep ? kErrTooDeep : 0u;
 		const float half_scale_exp2 = m.scale_exp2 * 0.5f;
 		f3 t_center{half_scale_exp2 * m.t_coef.x + t_corner.x, half_scale_exp2 * m.t_coef.y + t_corner.y,
 		            half_scale_exp2 * m.t_coef.z + t_corner.z};
-		if (tc_max < m.h)
+		if (tc_max < m.h && !too_deep)
 			stk_s[m.scale * kStride] = m.parent;
 		m.h = tc_max;
 		m.parent = m.cur & 0x3fffffffu;
-		--m.scale;
+		m.scale = too_deep ? kStackSize : m.scale - 1u;
 		m.scale_exp2 = half_scale_exp2;
 		const bool cx = t_center.x > m.t_min, cy = t_center.y > m.t_min, cz = t_center.z > m.t_min;
 		m.idx = (cx ? 1u : 0u) | (cy ? 2u : 0u) | (cz ? 4u : 0u);
@@ -309,6 +307,8 @@ __device__ __forceinline__ float beam_cell(const uint32_t *__restrict__ oct, uin
 	march_init<false>(o, d, m);
 	uint32_t iter = 0u;
 	for (;;) {
+		if (m.scale >= kStackSize) // after a push below the stack's last slot
+			break;
 		if (++iter > kMaxIter) {
 			err |= kErrIterCap;
 			break;
