@@ -455,25 +455,26 @@ __global__ void __launch_bounds__(kThreads)
 // of 16 counters (a line table with linear probing; the count's old value is the fragment's place
 // among the trip's fragments of its bin), and then adds each line's counts with 16 neighbouring
 // lanes, which the L1 merges into one request per line: a trip of 1024 fragments touches about
-// 40 lines instead of 200+.  The two tables alternate between trips, so each is cleared while the
-// other is in use.  A fragment that finds the table full takes its place with its own atomic.
+// 40 lines (a room scene at 2^10) to 150 (random slivers at 2^12) instead of 200+; kLines line slots
+// (128 up to level 11, 256 at level 12).  A fragment that finds the table full takes its place with
+// its own atomic.
 constexpr uint32_t kLineBins = 16; // 4-byte counters per 64-byte line
-constexpr uint32_t kLines = 128;   // line slots of one table
 
+template <uint32_t kLines>
 __global__ void __launch_bounds__(kThreads)
     build_tag_leaf(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
                    const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint2 *__restrict__ node,
                    uint32_t *cnt) {
-	__shared__ uint32_t s_line[2][kLines];
-	__shared__ uint32_t s_n[2][kLines * kLineBins];
+	__shared__ uint32_t line[kLines];
+	__shared__ uint32_t n[kLines * kLineBins];
 	for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
-		s_line[0][i] = s_line[1][i] = kEmpty;
+		line[i] = kEmpty;
 	for (uint32_t i = threadIdx.x; i < kLines * kLineBins; i += kThreads)
-		s_n[0][i] = s_n[1][i] = 0u;
+		n[i] = 0u;
 	__syncthreads();
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
-	uint32_t hint = 0, buf = 0;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride, buf ^= 1u) {
+	uint32_t hint = 0;
+	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
 		FragSrc src[kU];
 		frag_sources(segs, fbeg, S, Ftot, base, hint, src);
 		uint2 fr[kU];
@@ -494,7 +495,6 @@ __global__ void __launch_bounds__(kThreads)
 				below[u] = word & ((1u << (uint32_t)(P & 31u)) - 1u);
 			}
 		}
-		uint32_t *line = s_line[buf], *n = s_n[buf];
 		uint32_t slot[kU], local[kU];
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
@@ -503,7 +503,8 @@ __global__ void __launch_bounds__(kThreads)
 			local[u] = 0u;
 			if (bin[u] != kEmpty) {
 				const uint32_t key = bin[u] / kLineBins;
-				uint32_t h = (key * 2654435761u) >> 25, probes = 0; // 7 bits = kLines
+				static_assert(kLines == 128u || kLines == 256u, "kLines: 7 or 8 hash bits");
+				uint32_t h = (key * 2654435761u) >> (kLines == 256u ? 24 : 25), probes = 0;
 				for (;;) {
 					const uint32_t old = atomicCAS(&line[h], kEmpty, key);
 					if (old == kEmpty || old == key) {
@@ -518,26 +519,24 @@ __global__ void __launch_bounds__(kThreads)
 			}
 		}
 		__syncthreads();
-		{ // the other table was last read before the barrier above: clear it for the next trip
-			uint32_t *ol = s_line[buf ^ 1u], *on = s_n[buf ^ 1u];
-			for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
-				ol[i] = kEmpty;
-			for (uint32_t i = threadIdx.x; i < kLines * kLineBins; i += kThreads)
-				on[i] = 0u;
-		}
 		// one atomic per counter with fragments; lanes 16k..16k+15 hold the 16 counters of one line
-		uint32_t key[kLines * kLineBins / kThreads], c[kLines * kLineBins / kThreads];
+		constexpr uint32_t kPer = kLines * kLineBins / kThreads, kGroup = 8; // counters per thread, per batch
+		static_assert(kPer % kGroup == 0u, "whole batches");
+#pragma unroll 1
+		for (uint32_t k0 = 0; k0 < kPer; k0 += kGroup) {
+			uint32_t key[kGroup], c[kGroup];
 #pragma unroll
-		for (uint32_t k = 0; k < kLines * kLineBins / kThreads; ++k) {
-			const uint32_t p = threadIdx.x + k * kThreads;
-			key[k] = line[p / kLineBins];
-			c[k] = n[p];
-		}
+			for (uint32_t k = 0; k < kGroup; ++k) {
+				const uint32_t p = threadIdx.x + (k0 + k) * kThreads;
+				key[k] = line[p / kLineBins];
+				c[k] = n[p];
+			}
 #pragma unroll
-		for (uint32_t k = 0; k < kLines * kLineBins / kThreads; ++k) {
-			const uint32_t p = threadIdx.x + k * kThreads;
-			if (key[k] != kEmpty && c[k] != 0u)
-				n[p] = atomicAdd(&cnt[key[k] * kLineBins + p % kLineBins], c[k]);
+			for (uint32_t k = 0; k < kGroup; ++k) {
+				const uint32_t p = threadIdx.x + (k0 + k) * kThreads;
+				if (key[k] != kEmpty && c[k] != 0u)
+					n[p] = atomicAdd(&cnt[key[k] * kLineBins + p % kLineBins], c[k]);
+			}
 		}
 		__syncthreads();
 #pragma unroll
@@ -546,6 +545,12 @@ __global__ void __launch_bounds__(kThreads)
 			if (bin[u] != kEmpty)
 				node[f] = make_uint2(bin[u], slot[u] != kEmpty ? n[slot[u]] + local[u] : local[u]);
 		}
+		__syncthreads();
+		for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
+			line[i] = kEmpty;
+		for (uint32_t i = threadIdx.x; i < kLines * kLineBins; i += kThreads)
+			n[i] = 0u;
+		__syncthreads();
 	}
 }
 
@@ -1061,9 +1066,10 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 		return hip_check(ctx, e, "bin_zero");
 	const uint32_t gF = grid_tiles(bb.Ftot, ctx);
 	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
-	if (bb.Ftot > 0 && (e = launch(ctx, "build_tag_leaf", build_tag_leaf, dim3(gF), dim3(kThreads), 0, segs,
-	                               bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk, node,
-	                               bb.cnt)) != hipSuccess)
+	if (bb.Ftot > 0 &&
+	    (e = launch(ctx, "build_tag_leaf", bb.maxL >= 12u ? build_tag_leaf<256> : build_tag_leaf<128>, dim3(gF),
+	                dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk,
+	                node, bb.cnt)) != hipSuccess)
 		return hip_check(ctx, e, "build_tag_leaf");
 	BinCounts bc{bb.cnt, bb.nbins};
 	if ((e = launch(ctx, "bin_reduce", scan_reduce<BinCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
