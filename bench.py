@@ -138,18 +138,21 @@ def cpu_baseline(S, cfg, cam, words, budget_s=20.0):
     oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + 8, beam, threads=cores)
     per_row = (time.perf_counter() - t0) / 8
     rows = int(max(8, min(h, budget_s / max(per_row, 1e-6))))
-    reps = 1
-    if rows == h:
-        # the whole frame fits the budget: march it (beam pre-pass included) as many times as fit
-        reps = int(max(1, min(1000, budget_s / max(per_row * h + t_beam, 1e-6))))
     y0 = max(0, h // 2 - rows // 2)
     t0 = time.perf_counter()
-    for _ in range(reps):
-        if reps > 1:
-            beam, _ = oracle.beam(words, level, cam_arr, w, h, threads=cores)
-        oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + rows, beam, threads=cores)
+    oracle.march_frame_rows(words, level, cam_arr, w, h, y0, y0 + rows, beam, threads=cores)
     t_rows = time.perf_counter() - t0
-    t = t_rows if reps > 1 else t_rows + t_beam * rows / h  # the beam pass's share for those rows
+    t = t_rows + t_beam * rows / h  # the beam pass's share for those rows
+    reps = 1
+    if rows == h and t < budget_s:
+        # the whole frame took less than the budget: march it again (beam pre-pass included) as
+        # many times as fit
+        reps = int(max(1, min(1000, budget_s / max(t, 1e-6))))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            beam, _ = oracle.beam(words, level, cam_arr, w, h, threads=cores)
+            oracle.march_frame_rows(words, level, cam_arr, w, h, 0, h, beam, threads=cores)
+        t = time.perf_counter() - t0
     return {
         "value": round(reps * rows * w / t / 1e6, 4),
         "unit": "Mrays/s",
