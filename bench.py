@@ -428,6 +428,7 @@ def bench_serve(args, cfg):
     wall = time.perf_counter() - t0
     st = srv.stats()
     nb, nr = int(kinds.sum()), int((~kinds).sum())
+    capacity = serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits)
     line = {
         "metric": METRIC,
         "value": round(st["rays_traced"] / wall / 1e6, 3),
@@ -458,12 +459,91 @@ def bench_serve(args, cfg):
         "gpu_busy_build_s": round(st["busy_build_ms"] / 1e3, 3),
         "gpu_busy_rays_s": round(st["busy_rays_ms"] / 1e3, 3),
         "server_batches": st["batches"],
+        "capacity": capacity,
         "cpu_baseline": None,
         "roofline": None,
     }
     srv.close()
     ctx.close()
     return line
+
+
+def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1000, n_targets=64):
+    """Aggregate throughput of the same request list (10% builds, 90% ray queries of log-U[4K,2M] rays
+    against n_targets octrees) in the plain form, one request per launch with a wait after each (the
+    single-scene API), and in the served form, all requests submitted at once to the serving loop.
+    Targets are built beforehand in both forms from the same scenes and levels."""
+    from sparsevoxeloctree_amd._capi import lib
+
+    RMAX = 2 * 1024 * 1024
+    kinds = rng.uniform(size=n) < 0.1
+    Rs = np.exp(rng.uniform(np.log(4096), np.log(RMAX), n)).astype(np.int64)
+    scene_i = rng.integers(len(pool), size=n)
+    level_i = rng.integers(len(levels), size=n)
+    target_i = rng.integers(n_targets, size=n)
+    set_i = rng.integers(len(ray_sets), size=n)
+    tgt = [(int(i % len(pool)), int(levels[i % len(levels)])) for i in range(n_targets)]
+    bufs = [free_hits.get() for _ in range(8)]
+    # plain form: octrees through the single-scene API, then every request alone, waited for
+    keep = []
+    plain_octs = []
+    for sc, lv in tgt:
+        vox = S.Voxelizer.Create(pool[sc], lv)
+        vox.CmdVoxelize()
+        b = S.OctreeBuilder.Create(vox)
+        b.CmdBuild()
+        keep.append((vox, b))
+        plain_octs.append(b.GetOctree())
+    ctx.synchronize()
+    rays_plain = 0
+    t0 = time.perf_counter()
+    for k in range(n):
+        if kinds[k]:
+            vox = S.Voxelizer.Create(pool[int(scene_i[k])], int(levels[int(level_i[k])]))
+            vox.CmdVoxelize()
+            b = S.OctreeBuilder.Create(vox)
+            b.CmdBuild()
+            b.GetOctree()
+            ctx.synchronize()
+            del vox, b
+        else:
+            o = plain_octs[int(target_i[k])]
+            ctx.check(lib.svo_march_rays(ctx.h, o.h, ray_sets[int(set_i[k])].ptr, int(Rs[k]),
+                                         0, bufs[k % len(bufs)].ptr), "march rays")
+            ctx.synchronize()
+            rays_plain += int(Rs[k])
+    t_plain = time.perf_counter() - t0
+    del keep, plain_octs
+    # served form: the same requests, submitted at once
+    ids = [srv.submit_build(pool[sc], lv) for sc, lv in tgt]
+    targets = []
+    for rid in ids:
+        st, _ = srv.wait(rid)
+        targets.append(rid if st == 0 else None)
+    rays_served, failed = 0, 0
+    t0 = time.perf_counter()
+    reqs = []
+    for k in range(n):
+        if kinds[k]:
+            reqs.append(srv.submit_build(pool[int(scene_i[k])], int(levels[int(level_i[k])])))
+        elif targets[int(target_i[k])] is not None:
+            reqs.append(srv.submit_rays(targets[int(target_i[k])], ray_sets[int(set_i[k])], int(Rs[k]),
+                                        bufs[k % len(bufs)], 0))
+            rays_served += int(Rs[k])
+    for rid in reqs:
+        st, _ = srv.wait(rid)
+        failed += st != 0
+    t_served = time.perf_counter() - t0
+    for b in bufs:
+        free_hits.put(b)
+    return {
+        "requests": n, "build_requests": int(kinds.sum()), "ray_requests": int((~kinds).sum()),
+        "plain_s": round(t_plain, 4), "served_s": round(t_served, 4),
+        "plain_requests_per_s": round(n / t_plain, 2), "served_requests_per_s": round(n / t_served, 2),
+        "plain_mrays_per_s": round(rays_plain / t_plain / 1e6, 3),
+        "served_mrays_per_s": round(rays_served / t_served / 1e6, 3),
+        "speedup": round(t_plain / t_served, 3), "served_failed": int(failed),
+    }
 
 
 def copy_calibration(mib):
@@ -666,7 +746,7 @@ def main():
             "speedup_build", "speedup_march", "speedup_total", "hits_identical_to_sequential", "fragments"))
         line["served"] = summarise_extra(bench_serve, args, CONFIGS[5], (
             "value", "ray_p50_ms", "ray_p99_ms", "build_p50_ms", "build_p99_ms", "builds_per_s",
-            "ray_queries_per_s", "builds_rejected", "ray_queries_failed", "evictions"))
+            "ray_queries_per_s", "builds_rejected", "ray_queries_failed", "evictions", "capacity"))
     print(json.dumps(line))
     sys.stdout.flush()
 

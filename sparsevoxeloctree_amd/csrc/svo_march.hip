@@ -901,8 +901,9 @@ constexpr int kFrameShapes = 4;
 const char *const kFrameShapeName[kFrameShapes] = {"16x16", "8x8", "16x8", "32x8"};
 constexpr uint32_t kShapeWX[kFrameShapes] = {2, 1, 2, 4};
 constexpr uint32_t kShapeWY[kFrameShapes] = {2, 1, 1, 1};
-constexpr int kRayShapes = 3;
-constexpr uint32_t kRayBlock[kRayShapes] = {256, 128, 64};
+// ray-batch schedules: one ray per lane in blocks of 256 / 128 / 64 threads, or persistent refill
+constexpr int kRayShapes = 4;
+constexpr uint32_t kRayBlock[3] = {256, 128, 64};
 
 MarchSeg march_seg(const svo_octree *o, const svo_camera *cam) {
 	MarchSeg g{};
@@ -1065,6 +1066,12 @@ int march_rays_segments(svo_ctx *ctx, const svo_octree *const *octs, const float
 	if (!rc)
 		rc = upload_table(ctx, rbeg, rb.data(), sizeof(uint64_t) * (S + 1));
 	auto run = [&](int sh) -> hipError_t {
+		if (sh == 3) {
+			const SegmentRays src{(const MarchSeg *)segs, (const uint64_t *)rbeg, S};
+			return variant == SVO_MARCH_LEAF_ITER ? launch_persist<0>(ctx, "march_rays_batch", src, R)
+			       : variant == SVO_MARCH_LEAF    ? launch_persist<1>(ctx, "march_rays_batch", src, R)
+			                                      : launch_persist<2>(ctx, "march_rays_batch", src, R);
+		}
 		const uint32_t bs = kRayBlock[sh];
 		const dim3 grid(div_up(R, bs)), block(bs);
 #define SVO_RAYS(V, T)                                                                                                 \
@@ -1077,25 +1084,16 @@ int march_rays_segments(svo_ctx *ctx, const svo_octree *const *octs, const float
 		return bs == 256 ? SVO_RAYS(2, 256) : bs == 128 ? SVO_RAYS(2, 128) : SVO_RAYS(2, 64);
 #undef SVO_RAYS
 	};
-	if (!rc && ctx->march_mode == 1) { // explicit choice; the block-size autotuner covers the default
-		const SegmentRays src{(const MarchSeg *)segs, (const uint64_t *)rbeg, S};
-		ctx->t_rays.start(ctx->stream);
-		const hipError_t e = variant == SVO_MARCH_LEAF_ITER ? launch_persist<0>(ctx, "march_rays_batch", src, R)
-		                     : variant == SVO_MARCH_LEAF    ? launch_persist<1>(ctx, "march_rays_batch", src, R)
-		                                                    : launch_persist<2>(ctx, "march_rays_batch", src, R);
-		ctx->t_rays.stop(ctx->stream);
-		ctx->pool->free(segs);
-		ctx->pool->free(rbeg);
-		return hip_check(ctx, e, "march_rays_batch");
-	}
-	int shape = 0;
-	if (!rc && ctx->autotune) {
+	// an explicit schedule (svo_march_mode 1: persistent; 0, 2: one ray per lane in blocks of 256),
+	// else the autotuner's choice among the four
+	int shape = ctx->march_mode == 1 ? 3 : 0;
+	if (!rc && ctx->march_mode < 0 && ctx->autotune) {
 		uint32_t lg = 0;
 		while ((1ull << (lg + 1)) <= R)
 			++lg;
 		char key[96];
 		std::snprintf(key, sizeof key, "march_rays_batch variant=%d R~2^%u", variant, lg);
-		static const char *const names[kRayShapes] = {"block256", "block128", "block64"};
+		static const char *const names[kRayShapes] = {"block256", "block128", "block64", "persistent-refill"};
 		rc = tune(ctx, key, kRayShapes, names, run, shape);
 	}
 	if (!rc) {
