@@ -46,6 +46,7 @@ constexpr int kThreads = 256;
 constexpr uint32_t kLdsBucket = 4096; // largest bucket sorted in LDS
 constexpr uint32_t kRegBucket = 16;   // leaf blocks up to this size: one thread each (build_fold)
 constexpr uint32_t kWaveBucket = 64;  // up to this size: one wave each (build_fold_mid)
+constexpr uint32_t kWideBucket = 512; // up to this size: one wave each, sorted in LDS (build_fold_wide)
 
 // Effective voxel coordinate seen by the reference's descent: the coordinate's bits, saturating
 // at res-1 for coordinates >= res (shader/octree_tag_node.comp:18-30).
@@ -644,15 +645,18 @@ __device__ __forceinline__ void store_block(uint32_t *oct, uint64_t block, const
 	dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
 }
 
-// Work lists of leaf blocks too large for build_fold: list[0] = mid count, list[1] = big count,
-// mid blocks at list[2 ..], big blocks at list[2 + cap ..] (each list holds at most Ftot / 17).
-// Every lane of the wave calls it (the appends are aggregated per wave).
-__device__ __forceinline__ void push_bins(uint32_t *list, uint64_t cap, uint32_t j, bool mid, bool big) {
-	const uint32_t qm = wave_append(&list[0], mid), qb = wave_append(&list[1], big);
+// Work lists of leaf blocks too large for build_fold: list[0..2] = mid, big and wide counts; mid
+// blocks (17..64 fragments) at list[3 ..], big ones (> 512) at list[3 + cap ..], wide ones
+// (65..512) at list[3 + 2 cap ..] (each list holds at most Ftot / 17).  Every lane of the wave calls
+// it (the appends are aggregated per wave).
+__device__ __forceinline__ void push_bins(uint32_t *list, uint64_t cap, uint32_t j, bool mid, bool wide, bool big) {
+	const uint32_t qm = wave_append(&list[0], mid), qb = wave_append(&list[1], big), qw = wave_append(&list[2], wide);
 	if (mid && qm < cap)
-		list[2u + qm] = j;
+		list[3u + qm] = j;
 	if (big && qb < cap)
-		list[2u + cap + qb] = j;
+		list[3u + cap + qb] = j;
+	if (wide && qw < cap)
+		list[3u + 2u * cap + qw] = j;
 }
 
 // Octree block of global leaf block j (scene s): the scene's first leaf block + j's rank.
@@ -731,7 +735,9 @@ __global__ void __launch_bounds__(kThreads)
 			n = cnt[jj];
 			cls = n <= 4u ? 0u : n <= 8u ? 1u : n <= kRegBucket ? 2u : 3u;
 		}
-		push_bins(list, cap, (uint32_t)jj, cls == 3u && jj < nbins && n <= kWaveBucket, jj < nbins && n > kWaveBucket);
+		const bool rest = jj < nbins && cls == 3u;
+		push_bins(list, cap, (uint32_t)jj, rest && n <= kWaveBucket, rest && n > kWaveBucket && n <= kWideBucket,
+		          rest && n > kWideBucket);
 		const uint64_t m0 = __ballot(cls == 0u), m1 = __ballot(cls == 1u), m2 = __ballot(cls == 2u);
 		if (lane == 0u) {
 			s_cls[wave][0] = (uint32_t)__popcll(m0);
@@ -820,7 +826,7 @@ __global__ void __launch_bounds__(kThreads)
 		const uint64_t item = base + wave;
 		if (item >= count)
 			break; // wave-uniform; no block barrier follows
-		const uint32_t j = list[2u + item];
+		const uint32_t j = list[3u + item];
 		const uint32_t n = cnt[j];
 		const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
 		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
@@ -865,6 +871,87 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
+// Leaf blocks of 65..512 fragments, one wave each: the (slot, key) pairs, padded to a power of two
+// with (8, +inf), are bitonic-sorted in the wave's LDS (no block barriers: the wave's LDS accesses
+// are ordered between stages by wave barriers), so every leaf's fragments are consecutive and
+// ascending; lane s < 8 then folds the run of slot s.
+
+__global__ void __launch_bounds__(kThreads)
+    build_fold_wide(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
+                    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
+                    const uint32_t *list, uint64_t cap) {
+	constexpr uint32_t kWaves = kThreads / 64;
+	__shared__ uint32_t s_magic[65];
+	__shared__ uint64_t s_keys[kWaves][kWideBucket];
+	__shared__ uint8_t s_slots[kWaves][kWideBucket];
+	__shared__ uint32_t s_beg[kWaves][8], s_end[kWaves][8];
+	stage_magic(s_magic);
+	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+	uint64_t *keys = s_keys[wave];
+	uint8_t *slots = s_slots[wave];
+	uint64_t count = list[2];
+	if (count > cap)
+		count = cap;
+	for (uint64_t base = (uint64_t)blockIdx.x * kWaves; base < count; base += (uint64_t)gridDim.x * kWaves) {
+		const uint64_t item = base + wave;
+		if (item >= count)
+			break; // wave-uniform; no block barrier follows
+		const uint32_t j = list[3u + 2u * cap + item];
+		const uint32_t n = cnt[j];
+		const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
+		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
+		const uint2 *b = bucket + aux[j];
+		uint32_t m = 128u;
+		while (m < n)
+			m <<= 1;
+		for (uint32_t i = lane; i < m; i += 64u) {
+			const uint64_t key = i < n ? frag_key(b[i]) : ~0ull;
+			keys[i] = key;
+			slots[i] = (uint8_t)(i < n ? key_slot(key, t.res) : 8u);
+		}
+		if (lane < 8u)
+			s_beg[wave][lane] = s_end[wave][lane] = 0u;
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+		for (uint32_t k = 2; k <= m; k <<= 1) {
+			for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
+				for (uint32_t q = lane; q < m / 2u; q += 64u) {
+					const uint32_t blk = q / jj, off = q % jj;
+					const uint32_t a = jj == (k >> 1) ? blk * k + off : blk * 2u * jj + off;
+					const uint32_t c = jj == (k >> 1) ? blk * k + k - 1u - off : a + jj;
+					const uint64_t ka = keys[a], kc = keys[c];
+					const uint8_t sa = slots[a], sc = slots[c];
+					if (sc < sa || (sc == sa && kc < ka)) {
+						keys[a] = kc;
+						keys[c] = ka;
+						slots[a] = sc;
+						slots[c] = sa;
+					}
+				}
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+			}
+		}
+		for (uint32_t i = lane; i < n; i += 64u) {
+			const uint32_t sl = slots[i];
+			if (i == 0u || slots[i - 1u] != sl)
+				s_beg[wave][sl] = i;
+			if (i + 1u == n || slots[i + 1u] != sl)
+				s_end[wave][sl] = i + 1u;
+		}
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+		if (lane < 8u) {
+			uint32_t word = 0u;
+			for (uint32_t i = s_beg[wave][lane]; i < s_end[wave][lane]; ++i)
+				word = fold_word(word, (uint32_t)keys[i] & 0xffffffu, s_magic);
+			t.oct[8u * t.block + lane] = word;
+		}
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+	}
+}
+
 // Bitonic sort of n (<= m = 2^p) (slot, key) pairs held in LDS by one block, ascending; pairs past
 // n behave as +infinity padding.
 __device__ void block_bitonic_kv(uint32_t n, uint32_t m, uint64_t *keys, uint8_t *slots) {
@@ -896,7 +983,7 @@ __device__ void block_bitonic_kv(uint32_t n, uint32_t m, uint64_t *keys, uint8_t
 	}
 }
 
-// Leaf blocks of more than 64 fragments, one block each.  Up to 4096 fragments: bitonic sort of the
+// Leaf blocks of more than 512 fragments, one block each.  Up to 4096 fragments: bitonic sort of the
 // (slot, key) pairs in LDS, then lane s < 8 folds the run of slot s.  Beyond (a global-memory
 // sort, rare): keys sorted in place in the bucket and each of the 8 lanes walks all of them,
 // taking those of its slot.
@@ -913,7 +1000,7 @@ __global__ void __launch_bounds__(kThreads)
 	if (count > cap)
 		count = cap;
 	for (uint64_t item = blockIdx.x; item < count; item += gridDim.x) {
-		const uint32_t j = list[2u + cap + item];
+		const uint32_t j = list[3u + cap + item];
 		const uint32_t n = cnt[j];
 		const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
 		const LeafTarget t = leaf_target(segs[s], bbeg, s, j);
@@ -966,6 +1053,7 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void build_init_lists(uint32_t *list) {
 	list[0] = 0u;
 	list[1] = 0u;
+	list[2] = 0u;
 }
 
 // ---- scan plumbing for the bin offsets ---------------------------------------------------------
@@ -1096,6 +1184,10 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket,
 	                (const uint32_t *)bb.list, cap)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold_mid");
+	if ((e = launch(ctx, "build_fold_wide", build_fold_wide, dim3((uint32_t)ctx->cu_count * 8u), dim3(kThreads), 0,
+	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket,
+	                (const uint32_t *)bb.list, cap)) != hipSuccess)
+		return hip_check(ctx, e, "build_fold_wide");
 	if ((e = launch(ctx, "build_fold_big", build_fold_big, dim3((uint32_t)ctx->cu_count * 2u), dim3(kThreads), 0,
 	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, bucket,
 	                (const uint32_t *)bb.list, cap)) != hipSuccess)
@@ -1175,7 +1267,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW / 8u + 1)) != hipSuccess ||
 	    pool->alloc((void **)&bb.node, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
 	    pool->alloc((void **)&bb.bucket, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
-	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (2 + 2 * cap)) != hipSuccess)
+	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (3 + 3 * cap)) != hipSuccess)
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	if (!rc)
 		rc = upload_table(ctx, bb.segs, bb.h_segs.data(), sizeof(BuildSeg) * S);

@@ -102,14 +102,15 @@ def test_build_repeatable(ctx, no_device_errors):
 
 
 def test_build_bin_size_classes(ctx, no_device_errors):
-    # leaf blocks holding 1..16 (thread sorter), 17..64 (wave sorter), 65..300 (block sorter)
-    # fragments, spread over the 8 leaves with repeated coordinates and repeated full fragments
+    # leaf blocks holding 1..16 (thread sorter), 17..64 (wave shuffle sorter), 65..512 (wave LDS
+    # sorter) and 513..1500 (block LDS sorter) fragments, spread over the 8 leaves with repeated
+    # coordinates and repeated full fragments
     L = 7
     rng = np.random.default_rng(77)
     nodes = rng.choice(1 << (3 * (L - 1)), size=240, replace=False)
     parts = []
     for i, P in enumerate(nodes):
-        n = [rng.integers(1, 17), rng.integers(17, 65), rng.integers(65, 301)][i % 3]
+        n = [rng.integers(1, 17), rng.integers(17, 65), rng.integers(65, 513), rng.integers(513, 1501)][i % 4]
         X = Y = Z = 0
         for b in range(L - 1):  # de-interleave the Morton index of the depth-(L-1) node
             X |= ((int(P) >> (3 * b)) & 1) << b
