@@ -233,16 +233,20 @@ __global__ void __launch_bounds__(kThreads)
 		with_seg<kThreads>(dr, S, hint, base, i, [&](uint32_t s) {
 			const BuildSeg &g = segs[s];
 			const uint64_t w = i - dr[s];
-			const uint32_t *Bc = B + g.bm + g.off[d + 1u];
-			const uint64_t nwc = g.off[d + 2u] - g.off[d + 1u];
+			// the children of word w's 32 nodes: the 8-word chunk 8w of B[d+1] (B[d+1] is padded to
+			// whole chunks; a padding word of a small B[d] may have no chunk)
 			uint32_t bits = 0u;
+			if (8u * w < g.off[d + 2u] - g.off[d + 1u]) {
+				const uint4 *c = reinterpret_cast<const uint4 *>(B + g.bm + g.off[d + 1u] + 8u * w);
+				const uint4 a = c[0], b = c[1];
+				const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
 #pragma unroll
-			for (uint32_t k = 0; k < 8u; ++k) {
-				const uint64_t cw = 8u * w + k;
-				const uint32_t v = cw < nwc ? Bc[cw] : 0u;
-#pragma unroll
-				for (uint32_t b = 0; b < 4u; ++b)
-					bits |= (((v >> (8u * b)) & 0xffu) != 0u ? 1u : 0u) << (4u * k + b);
+				for (uint32_t k = 0; k < 8u; ++k) {
+					uint32_t x = v[k] | (v[k] >> 4u); // bit 8q of x: byte q of v[k] is nonzero
+					x |= x >> 2u;
+					x |= x >> 1u;
+					bits |= ((x & 1u) | ((x >> 7u) & 2u) | ((x >> 14u) & 4u) | ((x >> 21u) & 8u)) << (4u * k);
+				}
 			}
 			B[g.bm + g.off[d] + w] = bits;
 		});
