@@ -255,6 +255,7 @@ __global__ void __launch_bounds__(kThreads)
 
 // Set bits of 8-word chunk i.
 struct PopSrc {
+	static constexpr int kItems = 4;
 	const uint4 *B;
 	uint64_t n; // chunks
 	__device__ uint64_t size() const { return n; }
@@ -1062,6 +1063,7 @@ __global__ void build_init_lists(uint32_t *list) {
 
 // ---- scan plumbing for the bin offsets ---------------------------------------------------------
 struct BinCounts {
+	static constexpr int kItems = 4;
 	const uint32_t *cnt;
 	uint64_t n;
 	__device__ uint64_t size() const { return n; }

@@ -5,7 +5,8 @@
 //   scan_parts<Commit> 1 block: exclusive scan of parts -> parts[kScanBlocks + b], commit(total)
 //   scan_down<Src,Dst> kScanBlocks blocks: re-reads the chunk, block-local scan + chunk offset,
 //                      calls dst(i, exclusive_prefix, value) for every item
-// Src: __device__ uint64_t size() const; __device__ uint32_t operator()(uint64_t i) const
+// Src: __device__ uint64_t size() const; __device__ uint32_t operator()(uint64_t i) const;
+// static constexpr int kItems: consecutive items per thread and tile of scan_down
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -16,12 +17,12 @@
 
 namespace svo {
 
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanThreads * kScanItems;
+template <class Src> constexpr uint64_t scan_tile() { return (uint64_t)kScanThreads * Src::kItems; }
 
-__device__ __forceinline__ void scan_chunk(uint64_t n, uint32_t b, uint64_t &lo, uint64_t &hi) {
+// Block b's chunk of the n items: whole tiles of tile items.
+__device__ __forceinline__ void scan_chunk(uint64_t n, uint32_t b, uint64_t tile, uint64_t &lo, uint64_t &hi) {
 	uint64_t per = (n + kScanBlocks - 1) / kScanBlocks;
-	per = (per + kScanTile - 1) / kScanTile * kScanTile;
+	per = (per + tile - 1) / tile * tile;
 	lo = b * per;
 	lo = lo < n ? lo : n;
 	hi = lo + per;
@@ -69,9 +70,20 @@ __device__ __forceinline__ uint64_t block_excl_scan(uint64_t v, uint64_t &excl) 
 
 template <class Src> __global__ void __launch_bounds__(kScanThreads) scan_reduce(Src src, uint64_t *parts) {
 	uint64_t lo, hi;
-	scan_chunk(src.size(), blockIdx.x, lo, hi);
+	scan_chunk(src.size(), blockIdx.x, scan_tile<Src>(), lo, hi);
 	uint64_t acc = 0;
-	for (uint64_t i = lo + threadIdx.x; i < hi; i += kScanThreads)
+	uint64_t i = lo + threadIdx.x;
+	// 8 independent loads in flight per thread
+	for (; i + 7u * kScanThreads < hi; i += 8u * kScanThreads) {
+		uint32_t v[8];
+#pragma unroll
+		for (int k = 0; k < 8; ++k)
+			v[k] = src(i + (uint64_t)k * kScanThreads);
+#pragma unroll
+		for (int k = 0; k < 8; ++k)
+			acc += v[k];
+	}
+	for (; i < hi; i += kScanThreads)
 		acc += src(i);
 	acc = wave_sum(acc);
 	__shared__ uint64_t s[kScanThreads / 64];
@@ -108,18 +120,19 @@ template <class Commit> __global__ void __launch_bounds__(1024) scan_parts(uint6
 
 template <class Src, class Dst>
 __global__ void __launch_bounds__(kScanThreads) scan_down(Src src, Dst dst, const uint64_t *parts) {
+	constexpr int kItems = Src::kItems;
 	uint64_t lo, hi;
-	scan_chunk(src.size(), blockIdx.x, lo, hi);
+	scan_chunk(src.size(), blockIdx.x, scan_tile<Src>(), lo, hi);
 	if (lo >= hi || !dst.active())
 		return;
 	uint64_t base = parts[kScanBlocks + blockIdx.x];
-	for (uint64_t t0 = lo; t0 < hi; t0 += kScanTile) {
-		// each thread owns kScanItems consecutive items of the tile
-		uint32_t v[kScanItems];
-		uint64_t first = t0 + (uint64_t)threadIdx.x * kScanItems;
+	for (uint64_t t0 = lo; t0 < hi; t0 += scan_tile<Src>()) {
+		// each thread owns kItems consecutive items of the tile
+		uint32_t v[kItems];
+		uint64_t first = t0 + (uint64_t)threadIdx.x * kItems;
 		uint64_t sum = 0;
 #pragma unroll
-		for (int k = 0; k < kScanItems; ++k) {
+		for (int k = 0; k < kItems; ++k) {
 			uint64_t i = first + k;
 			v[k] = i < hi ? src(i) : 0u;
 			sum += v[k];
@@ -128,7 +141,7 @@ __global__ void __launch_bounds__(kScanThreads) scan_down(Src src, Dst dst, cons
 		uint64_t total = block_excl_scan(sum, excl);
 		uint64_t run = base + excl;
 #pragma unroll
-		for (int k = 0; k < kScanItems; ++k) {
+		for (int k = 0; k < kItems; ++k) {
 			uint64_t i = first + k;
 			if (i < hi)
 				dst(i, run, v[k]);

@@ -357,6 +357,7 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 struct CountSrc {
+	static constexpr int kItems = 4;
 	const uint64_t *counts;
 	uint64_t n;
 	__device__ uint64_t size() const { return n; }
