@@ -546,6 +546,25 @@ def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1
     }
 
 
+def copy_gbs(S, ctx, mib=1024, reps=5):
+    """Achievable HBM bandwidth of a plain device copy of mib MiB (read + write bytes / time)."""
+    from sparsevoxeloctree_amd._capi import lib
+
+    n = mib << 20
+    a, b = ctx.alloc(n), ctx.alloc(n)
+    ctx.check(lib.svo_copy_device(ctx.h, b.ptr, a.ptr, n))  # warm-up
+    ctx.synchronize()
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(reps):
+        ctx.check(lib.svo_copy_device(ctx.h, b.ptr, a.ptr, n))
+    ctx.synchronize()
+    ctx.prof_enable(False)
+    cnt, ms = ctx.prof_stats()["copy_u128"]
+    del a, b
+    return 2 * n / (ms / cnt * 1e-3) / 1e9
+
+
 def copy_calibration(mib):
     import sparsevoxeloctree_amd as S
     from sparsevoxeloctree_amd._capi import lib
@@ -670,6 +689,12 @@ def main():
     kernel_ms = {k: v[1] / max(v[0], 1) for k, v in prof.items()}
     kernel_total = {k: v[1] for k, v in prof.items()}
     dominant = max(kernel_total, key=kernel_total.get)
+    copy = copy_gbs(S, ctx)
+    # algorithmic GB/s of every kernel with a byte formula, against the peak and the measured copy
+    kernel_gbs = {k: {"gbs": round(alg[k] / (kernel_ms[k] * 1e-3) / 1e9, 1),
+                      "frac_peak": round(alg[k] / (kernel_ms[k] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                      "frac_copy": round(alg[k] / (kernel_ms[k] * 1e-3) / 1e9 / copy, 4)}
+                  for k in alg if k in kernel_ms and kernel_ms[k] > 0}
     roofline = None
     if dominant in alg:
         achieved = alg[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
@@ -688,6 +713,8 @@ def main():
             "traffic": traffic,
             "alg_bytes_per_launch": alg[dominant],
             "avg_launch_ms": round(kernel_ms[dominant], 5),
+            "copy_gbs": round(copy, 1),
+            "frac_of_copy": round(achieved / copy, 5),
         }
     build_bytes = 8 * F * L + 12 * W  # B_build
     vox_bytes = 40 * T + 8 * F  # B_vox
@@ -724,6 +751,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in st.items()},
         "kernel_ms": {k: round(v, 5) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernel_total[kv[0]])},
         "roofline": roofline,
+        "kernel_gbs": kernel_gbs,
     }
     if cfg["bounce"]:
         line["bounce_rays"] = nb

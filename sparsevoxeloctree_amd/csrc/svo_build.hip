@@ -1078,10 +1078,12 @@ struct BinOffsets {
 	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { aux[i] = (uint32_t)k; }
 };
 
-// Blocks for a grid-stride loop over items (kThreads per block and trip), at most 16 per CU.
-uint32_t grid_for(uint64_t items, svo_ctx *ctx) {
+// Blocks for a grid-stride loop over items (kThreads per block and trip), at most per_cu per CU:
+// more blocks than the CU holds at once keep more loads in flight (64 per CU measured best for
+// most passes; kernels that set up an LDS table per block want fewer, pure streams more).
+uint32_t grid_for(uint64_t items, svo_ctx *ctx, uint32_t per_cu = 64u) {
 	uint64_t g = (items + kThreads - 1) / kThreads;
-	const uint64_t cap = (uint64_t)ctx->cu_count * 16u;
+	const uint64_t cap = (uint64_t)ctx->cu_count * per_cu;
 	if (g > cap)
 		g = cap;
 	return (uint32_t)(g ? g : 1);
@@ -1101,7 +1103,7 @@ void levels_for(uint32_t L, uint64_t *off) {
 }
 
 // Blocks for the per-fragment passes (kU fragments per thread and trip).
-uint32_t grid_tiles(uint64_t F, svo_ctx *ctx) { return grid_for(div_up(F, kU), ctx); }
+uint32_t grid_tiles(uint64_t F, svo_ctx *ctx, uint32_t per_cu = 64u) { return grid_for(div_up(F, kU), ctx, per_cu); }
 
 uint64_t big_capacity(uint64_t F) { return F / (kRegBucket + 1u) + 1u; }
 
@@ -1176,8 +1178,9 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	                BinOffsets{bb.aux}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 		return hip_check(ctx, e, "bin_offsets");
 	uint2 *bucket = reinterpret_cast<uint2 *>(bb.bucket);
-	if (bb.Ftot > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(gF), dim3(kThreads), 0, segs, bb.fbeg(),
-	                               S, bb.Ftot, (const uint2 *)node, (const uint32_t *)bb.aux, bucket)) != hipSuccess)
+	if (bb.Ftot > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(grid_tiles(bb.Ftot, ctx, 256u)),
+	                               dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint2 *)node,
+	                               (const uint32_t *)bb.aux, bucket)) != hipSuccess)
 		return hip_check(ctx, e, "build_scatter");
 	if ((e = launch(ctx, "list_reset", build_init_lists, dim3(1), dim3(1), 0, bb.list)) != hipSuccess)
 		return hip_check(ctx, e, "list_reset");

@@ -442,7 +442,7 @@ __global__ void __launch_bounds__(kThreads) norm_apply(float *xyz, uint64_t nver
 
 uint32_t grid_for(uint64_t items, const svo_ctx *ctx) {
 	uint64_t g = (items + kThreads - 1) / kThreads;
-	uint64_t cap = (uint64_t)ctx->cu_count * 16u;
+	uint64_t cap = (uint64_t)ctx->cu_count * 64u; // 64 blocks per CU: more loads in flight than the CU holds at once
 	if (g > cap)
 		g = cap;
 	return (uint32_t)(g ? g : 1);
