@@ -147,6 +147,112 @@ __device__ __forceinline__ void row_span(const Setup &s, int32_t j, int32_t &i0,
 	}
 }
 
+// Exact floor(a / b) for b > 0 and |a| < 2^53: the double quotient is within one of it, and the
+// integer remainder corrects it.
+__device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
+	int64_t q = (int64_t)floor((double)a / (double)b);
+	const int64_t r = a - q * b;
+	q -= r < 0 ? 1 : 0;
+	q += r >= b ? 1 : 0;
+	return q;
+}
+
+// floor(X_j / p) for X_j = X_0 + j * s, kept as X_j = q p + r (0 <= r < p < 2^31) and advanced one
+// step at a time with integer adds only.
+struct FloorTrack {
+	int64_t q, dq;
+	uint32_t r, dr, p;
+	__device__ void init(int64_t X, int64_t step, uint32_t d) {
+		p = d;
+		q = floor_div(X, d);
+		r = (uint32_t)(X - q * (int64_t)d);
+		dq = floor_div(step, d);
+		dr = (uint32_t)(step - dq * (int64_t)d);
+	}
+	__device__ __forceinline__ void next() {
+		q += dq;
+		r += dr;
+		if (r >= p) {
+			r -= p;
+			++q;
+		}
+	}
+};
+
+// The covered spans of consecutive rows, as row_span computes them, with no division per row.  An
+// edge with sx = 0 constrains only which rows are covered, once per triangle (rows() narrows the
+// triangle's row range).  For the others, with p = |sx| and U_j, V_j the constraint's two sides
+// mirrored to a positive divisor (both change by the same step per row), row j's span is
+// [max(ilo, ceil(U_j / p)), min(ihi, floor(V_j / p))], and ceil(U / p) = -floor(-U / p).
+struct RowSpans {
+	FloorTrack lo[3], hi[3];
+	bool on[3];
+	int32_t ilo, ihi;
+	// rows [j0, j1] that every sx = 0 edge admits, within the triangle's [jlo, jhi]; then the
+	// trackers start at row j0
+	__device__ void init(const Setup &s, int32_t &j0, int32_t &j1) {
+		ilo = s.ilo;
+		ihi = s.ihi;
+		int64_t jl = s.jlo, jh = s.jhi;
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			const Edge &e = s.e[k];
+			on[k] = e.sx != 0;
+			if (e.sx != 0)
+				continue;
+			// covered iff lo <= sy j <= hi
+			if (e.sy == 0) {
+				if (e.lo > 0 || e.hi < 0)
+					jh = jl - 1;
+			} else if (e.sy > 0) {
+				jl = imax64(jl, -floor_div(-e.lo, e.sy));
+				jh = imin64(jh, floor_div(e.hi, e.sy));
+			} else {
+				jl = imax64(jl, -floor_div(e.hi, -e.sy));
+				jh = imin64(jh, floor_div(-e.lo, -e.sy));
+			}
+		}
+		j0 = (int32_t)jl;
+		j1 = (int32_t)imax64(jh, jl - 1);
+#pragma unroll
+		for (int k = 0; k < 3; ++k) {
+			const Edge &e = s.e[k];
+			if (!on[k])
+				continue;
+			const int64_t nlo = e.lo - e.sy * (int64_t)j0, nhi = e.hi - e.sy * (int64_t)j0;
+			const uint32_t pk = (uint32_t)(e.sx > 0 ? e.sx : -e.sx);
+			const int64_t step = e.sx > 0 ? -e.sy : e.sy;   // per row, of U and V
+			const int64_t U = e.sx > 0 ? nlo : -nhi, V = e.sx > 0 ? nhi : -nlo;
+			lo[k].init(-U, -step, pk); // floor(-U_j / p)
+			hi[k].init(V, step, pk);   // floor(V_j / p)
+		}
+	}
+	__device__ __forceinline__ void span(int32_t &i0, int32_t &i1) const {
+		int64_t a = ilo, b = ihi;
+#pragma unroll
+		for (int k = 0; k < 3; ++k)
+			if (on[k]) {
+				a = imax64(a, -lo[k].q);
+				b = imin64(b, hi[k].q);
+			}
+		if (a > b) {
+			i0 = 1;
+			i1 = 0;
+		} else {
+			i0 = (int32_t)a;
+			i1 = (int32_t)b;
+		}
+	}
+	__device__ __forceinline__ void next() {
+#pragma unroll
+		for (int k = 0; k < 3; ++k)
+			if (on[k]) {
+				lo[k].next();
+				hi[k].next();
+			}
+	}
+};
+
 __global__ void vox_zero_counter(uint32_t *c) { c[0] = 0u; }
 
 // Count pass.  Triangles with more than kBigTri fragments are also appended to `big` (big[0] is
@@ -173,13 +279,18 @@ __global__ void __launch_bounds__(kThreads)
 		Setup s;
 		tri_setup(xyz, tl, 1u << level, s);
 		uint32_t c = 0;
-		if (!s.empty)
-			for (int32_t j = s.jlo; j <= s.jhi; ++j) {
+		if (!s.empty) {
+			RowSpans rs;
+			int32_t j0, j1;
+			rs.init(s, j0, j1);
+			for (int32_t j = j0; j <= j1; ++j) {
 				int32_t i0, i1;
-				row_span(s, j, i0, i1);
+				rs.span(i0, i1);
 				if (i0 <= i1)
 					c += (uint32_t)(i1 - i0 + 1);
+				rs.next();
 			}
+		}
 		counts[t] = c;
 		const uint32_t q = wave_append(&big[0], c > kBigTri);
 		if (c > kBigTri)
@@ -251,9 +362,13 @@ __global__ void __launch_bounds__(kThreads)
 		Setup s;
 		tri_setup(xyz, tl, res, s);
 		const uint32_t colour = rgb[tl] & 0x00ffffffu;
-		for (int32_t j = s.jlo; j <= s.jhi && k < kend; ++j) {
+		RowSpans rs;
+		int32_t j0, j1;
+		rs.init(s, j0, j1);
+		for (int32_t j = j0; j <= j1 && k < kend; ++j) {
 			int32_t i0, i1;
-			row_span(s, j, i0, i1);
+			rs.span(i0, i1);
+			rs.next();
 			const float dy = s.dzdy * (((float)j + 0.5f) - s.py0);
 			for (int32_t i = i0; i <= i1 && k < kend; ++i)
 				frags[(k++) - fb] = make_frag(s, i, j, dy, res, colour);
