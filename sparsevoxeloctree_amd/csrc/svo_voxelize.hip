@@ -157,19 +157,21 @@ __device__ __forceinline__ int64_t floor_div(int64_t a, int64_t b) {
 	return q;
 }
 
-// floor(X_j / p) for X_j = X_0 + j * s, kept as X_j = q p + r (0 <= r < p < 2^31) and advanced one
-// step at a time with integer adds only.
+// floor(X_j / p) for X_j = X_0 + j * s, kept as X_j = q p + r (0 <= r < p) and advanced one step at
+// a time with integer adds only.  p = 256 |A| and |s| = 256 |B| for snapped edge vectors (A, B)
+// below 2^21, so p < 2^29, r fits 32 bits and the per-step quotient |floor(s / p)| < 2^21.
 struct FloorTrack {
-	int64_t q, dq;
-	uint32_t r, dr, p;
-	__device__ void init(int64_t X, int64_t step, uint32_t d) {
-		p = d;
-		q = floor_div(X, d);
-		r = (uint32_t)(X - q * (int64_t)d);
-		dq = floor_div(step, d);
-		dr = (uint32_t)(step - dq * (int64_t)d);
+	int64_t q;
+	int32_t dq;
+	uint32_t r, dr;
+	__device__ void init(int64_t X, int64_t step, uint32_t p) {
+		q = floor_div(X, p);
+		r = (uint32_t)(X - q * (int64_t)p);
+		const int64_t d = floor_div(step, p);
+		dq = (int32_t)d;
+		dr = (uint32_t)(step - d * (int64_t)p);
 	}
-	__device__ __forceinline__ void next() {
+	__device__ __forceinline__ void next(uint32_t p) {
 		q += dq;
 		r += dr;
 		if (r >= p) {
@@ -186,6 +188,7 @@ struct FloorTrack {
 // [max(ilo, ceil(U_j / p)), min(ihi, floor(V_j / p))], and ceil(U / p) = -floor(-U / p).
 struct RowSpans {
 	FloorTrack lo[3], hi[3];
+	uint32_t p[3]; // |sx| per edge
 	bool on[3];
 	int32_t ilo, ihi;
 	// rows [j0, j1] that every sx = 0 edge admits, within the triangle's [jlo, jhi]; then the
@@ -221,6 +224,7 @@ struct RowSpans {
 				continue;
 			const int64_t nlo = e.lo - e.sy * (int64_t)j0, nhi = e.hi - e.sy * (int64_t)j0;
 			const uint32_t pk = (uint32_t)(e.sx > 0 ? e.sx : -e.sx);
+			p[k] = pk;
 			const int64_t step = e.sx > 0 ? -e.sy : e.sy;   // per row, of U and V
 			const int64_t U = e.sx > 0 ? nlo : -nhi, V = e.sx > 0 ? nhi : -nlo;
 			lo[k].init(-U, -step, pk); // floor(-U_j / p)
@@ -247,8 +251,8 @@ struct RowSpans {
 #pragma unroll
 		for (int k = 0; k < 3; ++k)
 			if (on[k]) {
-				lo[k].next();
-				hi[k].next();
+				lo[k].next(p[k]);
+				hi[k].next(p[k]);
 			}
 	}
 };
