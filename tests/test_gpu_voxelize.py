@@ -71,6 +71,34 @@ def test_voxelize_level12_random_triangles(ctx, no_device_errors):
     assert np.array_equal(fr, oracle.voxelize(xyz, rgb, 12))
 
 
+def test_voxelize_row_spans_axis_parallel_and_long(ctx, no_device_errors):
+    # the per-row span trackers: right triangles whose legs are parallel to the rows or columns of
+    # the projection (edges that only narrow the row range), and thin slivers crossing hundreds of
+    # rows at level 12 (many tracker steps, large edge constants)
+    rng = np.random.default_rng(31)
+    tris = []
+    for _ in range(1500):
+        o = rng.uniform(-0.9, 0.9, 3)
+        a, b = rng.uniform(0.001, 0.02, 2)
+        ax = rng.permutation(3)
+        p1, p2 = o.copy(), o.copy()
+        p1[ax[0]] += a
+        p2[ax[1]] += b
+        tris.append(np.concatenate([o, p1, p2]))
+    for _ in range(1500):
+        o = rng.uniform(-0.8, 0.8, 3)
+        d = rng.normal(size=3)
+        d *= rng.uniform(0.05, 0.3) / np.linalg.norm(d)
+        w = rng.normal(size=3) * 2e-4
+        tris.append(np.concatenate([o, o + d, o + d * 0.5 + w]))
+    xyz = np.clip(np.array(tris), -1.0, 1.0).astype(np.float32)
+    rgb = rng.integers(0, 1 << 24, xyz.shape[0]).astype(np.uint32)
+    for level in (10, 12):
+        print("row spans level=%d T=%d" % (level, xyz.shape[0]), flush=True)
+        _, _, fr = gpu_voxelize(ctx, xyz, rgb, level, normalize=False)
+        assert np.array_equal(fr, oracle.voxelize(xyz, rgb, level))
+
+
 def test_voxelize_degenerate_and_empty(ctx, no_device_errors):
     xyz = np.zeros((3, 9), np.float32)
     xyz[1] = [0.1, 0.1, 0.1, 0.1, 0.1, 0.1, 0.5, 0.5, 0.5]  # collinear -> n == 0
