@@ -11,7 +11,8 @@
 //   octree_tag_node.comp      -> bm_mark: each fragment sets the bit of its depth-(L-1) node in
 //   (levels 1..L-1)              bitmap B[L-1]; bm_reduce ORs each byte of B[d+1] (8 children) into
 //                                one bit of B[d], d = L-2..1
-//   octree_alloc_node.comp    -> one exclusive popcount scan over all bitmaps in 8-word chunks
+//   octree_alloc_node.comp    -> one exclusive scan over the set-bit counts of all bitmaps' 8-word
+//                                chunks (stored by bm_reduce while it reads them)
 //                                (rank of every occupied node among its depth = its block among
 //                                the allocated children blocks); child pointer = 8 * (first block
 //                                of depth + rank)
@@ -220,10 +221,12 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Bit q of B[d] = OR of byte q of B[d+1] (node q occupied iff one of its 8 children is); items are
-// the words of B[d] of the scenes that have B[d+1] (begin table dr).
+// the words of B[d] of the scenes that have B[d+1] (begin table dr).  Word w of B[d] is made from
+// chunk w of B[d+1], so the same thread also stores that chunk's set-bit count into R, which the
+// rank scan then turns into ranks in place (the bitmaps are not read again for it).
 __global__ void __launch_bounds__(kThreads)
     bm_reduce(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dr, uint32_t S, uint64_t n, uint32_t d,
-              uint32_t *B) {
+              uint32_t *B, uint32_t *R) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
@@ -240,30 +243,40 @@ __global__ void __launch_bounds__(kThreads)
 				const uint4 *c = reinterpret_cast<const uint4 *>(B + g.bm + g.off[d + 1u] + 8u * w);
 				const uint4 a = c[0], b = c[1];
 				const uint32_t v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+				uint32_t pop = 0u;
 #pragma unroll
 				for (uint32_t k = 0; k < 8u; ++k) {
 					uint32_t x = v[k] | (v[k] >> 4u); // bit 8q of x: byte q of v[k] is nonzero
 					x |= x >> 2u;
 					x |= x >> 1u;
 					bits |= ((x & 1u) | ((x >> 7u) & 2u) | ((x >> 14u) & 4u) | ((x >> 21u) & 8u)) << (4u * k);
+					pop += (uint32_t)__popc(v[k]);
 				}
+				R[((g.bm + g.off[d + 1u]) >> 3) + w] = pop;
 			}
 			B[g.bm + g.off[d] + w] = bits;
 		});
 	}
 }
 
-// Set bits of 8-word chunk i.
-struct PopSrc {
+// Set-bit count of the one chunk of B[1] (8 nodes in word 0, padding words zero) of every scene that
+// has bitmaps; bm_reduce stores the counts of all deeper chunks.
+__global__ void rank_top(const BuildSeg *__restrict__ segs, uint32_t S, const uint32_t *__restrict__ B, uint32_t *R) {
+	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S)
+		return;
+	const BuildSeg &g = segs[s];
+	if (g.L > 1u)
+		R[g.bm >> 3] = (uint32_t)__popc(B[g.bm]);
+}
+
+// Set bits of 8-word chunk i, as stored by rank_top and bm_reduce.
+struct ChunkCounts {
 	static constexpr int kItems = 4;
-	const uint4 *B;
+	const uint32_t *R;
 	uint64_t n; // chunks
 	__device__ uint64_t size() const { return n; }
-	__device__ uint32_t operator()(uint64_t i) const {
-		const uint4 a = B[2u * i], b = B[2u * i + 1u];
-		return (uint32_t)(__popc(a.x) + __popc(a.y) + __popc(a.z) + __popc(a.w) + __popc(b.x) + __popc(b.y) +
-		                  __popc(b.z) + __popc(b.w));
-	}
+	__device__ uint32_t operator()(uint64_t i) const { return R[i]; }
 };
 struct RankTotal {
 	uint32_t *R;
@@ -1122,18 +1135,22 @@ int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
 			const uint64_t n = bb.h_index[(3 + kMaxLevel + d) * (uint64_t)(S + 1) + S];
 			if (n > 0 && (e = launch(ctx, "bm_reduce", bm_reduce, dim3(grid_for(n, ctx)), dim3(kThreads), 0,
 			                         (const BuildSeg *)bb.segs, bb.index + (3 + kMaxLevel + d) * (uint64_t)(S + 1), S, n,
-			                         d, bb.bm)) != hipSuccess)
+			                         d, bb.bm, bb.rk)) != hipSuccess)
 				return hip_check(ctx, e, "bm_reduce");
 		}
-		PopSrc src{reinterpret_cast<const uint4 *>(bb.bm), bb.TW / 8u};
-		if ((e = launch(ctx, "rank_reduce", scan_reduce<PopSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
+		if ((e = launch(ctx, "rank_top", rank_top, dim3(div_up(S, 64)), dim3(64), 0, (const BuildSeg *)bb.segs, S,
+		                (const uint32_t *)bb.bm, bb.rk)) != hipSuccess)
+			return hip_check(ctx, e, "rank_top");
+		// exclusive scan of the chunk counts, in place (each item is read and written by one thread)
+		ChunkCounts src{bb.rk, bb.TW / 8u};
+		if ((e = launch(ctx, "rank_reduce", scan_reduce<ChunkCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_reduce");
 		if ((e = launch(ctx, "rank_parts", scan_parts<RankTotal>, dim3(1), dim3(1024), 0, ctx->d_scan_parts,
 		                RankTotal{bb.rk, bb.TW / 8u})) != hipSuccess)
 			return hip_check(ctx, e, "rank_parts");
-		if ((e = launch(ctx, "rank_down", scan_down<PopSrc, RankDst>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
-		                RankDst{bb.rk}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
+		if ((e = launch(ctx, "rank_down", scan_down<ChunkCounts, RankDst>, dim3(kScanBlocks), dim3(kScanThreads), 0,
+		                src, RankDst{bb.rk}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_down");
 	}
 	if ((e = launch(ctx, "bm_levels", bm_levels, dim3(div_up(S, 64)), dim3(64), 0, bb.segs, S,
