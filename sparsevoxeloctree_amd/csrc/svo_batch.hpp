@@ -133,7 +133,8 @@ struct BuildBatch {
 	uint64_t Ftot = 0, TW = 0, nbins = 0;
 	std::vector<BuildSeg> h_segs;
 	// begin tables, S + 1 entries each: fbeg (fragments), bbeg (leaf blocks), dw[d] at 2 + d (words of
-	// B[d] of the scenes with L > d), dr[d] at 3 + kMaxLevel + d (same, scenes with L > d + 1)
+	// B[d] of the scenes with L > d), dr[d] at 3 + kMaxLevel + d (same, scenes with L > d + 1); then
+	// pw (pw_at): the block writer's segments
 	std::vector<uint64_t> h_index;
 	BuildSeg *segs = nullptr;
 	uint64_t *index = nullptr;
@@ -148,6 +149,10 @@ struct BuildBatch {
 	uint64_t h_dw(uint32_t d, uint32_t s) const { return h_index[(2 + d) * (uint64_t)(S + 1) + s]; }
 	const uint64_t *dr(uint32_t d) const { return index + (3 + kMaxLevel + d) * (uint64_t)(S + 1); }
 	uint64_t h_dr(uint32_t d, uint32_t s) const { return h_index[(3 + kMaxLevel + d) * (uint64_t)(S + 1) + s]; }
+	// block writer's parent words, after the per-depth tables: (kMaxLevel - 2) S segments + 1 entries
+	uint64_t pw_at() const { return (3 + 2 * (uint64_t)kMaxLevel + 1) * (S + 1); }
+	uint32_t pw_segs() const { return (kMaxLevel - 2u) * S; }
+	const uint64_t *pw() const { return index + pw_at(); }
 };
 
 void vox_batch_release(svo_ctx *ctx, VoxBatch *vb);

@@ -259,15 +259,56 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
-// Set-bit count of the one chunk of B[1] (8 nodes in word 0, padding words zero) of every scene that
-// has bitmaps; bm_reduce stores the counts of all deeper chunks.
-__global__ void rank_top(const BuildSeg *__restrict__ segs, uint32_t S, const uint32_t *__restrict__ B, uint32_t *R) {
-	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= S)
-		return;
-	const BuildSeg &g = segs[s];
-	if (g.L > 1u)
-		R[g.bm >> 3] = (uint32_t)__popc(B[g.bm]);
+// The small depths d = min(L-2, kTopDepth) .. 1 of one scene per block: what one bm_reduce launch
+// per depth does, with B[d] (at most 8^kTopDepth / 32 = 1024 words) kept in LDS from one depth to
+// the next.  Also stores the set-bit count of B[1]'s one chunk (word 0; the padding words are zero)
+// for every scene with L > 1.
+constexpr uint32_t kTopDepth = 5;
+constexpr uint32_t kTopWords = 1u << (3u * kTopDepth - 5u);
+
+__global__ void __launch_bounds__(kThreads) bm_reduce_top(const BuildSeg *__restrict__ segs, uint32_t *B, uint32_t *R) {
+	__shared__ uint32_t s_b[2][kTopWords]; // B[d+1] of the previous depth, B[d]
+	const BuildSeg &g = segs[blockIdx.x];
+	const uint32_t L = g.L;
+	if (L <= 1u)
+		return; // no bitmaps
+	const uint32_t d0 = L - 2u < kTopDepth ? L - 2u : kTopDepth;
+	uint32_t cur = 0u; // s_b[cur] holds B[d+1] below depth d0
+	for (uint32_t d = d0; d >= 1u; --d) {
+		const uint32_t nw = (uint32_t)(g.off[d + 1u] - g.off[d]);      // words of B[d], padded
+		const uint32_t nc = (uint32_t)(g.off[d + 2u] - g.off[d + 1u]); // words of B[d+1], padded
+		for (uint32_t w = threadIdx.x; w < nw; w += kThreads) {
+			uint32_t bits = 0u;
+			if (8u * w < nc) {
+				uint32_t v[8];
+				if (d == d0) {
+					const uint4 *c = reinterpret_cast<const uint4 *>(B + g.bm + g.off[d + 1u] + 8u * w);
+					const uint4 a = c[0], b = c[1];
+					v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+				} else {
+#pragma unroll
+					for (uint32_t k = 0; k < 8u; ++k)
+						v[k] = s_b[cur][8u * w + k];
+				}
+				uint32_t pop = 0u;
+#pragma unroll
+				for (uint32_t k = 0; k < 8u; ++k) {
+					uint32_t x = v[k] | (v[k] >> 4u); // bit 8q of x: byte q of v[k] is nonzero
+					x |= x >> 2u;
+					x |= x >> 1u;
+					bits |= ((x & 1u) | ((x >> 7u) & 2u) | ((x >> 14u) & 4u) | ((x >> 21u) & 8u)) << (4u * k);
+					pop += (uint32_t)__popc(v[k]);
+				}
+				R[((g.bm + g.off[d + 1u]) >> 3) + w] = pop;
+			}
+			B[g.bm + g.off[d] + w] = bits;
+			s_b[cur ^ 1u][w] = bits;
+		}
+		__syncthreads();
+		cur ^= 1u;
+	}
+	if (threadIdx.x == 0u)
+		R[g.bm >> 3] = (uint32_t)__popc(d0 >= 1u ? s_b[cur][0] : B[g.bm]);
 }
 
 // Set bits of 8-word chunk i, as stored by rank_top and bm_reduce.
@@ -313,28 +354,6 @@ __global__ void bm_levels(BuildSeg *segs, uint32_t S, const uint32_t *__restrict
 	g.nbins = nlast;
 }
 
-// The root block of every scene with L > 1 (one thread per scene): its 8 words point to the depth-1
-// blocks of the occupied children (bits of B[1]) or are 0.
-__global__ void bm_write_root(const BuildSeg *__restrict__ segs, uint32_t S, const uint32_t *__restrict__ B) {
-	const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= S)
-		return;
-	const BuildSeg &g = segs[s];
-	if (g.L <= 1u)
-		return; // the root block holds the leaves (build_fold)
-	const uint32_t byte = B[g.bm + g.off[1]] & 0xffu, fbd = g.fb[1];
-	uint32_t wd[8], r = 0u;
-#pragma unroll
-	for (uint32_t t = 0; t < 8u; ++t) {
-		const bool set = (byte >> t) & 1u;
-		wd[t] = set ? (0x80000000u | ((fbd + r) << 3u)) : 0u;
-		r += set ? 1u : 0u;
-	}
-	uint4 *dst = reinterpret_cast<uint4 *>(g.words);
-	dst[0] = make_uint4(wd[0], wd[1], wd[2], wd[3]);
-	dst[1] = make_uint4(wd[4], wd[5], wd[6], wd[7]);
-}
-
 // The 8 words of the block of one depth-(d-1) parent: pointers to the blocks of its occupied
 // children (bits of its byte in B[d]), zeros otherwise; r = rank of its first child among depth d.
 __device__ __forceinline__ void store_parent_block(uint32_t *oct, uint64_t block, uint32_t byte, uint32_t fbd,
@@ -366,32 +385,49 @@ __device__ __forceinline__ uint32_t nth_bit(uint32_t v, uint32_t n) {
 	return pos;
 }
 
-// Blocks of depth d >= 2 (the children of depth-(d-1) nodes): every occupied parent's block (the
-// parent's rank among depth d-1) is written whole.  Items are the words of B[d-1] of the scenes
-// with L > d (begin table dr[d-1]), so the scan covers the parents' bitmap, 8x smaller than the
-// children's.  A wave whose 64 words lie in one scene works cooperatively: the lanes' popcounts
-// are prefix-summed, and lane k of each round writes the wave's k-th occupied parent, so every
-// lane has work and the 64 blocks of a round are contiguous.  A wave across a scene boundary
-// walks each word's set bits in its own lane.
+// Blocks of every depth d >= 2 (the children of depth-(d-1) nodes), in one launch: every occupied
+// parent's block (the parent's rank among depth d-1) is written whole.  Items are the words of
+// B[d-1] of the scenes with L > d, depth after depth (begin table pw: segment k = (d - 2) S + s), so
+// the scan covers the parents' bitmaps, 8x smaller than the children's.  A wave whose 64 words lie
+// in one segment works cooperatively: the lanes' popcounts are prefix-summed, and lane k of each
+// round writes the wave's k-th occupied parent, so every lane has work and the 64 blocks of a round
+// are contiguous.  A wave across a segment boundary walks each word's set bits in its own lane.
+// Block 0 also writes the root block of every scene with L > 1 (its 8 words point to the depth-1
+// blocks of the occupied children, the bits of B[1]).  The same launch clears the nb leaf-block
+// counters of build_tag_leaf and the three counts of the fold's work lists.
 __global__ void __launch_bounds__(kThreads)
-    bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ dp, uint32_t S, uint64_t n,
-                    uint32_t d, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R) {
+    bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ pw, uint32_t M, uint32_t S,
+                    uint64_t n, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint32_t *cnt,
+                    uint64_t nb, uint32_t *list) {
 	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
+	for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nb; i += stride)
+		cnt[i] = 0u;
+	if (blockIdx.x == 0u) {
+		if (threadIdx.x < 3u)
+			list[threadIdx.x] = 0u;
+		for (uint32_t s = threadIdx.x; s < S; s += kThreads) {
+			const BuildSeg &g = segs[s];
+			if (g.L > 1u) // L == 1: the root block holds the leaves (build_fold)
+				store_parent_block(g.words, 0u, B[g.bm + g.off[1]] & 0xffu, g.fb[1], 0u);
+		}
+	}
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < n; base += stride) {
 		const uint64_t i = base + threadIdx.x;
-		uint32_t s = kEmpty;
+		uint32_t k = kEmpty;
 		if (i < n)
-			with_seg<kThreads>(dp, S, hint, base, i, [&](uint32_t t) { s = t; });
-		const uint32_t s0 = __shfl(s, 0, 64);
-		if (s0 == kEmpty)
+			with_seg<kThreads>(pw, M, hint, base, i, [&](uint32_t t) { k = t; });
+		const uint32_t k0 = __shfl(k, 0, 64);
+		if (k0 == kEmpty)
 			continue; // the wave is past the end (lane 0 holds its first item)
-		if (__all(s == s0 || s == kEmpty)) {
-			const BuildSeg &g = segs[__builtin_amdgcn_readfirstlane(s0)];
+		if (__all(k == k0 || k == kEmpty)) {
+			const uint32_t ku = __builtin_amdgcn_readfirstlane(k0);
+			const uint32_t d = 2u + ku / S;
+			const BuildSeg &g = segs[ku % S];
 			const uint64_t offp = g.bm + g.off[d - 1u], offc = g.bm + g.off[d]; // both start chunks
-			const uint64_t w0 = (i - lane) - dp[__builtin_amdgcn_readfirstlane(s0)]; // lane 0's word
-			const uint32_t v = s != kEmpty ? B[offp + w0 + lane] : 0u, c = (uint32_t)__popc(v);
+			const uint64_t w0 = (i - lane) - pw[ku]; // lane 0's word
+			const uint32_t v = k != kEmpty ? B[offp + w0 + lane] : 0u, c = (uint32_t)__popc(v);
 			uint32_t incl = c;
 #pragma unroll
 			for (uint32_t dd = 1; dd < 64u; dd <<= 1) {
@@ -412,29 +448,30 @@ __global__ void __launch_bounds__(kThreads)
 			// words: an empty padding word of a small B[d-1] may have no chunk)
 			const uint32_t rc = v != 0u ? R[(offc >> 3) + w0 + lane] - R[offc >> 3] : 0u;
 			for (uint32_t t0 = 0; t0 < T; t0 += 64u) {
-				const uint32_t k = t0 + lane;
-				uint32_t o = 0u; // owner lane: the last with excl <= k
+				const uint32_t j = t0 + lane; // the wave's j-th occupied parent
+				uint32_t o = 0u; // owner lane: the last with excl <= j
 #pragma unroll
 				for (uint32_t step = 32u; step >= 1u; step >>= 1) {
 					const uint32_t e = __shfl(excl, (int)(o + step), 64);
-					o = e <= k ? o + step : o;
+					o = e <= j ? o + step : o;
 				}
 				const uint32_t vo = __shfl(v, (int)o, 64), eo = __shfl(excl, (int)o, 64), ro = __shfl(rc, (int)o, 64);
-				if (k < T) {
-					const uint32_t b = nth_bit(vo, k - eo); // the parent's bit in the owner's word
+				if (j < T) {
+					const uint32_t b = nth_bit(vo, j - eo); // the parent's bit in the owner's word
 					uint32_t cv; // the owner's chunk of B[d]: its word b / 4 holds the parent's byte
 					const uint32_t before = chunk_before(B + offc + 8u * (w0 + o), b >> 2u, cv);
 					const uint32_t sh = 8u * (b & 3u);
 					const uint32_t r = ro + before + (uint32_t)__popc(cv & ((1u << sh) - 1u));
-					store_parent_block(g.words, blk0 + k, (cv >> sh) & 0xffu, fbd, r);
+					store_parent_block(g.words, blk0 + j, (cv >> sh) & 0xffu, fbd, r);
 				}
 			}
 			continue;
 		}
-		if (s == kEmpty)
+		if (k == kEmpty)
 			continue;
-		const BuildSeg &g = segs[s];
-		const uint64_t w = i - dp[s];
+		const uint32_t d = 2u + k / S;
+		const BuildSeg &g = segs[k % S];
+		const uint64_t w = i - pw[k];
 		const uint64_t offp = g.bm + g.off[d - 1u], offc = g.bm + g.off[d]; // both start chunks
 		uint32_t v;
 		const uint32_t rp = bits_before(B, R, offp + w, v);
@@ -1068,11 +1105,6 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
-__global__ void build_init_lists(uint32_t *list) {
-	list[0] = 0u;
-	list[1] = 0u;
-	list[2] = 0u;
-}
 
 // ---- scan plumbing for the bin offsets ---------------------------------------------------------
 struct BinCounts {
@@ -1081,9 +1113,6 @@ struct BinCounts {
 	uint64_t n;
 	__device__ uint64_t size() const { return n; }
 	__device__ uint32_t operator()(uint64_t i) const { return cnt[i]; }
-};
-struct NoCommit {
-	__device__ void operator()(uint64_t) const {}
 };
 struct BinOffsets {
 	uint32_t *aux;
@@ -1131,26 +1160,24 @@ int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
 		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_tiles(bb.Ftot, ctx)), dim3(kThreads), 0,
 		                (const BuildSeg *)bb.segs, bb.fbeg(), S, bb.Ftot, bb.bm)) != hipSuccess)
 			return hip_check(ctx, e, "bm_mark");
-		for (uint32_t d = bb.maxL >= 3 ? bb.maxL - 2 : 0; d >= 1; --d) {
+		for (uint32_t d = bb.maxL >= 3 ? bb.maxL - 2 : 0; d > kTopDepth; --d) {
 			const uint64_t n = bb.h_index[(3 + kMaxLevel + d) * (uint64_t)(S + 1) + S];
 			if (n > 0 && (e = launch(ctx, "bm_reduce", bm_reduce, dim3(grid_for(n, ctx)), dim3(kThreads), 0,
 			                         (const BuildSeg *)bb.segs, bb.index + (3 + kMaxLevel + d) * (uint64_t)(S + 1), S, n,
 			                         d, bb.bm, bb.rk)) != hipSuccess)
 				return hip_check(ctx, e, "bm_reduce");
 		}
-		if ((e = launch(ctx, "rank_top", rank_top, dim3(div_up(S, 64)), dim3(64), 0, (const BuildSeg *)bb.segs, S,
-		                (const uint32_t *)bb.bm, bb.rk)) != hipSuccess)
-			return hip_check(ctx, e, "rank_top");
+		if ((e = launch(ctx, "bm_reduce_top", bm_reduce_top, dim3(S), dim3(kThreads), 0, (const BuildSeg *)bb.segs,
+		                bb.bm, bb.rk)) != hipSuccess)
+			return hip_check(ctx, e, "bm_reduce_top");
 		// exclusive scan of the chunk counts, in place (each item is read and written by one thread)
 		ChunkCounts src{bb.rk, bb.TW / 8u};
 		if ((e = launch(ctx, "rank_reduce", scan_reduce<ChunkCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_reduce");
-		if ((e = launch(ctx, "rank_parts", scan_parts<RankTotal>, dim3(1), dim3(1024), 0, ctx->d_scan_parts,
+		if ((e = launch(ctx, "rank_down", scan_down<ChunkCounts, RankDst, RankTotal>, dim3(kScanBlocks),
+		                dim3(kScanThreads), 0, src, RankDst{bb.rk}, (const uint64_t *)ctx->d_scan_parts,
 		                RankTotal{bb.rk, bb.TW / 8u})) != hipSuccess)
-			return hip_check(ctx, e, "rank_parts");
-		if ((e = launch(ctx, "rank_down", scan_down<ChunkCounts, RankDst>, dim3(kScanBlocks), dim3(kScanThreads), 0,
-		                src, RankDst{bb.rk}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "rank_down");
 	}
 	if ((e = launch(ctx, "bm_levels", bm_levels, dim3(div_up(S, 64)), dim3(64), 0, bb.segs, S,
@@ -1163,20 +1190,15 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	const uint32_t S = bb.S;
 	const BuildSeg *segs = bb.segs;
 	hipError_t e;
-	if (bb.maxL > 1u &&
-	    (e = launch(ctx, "bm_write_root", bm_write_root, dim3(div_up(S, 64)), dim3(64), 0, segs, S,
-	                (const uint32_t *)bb.bm)) != hipSuccess)
-		return hip_check(ctx, e, "bm_write_root");
-	for (uint32_t d = 2; d + 1 <= bb.maxL; ++d) {
-		const uint64_t n = bb.h_dr(d - 1, S); // words of B[d-1], scenes with L > d
-		if (n > 0 && (e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(n, ctx)), dim3(kThreads), 0,
-		                         segs, bb.dr(d - 1), S, n, d, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk)) !=
-		                 hipSuccess)
+	{
+		// with maxL == 1 there are no parent words and no root blocks to write (the bitmaps are empty)
+		const uint32_t M = bb.pw_segs();
+		const uint64_t n = bb.maxL > 1u ? bb.h_index[bb.pw_at() + M] : 0u; // parent words of all depths >= 2
+		if ((e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(std::max(n, bb.nbins), ctx)),
+		                dim3(kThreads), 0, segs, bb.pw(), M, bb.maxL > 1u ? S : 0u, n, (const uint32_t *)bb.bm,
+		                (const uint32_t *)bb.rk, bb.cnt, bb.nbins, bb.list)) != hipSuccess)
 			return hip_check(ctx, e, "bm_write_blocks");
 	}
-	if ((e = launch(ctx, "bin_zero", bm_zero, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, bb.cnt, bb.nbins)) !=
-	    hipSuccess)
-		return hip_check(ctx, e, "bin_zero");
 	const uint32_t gF = grid_tiles(bb.Ftot, ctx);
 	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
 	if (bb.Ftot > 0 &&
@@ -1188,19 +1210,14 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	if ((e = launch(ctx, "bin_reduce", scan_reduce<BinCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
 	                ctx->d_scan_parts)) != hipSuccess)
 		return hip_check(ctx, e, "bin_reduce");
-	if ((e = launch(ctx, "bin_parts", scan_parts<NoCommit>, dim3(1), dim3(1024), 0, ctx->d_scan_parts, NoCommit{})) !=
-	    hipSuccess)
-		return hip_check(ctx, e, "bin_parts");
 	if ((e = launch(ctx, "bin_offsets", scan_down<BinCounts, BinOffsets>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
-	                BinOffsets{bb.aux}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
+	                BinOffsets{bb.aux}, (const uint64_t *)ctx->d_scan_parts, NoCommit{})) != hipSuccess)
 		return hip_check(ctx, e, "bin_offsets");
 	uint2 *bucket = reinterpret_cast<uint2 *>(bb.bucket);
 	if (bb.Ftot > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(grid_tiles(bb.Ftot, ctx, 256u)),
 	                               dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint2 *)node,
 	                               (const uint32_t *)bb.aux, bucket)) != hipSuccess)
 		return hip_check(ctx, e, "build_scatter");
-	if ((e = launch(ctx, "list_reset", build_init_lists, dim3(1), dim3(1), 0, bb.list)) != hipSuccess)
-		return hip_check(ctx, e, "list_reset");
 	const uint64_t cap = big_capacity(bb.Ftot);
 	if ((e = launch(ctx, "build_fold", build_fold, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, segs, bb.bbeg(),
 	                S, bb.nbins, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket, bb.list,
@@ -1250,7 +1267,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	bb.S = S;
 	bb.h_segs.assign(S, BuildSeg{});
 	const uint64_t stride = S + 1;
-	bb.h_index.assign((3 + 2 * (uint64_t)kMaxLevel + 1) * stride, 0);
+	bb.h_index.assign(bb.pw_at() + bb.pw_segs() + 1, 0);
 	uint64_t *fbeg = bb.h_index.data();
 	for (uint32_t s = 0; s < S; ++s) {
 		const svo_fraglist *fl = octs[s]->src;
@@ -1282,6 +1299,17 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 		}
 		dw[S] = aw;
 		dr[S] = ar;
+	}
+	// parent words of the block writer: B[d-1] of the scenes with L > d, for d = 2..kMaxLevel-1
+	{
+		uint64_t *pw = bb.h_index.data() + bb.pw_at(), a = 0;
+		for (uint32_t d = 2; d < kMaxLevel; ++d)
+			for (uint32_t s = 0; s < S; ++s) {
+				const BuildSeg &g = bb.h_segs[s];
+				*pw++ = a;
+				a += g.L > d ? g.off[d] - g.off[d - 1] : 0;
+			}
+		*pw = a;
 	}
 	const uint64_t Fa = bb.Ftot ? bb.Ftot : 1;
 	const uint64_t cap = big_capacity(bb.Ftot);

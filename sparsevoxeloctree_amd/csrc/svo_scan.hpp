@@ -1,10 +1,11 @@
 // Device-wide exclusive scan (reduce-then-scan, no inter-block waiting) over a sequence whose
 // length may live in device memory, so a scan can follow a kernel that decides its size without a
-// host round trip.  Three launches:
-//   scan_reduce<Src>   kScanBlocks blocks: block b sums its contiguous chunk -> parts[b]
-//   scan_parts<Commit> 1 block: exclusive scan of parts -> parts[kScanBlocks + b], commit(total)
-//   scan_down<Src,Dst> kScanBlocks blocks: re-reads the chunk, block-local scan + chunk offset,
-//                      calls dst(i, exclusive_prefix, value) for every item
+// host round trip.  Two launches:
+//   scan_reduce<Src>          kScanBlocks blocks: block b sums its contiguous chunk -> parts[b]
+//   scan_down<Src,Dst,Commit> kScanBlocks blocks: block b adds up parts[0..b) for its chunk's offset
+//                             (block 0 also passes the grand total to commit), re-reads the chunk,
+//                             block-local scan + chunk offset, calls dst(i, exclusive_prefix, value)
+//                             for every item
 // Src: __device__ uint64_t size() const; __device__ uint32_t operator()(uint64_t i) const;
 // static constexpr int kItems: consecutive items per thread and tile of scan_down
 #pragma once
@@ -98,34 +99,50 @@ template <class Src> __global__ void __launch_bounds__(kScanThreads) scan_reduce
 	}
 }
 
-// One block of 1024 threads (= kScanBlocks).
-template <class Commit> __global__ void __launch_bounds__(1024) scan_parts(uint64_t *parts, Commit commit) {
-	static_assert(kScanBlocks == 1024, "scan_parts assumes one part per thread");
-	__shared__ uint64_t s_wave[16];
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	uint64_t v = parts[threadIdx.x];
-	uint64_t incl = wave_incl_scan(v);
-	if (lane == 63)
-		s_wave[wave] = incl;
-	__syncthreads();
-	uint64_t off = 0, total = 0;
-	for (int w = 0; w < 16; ++w) {
-		off += (w < wave) ? s_wave[w] : 0;
-		total += s_wave[w];
+// Sum of parts[0..blockIdx.x) (the chunk totals of the blocks before this one); block 0 passes the
+// sum of all parts to commit.  Every thread of the block must call it.
+template <class Commit> __device__ __forceinline__ uint64_t chunk_offset(const uint64_t *parts, Commit &commit) {
+	static_assert(kScanBlocks % kScanThreads == 0, "parts are read kScanThreads at a time");
+	__shared__ uint64_t s_below[kScanThreads / 64], s_all[kScanThreads / 64];
+	uint64_t below = 0, all = 0;
+#pragma unroll
+	for (int j0 = 0; j0 < kScanBlocks; j0 += kScanThreads) {
+		const int j = j0 + (int)threadIdx.x;
+		const uint64_t p = parts[j];
+		below += j < (int)blockIdx.x ? p : 0;
+		all += p;
 	}
-	parts[kScanBlocks + threadIdx.x] = off + incl - v;
-	if (threadIdx.x == 0)
-		commit(total);
+	below = wave_sum(below);
+	all = wave_sum(all);
+	if ((threadIdx.x & 63) == 0) {
+		s_below[threadIdx.x >> 6] = below;
+		s_all[threadIdx.x >> 6] = all;
+	}
+	__syncthreads();
+	below = all = 0;
+#pragma unroll
+	for (int w = 0; w < kScanThreads / 64; ++w) {
+		below += s_below[w];
+		all += s_all[w];
+	}
+	if (blockIdx.x == 0 && threadIdx.x == 0)
+		commit(all);
+	return below;
 }
 
-template <class Src, class Dst>
-__global__ void __launch_bounds__(kScanThreads) scan_down(Src src, Dst dst, const uint64_t *parts) {
+struct NoCommit {
+	__device__ void operator()(uint64_t) const {}
+};
+
+template <class Src, class Dst, class Commit = NoCommit>
+__global__ void __launch_bounds__(kScanThreads)
+    scan_down(Src src, Dst dst, const uint64_t *parts, Commit commit = Commit{}) {
 	constexpr int kItems = Src::kItems;
+	uint64_t base = chunk_offset(parts, commit);
 	uint64_t lo, hi;
 	scan_chunk(src.size(), blockIdx.x, scan_tile<Src>(), lo, hi);
 	if (lo >= hi || !dst.active())
 		return;
-	uint64_t base = parts[kScanBlocks + blockIdx.x];
 	for (uint64_t t0 = lo; t0 < hi; t0 += scan_tile<Src>()) {
 		// each thread owns kItems consecutive items of the tile
 		uint32_t v[kItems];

@@ -651,11 +651,9 @@ int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, Vox
 		if ((e = launch(ctx, "vox_scan_reduce", scan_reduce<CountSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
 		                ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "vox_scan_reduce");
-		if ((e = launch(ctx, "vox_scan_parts", scan_parts<TotalCommit>, dim3(1), dim3(1024), 0, ctx->d_scan_parts,
+		if ((e = launch(ctx, "vox_scan_down", scan_down<CountSrc, OffsetDst, TotalCommit>, dim3(kScanBlocks),
+		                dim3(kScanThreads), 0, src, OffsetDst{vb->offsets}, (const uint64_t *)ctx->d_scan_parts,
 		                TotalCommit{total})) != hipSuccess)
-			return hip_check(ctx, e, "vox_scan_parts");
-		if ((e = launch(ctx, "vox_scan_down", scan_down<CountSrc, OffsetDst>, dim3(kScanBlocks), dim3(kScanThreads),
-		                0, src, OffsetDst{vb->offsets}, (const uint64_t *)ctx->d_scan_parts)) != hipSuccess)
 			return hip_check(ctx, e, "vox_scan_down");
 	} else if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0,
 	                       reinterpret_cast<uint32_t *>(total))) != hipSuccess ||
