@@ -44,7 +44,7 @@ namespace svo {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kLdsBucket = 4096; // largest bucket sorted in LDS
+constexpr uint32_t kLdsBucket = 2048; // largest bucket sorted in LDS
 constexpr uint32_t kRegBucket = 16;   // leaf blocks up to this size: one thread each (build_fold)
 constexpr uint32_t kWaveBucket = 64;  // up to this size: one wave each (build_fold_mid)
 constexpr uint32_t kWideBucket = 512; // up to this size: one wave each, sorted in LDS (build_fold_wide)
@@ -865,19 +865,17 @@ template <class Get, class Put> __device__ void block_bitonic(uint32_t n, uint32
 // Leaf blocks of 17..64 fragments, one wave each: lane i holds key i and its leaf slot (padding:
 // slot 8, key +inf); a bitonic sort over lane shuffles orders them by (slot, key), so every leaf's
 // fragments are consecutive and ascending.  Lane s < 8 then folds the run of slot s from LDS.
-__global__ void __launch_bounds__(kThreads)
-    build_fold_mid(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
-                   const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
-                   const uint32_t *list, uint64_t cap) {
+// Blocks blk of nblk of the list launch; s_col: 64 words of LDS per wave.
+__device__ void fold_mid_blocks(uint32_t blk, uint32_t nblk, const BuildSeg *__restrict__ segs,
+                                const uint64_t *__restrict__ bbeg, uint32_t S, const uint32_t *__restrict__ cnt,
+                                const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket, const uint32_t *list,
+                                uint64_t cap, uint32_t (*s_col)[64], const uint32_t *s_magic) {
 	constexpr uint32_t kWaves = kThreads / 64;
-	__shared__ uint32_t s_magic[65];
-	__shared__ uint32_t s_col[kWaves][64];
-	stage_magic(s_magic);
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 	uint64_t count = list[0];
 	if (count > cap)
 		count = cap;
-	for (uint64_t base = (uint64_t)blockIdx.x * kWaves; base < count; base += (uint64_t)gridDim.x * kWaves) {
+	for (uint64_t base = (uint64_t)blk * kWaves; base < count; base += (uint64_t)nblk * kWaves) {
 		const uint64_t item = base + wave;
 		if (item >= count)
 			break; // wave-uniform; no block barrier follows
@@ -931,23 +929,20 @@ __global__ void __launch_bounds__(kThreads)
 // are ordered between stages by wave barriers), so every leaf's fragments are consecutive and
 // ascending; lane s < 8 then folds the run of slot s.
 
-__global__ void __launch_bounds__(kThreads)
-    build_fold_wide(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
-                    const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
-                    const uint32_t *list, uint64_t cap) {
+// Blocks blk of nblk of the list launch; s_keys / s_slots: kWideBucket entries per wave.
+__device__ void fold_wide_blocks(uint32_t blk, uint32_t nblk, const BuildSeg *__restrict__ segs,
+                                 const uint64_t *__restrict__ bbeg, uint32_t S, const uint32_t *__restrict__ cnt,
+                                 const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket, const uint32_t *list,
+                                 uint64_t cap, uint64_t *s_keys, uint8_t *s_slots, uint32_t (*s_beg)[8],
+                                 uint32_t (*s_end)[8], const uint32_t *s_magic) {
 	constexpr uint32_t kWaves = kThreads / 64;
-	__shared__ uint32_t s_magic[65];
-	__shared__ uint64_t s_keys[kWaves][kWideBucket];
-	__shared__ uint8_t s_slots[kWaves][kWideBucket];
-	__shared__ uint32_t s_beg[kWaves][8], s_end[kWaves][8];
-	stage_magic(s_magic);
 	const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-	uint64_t *keys = s_keys[wave];
-	uint8_t *slots = s_slots[wave];
+	uint64_t *keys = s_keys + wave * kWideBucket;
+	uint8_t *slots = s_slots + wave * kWideBucket;
 	uint64_t count = list[2];
 	if (count > cap)
 		count = cap;
-	for (uint64_t base = (uint64_t)blockIdx.x * kWaves; base < count; base += (uint64_t)gridDim.x * kWaves) {
+	for (uint64_t base = (uint64_t)blk * kWaves; base < count; base += (uint64_t)nblk * kWaves) {
 		const uint64_t item = base + wave;
 		if (item >= count)
 			break; // wave-uniform; no block barrier follows
@@ -1038,23 +1033,20 @@ __device__ void block_bitonic_kv(uint32_t n, uint32_t m, uint64_t *keys, uint8_t
 	}
 }
 
-// Leaf blocks of more than 512 fragments, one block each.  Up to 4096 fragments: bitonic sort of the
-// (slot, key) pairs in LDS, then lane s < 8 folds the run of slot s.  Beyond (a global-memory
+// Leaf blocks of more than 512 fragments, one block each.  Up to kLdsBucket fragments: bitonic sort
+// of the (slot, key) pairs in LDS, then lane s < 8 folds the run of slot s.  Beyond (a global-memory
 // sort, rare): keys sorted in place in the bucket and each of the 8 lanes walks all of them,
-// taking those of its slot.
-__global__ void __launch_bounds__(kThreads)
-    build_fold_big(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
-                   const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, uint2 *bucket,
-                   const uint32_t *list, uint64_t cap) {
-	__shared__ uint32_t s_magic[65];
-	__shared__ uint64_t s_keys[kLdsBucket];
-	__shared__ uint8_t s_slots[kLdsBucket];
-	__shared__ uint32_t s_beg[8], s_end[8];
-	stage_magic(s_magic);
+// taking those of its slot.  Blocks blk of nblk of the list launch; s_keys / s_slots: kLdsBucket
+// entries for the block.
+__device__ void fold_big_blocks(uint32_t blk, uint32_t nblk, const BuildSeg *__restrict__ segs,
+                                const uint64_t *__restrict__ bbeg, uint32_t S, const uint32_t *__restrict__ cnt,
+                                const uint32_t *__restrict__ aux, uint2 *bucket, const uint32_t *list, uint64_t cap,
+                                uint64_t *s_keys, uint8_t *s_slots, uint32_t *s_beg, uint32_t *s_end,
+                                const uint32_t *s_magic) {
 	uint64_t count = list[1];
 	if (count > cap)
 		count = cap;
-	for (uint64_t item = blockIdx.x; item < count; item += gridDim.x) {
+	for (uint64_t item = blk; item < count; item += nblk) {
 		const uint32_t j = list[3u + cap + item];
 		const uint32_t n = cnt[j];
 		const uint32_t s = S == 1u ? 0u : seg_find(bbeg, S, j);
@@ -1105,6 +1097,31 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
+
+// The fold's three work lists in one launch: blocks [0, g_mid) take the 17..64-fragment leaf blocks,
+// the next g_wide the 65..512-fragment ones, the rest the larger ones.  Each role is block-uniform,
+// so the big role's block barriers are safe; the LDS is shared between the roles.
+__global__ void __launch_bounds__(kThreads)
+    build_fold_lists(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
+                     const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, uint2 *bucket,
+                     const uint32_t *list, uint64_t cap, uint32_t g_mid, uint32_t g_wide) {
+	constexpr uint32_t kWaves = kThreads / 64;
+	static_assert(kWaves * kWideBucket == kLdsBucket, "the wide and big roles share the key arrays");
+	__shared__ uint32_t s_magic[65];
+	__shared__ uint64_t s_keys[kLdsBucket];
+	__shared__ uint8_t s_slots[kLdsBucket];
+	__shared__ uint32_t s_beg[kWaves][8], s_end[kWaves][8];
+	stage_magic(s_magic);
+	if (blockIdx.x < g_mid)
+		fold_mid_blocks(blockIdx.x, g_mid, segs, bbeg, S, cnt, aux, bucket, list, cap,
+		                reinterpret_cast<uint32_t (*)[64]>(s_keys), s_magic);
+	else if (blockIdx.x < g_mid + g_wide)
+		fold_wide_blocks(blockIdx.x - g_mid, g_wide, segs, bbeg, S, cnt, aux, bucket, list, cap, s_keys, s_slots, s_beg,
+		                 s_end, s_magic);
+	else
+		fold_big_blocks(blockIdx.x - g_mid - g_wide, gridDim.x - g_mid - g_wide, segs, bbeg, S, cnt, aux, bucket, list,
+		                cap, s_keys, s_slots, s_beg[0], s_end[0], s_magic);
+}
 
 // ---- scan plumbing for the bin offsets ---------------------------------------------------------
 struct BinCounts {
@@ -1223,18 +1240,12 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	                S, bb.nbins, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket, bb.list,
 	                cap)) != hipSuccess)
 		return hip_check(ctx, e, "build_fold");
-	if ((e = launch(ctx, "build_fold_mid", build_fold_mid, dim3((uint32_t)ctx->cu_count * 8u), dim3(kThreads), 0,
-	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket,
-	                (const uint32_t *)bb.list, cap)) != hipSuccess)
-		return hip_check(ctx, e, "build_fold_mid");
-	if ((e = launch(ctx, "build_fold_wide", build_fold_wide, dim3((uint32_t)ctx->cu_count * 8u), dim3(kThreads), 0,
-	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket,
-	                (const uint32_t *)bb.list, cap)) != hipSuccess)
-		return hip_check(ctx, e, "build_fold_wide");
-	if ((e = launch(ctx, "build_fold_big", build_fold_big, dim3((uint32_t)ctx->cu_count * 2u), dim3(kThreads), 0,
-	                segs, bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, bucket,
-	                (const uint32_t *)bb.list, cap)) != hipSuccess)
-		return hip_check(ctx, e, "build_fold_big");
+	const uint32_t g_mid = (uint32_t)ctx->cu_count * 8u, g_wide = (uint32_t)ctx->cu_count * 8u,
+	               g_big = (uint32_t)ctx->cu_count * 2u;
+	if ((e = launch(ctx, "build_fold_lists", build_fold_lists, dim3(g_mid + g_wide + g_big), dim3(kThreads), 0, segs,
+	                bb.bbeg(), S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, bucket, (const uint32_t *)bb.list,
+	                cap, g_mid, g_wide)) != hipSuccess)
+		return hip_check(ctx, e, "build_fold_lists");
 	return SVO_OK;
 }
 
