@@ -33,6 +33,8 @@
 // chunk's words before it.  Every rank is a difference of two R-based counts of the same scene,
 // so modular uint32 arithmetic stays exact.
 #include <algorithm>
+#include <cstring>
+#include <vector>
 #include <string>
 
 #include "svo_batch.hpp"
@@ -1258,8 +1260,8 @@ uint64_t build_bitmap_words(uint32_t L) {
 }
 
 void build_batch_release(svo_ctx *ctx, BuildBatch *bb) {
-	for (void *p : {(void *)bb->segs, (void *)bb->index, (void *)bb->bm, (void *)bb->rk, (void *)bb->cnt,
-	                (void *)bb->aux, (void *)bb->node, (void *)bb->bucket, (void *)bb->list})
+	for (void *p : {(void *)bb->segs, (void *)bb->bm, (void *)bb->rk, (void *)bb->cnt, (void *)bb->aux,
+	                (void *)bb->node, (void *)bb->bucket, (void *)bb->list}) // index lives in segs' allocation
 		ctx->pool->free(p);
 	bb->segs = nullptr;
 	bb->index = nullptr;
@@ -1326,18 +1328,26 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	const uint64_t cap = big_capacity(bb.Ftot);
 	Pool *pool = ctx->pool;
 	int rc = SVO_OK;
-	if (pool->alloc((void **)&bb.segs, sizeof(BuildSeg) * S) != hipSuccess ||
-	    pool->alloc((void **)&bb.index, sizeof(uint64_t) * bb.h_index.size()) != hipSuccess ||
+	// the descriptors and the begin tables share one allocation, so each upload is one copy
+	const size_t seg_bytes = (sizeof(BuildSeg) * S + 255) & ~size_t(255);
+	const size_t idx_bytes = sizeof(uint64_t) * bb.h_index.size();
+	std::vector<uint8_t> tables(seg_bytes + idx_bytes);
+	auto upload_tables = [&]() {
+		std::memcpy(tables.data(), bb.h_segs.data(), sizeof(BuildSeg) * S);
+		std::memcpy(tables.data() + seg_bytes, bb.h_index.data(), idx_bytes);
+		return upload_table(ctx, bb.segs, tables.data(), tables.size());
+	};
+	if (pool->alloc((void **)&bb.segs, seg_bytes + idx_bytes) != hipSuccess ||
 	    pool->alloc((void **)&bb.bm, sizeof(uint32_t) * (bb.TW ? bb.TW : 1)) != hipSuccess ||
 	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW / 8u + 1)) != hipSuccess ||
 	    pool->alloc((void **)&bb.node, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
 	    pool->alloc((void **)&bb.bucket, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
 	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (3 + 3 * cap)) != hipSuccess)
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
-	if (!rc)
-		rc = upload_table(ctx, bb.segs, bb.h_segs.data(), sizeof(BuildSeg) * S);
-	if (!rc)
-		rc = upload_table(ctx, bb.index, bb.h_index.data(), sizeof(uint64_t) * bb.h_index.size());
+	if (!rc) {
+		bb.index = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(bb.segs) + seg_bytes);
+		rc = upload_tables();
+	}
 	// structure phase: bitmaps + ranks decide every W exactly; one read-back sizes the octrees
 	if (!rc) {
 		ctx->t_build.start(ctx->stream);
@@ -1374,9 +1384,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	            pool->alloc((void **)&bb.aux, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess))
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	if (!rc)
-		rc = upload_table(ctx, bb.segs, bb.h_segs.data(), sizeof(BuildSeg) * S);
-	if (!rc)
-		rc = upload_table(ctx, bb.index, bb.h_index.data(), sizeof(uint64_t) * bb.h_index.size());
+		rc = upload_tables();
 	if (!rc) {
 		ctx->t_build2.start(ctx->stream);
 		rc = node_phase(ctx, bb);

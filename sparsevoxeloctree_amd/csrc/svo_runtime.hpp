@@ -87,7 +87,7 @@ private:
 class Staging {
 public:
 	static constexpr uint32_t kSlots = 8;
-	static constexpr size_t kSlotBytes = size_t(512) << 10;
+	static constexpr size_t kSlotBytes = size_t(1) << 20; // a build batch of 1024 scenes: 0.55 MB
 	hipError_t init();
 	void destroy();
 	// Next free slot (waits for its previous copy); null when bytes exceed a slot.

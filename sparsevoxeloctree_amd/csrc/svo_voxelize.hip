@@ -17,7 +17,9 @@
 // scene's fragments land in its own list.
 //
 // Scene normalisation (src/Scene.cpp:62-99) runs here too: bounding-box reduction + transform.
+#include <cstring>
 #include <string>
+#include <vector>
 
 #include "svo_batch.hpp"
 #include "svo_device.hpp"
@@ -593,7 +595,7 @@ int normalize_scene(svo_ctx *ctx, svo_scene *scene) {
 }
 
 void vox_batch_release(svo_ctx *ctx, VoxBatch *vb) {
-	for (void *p : {(void *)vb->segs, (void *)vb->tbeg, (void *)vb->offsets, (void *)vb->big})
+	for (void *p : {(void *)vb->segs, (void *)vb->offsets}) // tbeg and big live in segs' allocation
 		ctx->pool->free(p);
 	vb->segs = nullptr;
 	vb->tbeg = nullptr;
@@ -628,20 +630,25 @@ int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, Vox
 		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 triangles in one voxelize call");
 	Pool *pool = ctx->pool;
 	hipError_t e;
-	if (pool->alloc((void **)&vb->segs, sizeof(VoxSeg) * S) != hipSuccess ||
-	    pool->alloc((void **)&vb->tbeg, sizeof(uint64_t) * (S + 1)) != hipSuccess ||
-	    pool->alloc((void **)&vb->offsets, sizeof(uint64_t) * (T ? T : 1)) != hipSuccess ||
-	    pool->alloc((void **)&vb->big, sizeof(uint32_t) * (T + 1)) != hipSuccess)
+	// descriptors, triangle begins and the big-triangle list share one allocation, so one copy
+	// uploads the tables and clears the list's count
+	const size_t seg_bytes = (sizeof(VoxSeg) * S + 255) & ~size_t(255);
+	const size_t tbeg_bytes = (sizeof(uint64_t) * (S + 1) + 255) & ~size_t(255);
+	if (pool->alloc((void **)&vb->segs, seg_bytes + tbeg_bytes + sizeof(uint32_t) * (T + 1)) != hipSuccess ||
+	    pool->alloc((void **)&vb->offsets, sizeof(uint64_t) * (T ? T : 1)) != hipSuccess)
 		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "count-pass allocation failed");
-	int rc = upload_table(ctx, vb->segs, vb->h_segs.data(), sizeof(VoxSeg) * S);
-	if (!rc)
-		rc = upload_table(ctx, vb->tbeg, vb->h_tbeg.data(), sizeof(uint64_t) * (S + 1));
-	if (rc)
-		return rc;
+	vb->tbeg = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(vb->segs) + seg_bytes);
+	vb->big = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(vb->segs) + seg_bytes + tbeg_bytes);
+	{
+		std::vector<uint8_t> tables(seg_bytes + tbeg_bytes + sizeof(uint32_t), 0);
+		std::memcpy(tables.data(), vb->h_segs.data(), sizeof(VoxSeg) * S);
+		std::memcpy(tables.data() + seg_bytes, vb->h_tbeg.data(), sizeof(uint64_t) * (S + 1));
+		if (int rc = upload_table(ctx, vb->segs, tables.data(), tables.size()))
+			return rc; // big[0] = 0 came with the tables
+	}
+	int rc = SVO_OK;
 	uint64_t *total = ctx->d_scan_parts + 2 * kScanBlocks;
 	ctx->t_count.start(ctx->stream);
-	if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0, vb->big)) != hipSuccess)
-		return hip_check(ctx, e, "vox_zero");
 	if (T > 0) {
 		if ((e = launch(ctx, "vox_count", vox_count, dim3(grid_for(T, ctx)), dim3(kThreads), 0,
 		                (const VoxSeg *)vb->segs, (const uint64_t *)vb->tbeg, S, T, vb->offsets, vb->big)) !=
