@@ -102,15 +102,18 @@ def test_build_repeatable(ctx, no_device_errors):
 
 
 def test_build_bin_size_classes(ctx, no_device_errors):
-    # leaf blocks holding 1..16 (thread sorter), 17..64 (wave shuffle sorter), 65..512 (wave LDS
-    # sorter) and 513..1500 (block LDS sorter) fragments, spread over the 8 leaves with repeated
+    # leaf blocks holding 1..4, 5..8, 9..16 (thread sorter, three networks), 17..64 (wave shuffle
+    # sorter), 65..512 (wave LDS sorter), 513..2048 (block LDS sorter) and more (in-place HBM sort)
+    # fragments, every class boundary exactly once, spread over the 8 leaves with repeated
     # coordinates and repeated full fragments
     L = 7
     rng = np.random.default_rng(77)
-    nodes = rng.choice(1 << (3 * (L - 1)), size=240, replace=False)
+    nodes = rng.choice(1 << (3 * (L - 1)), size=252, replace=False)
+    edges = [1, 4, 5, 8, 9, 16, 17, 64, 65, 512, 513, 2048, 2049]
+    classes = [(1, 9), (9, 17), (17, 65), (65, 513), (513, 2049), (2049, 2400)]  # [lo, hi)
     parts = []
     for i, P in enumerate(nodes):
-        n = [rng.integers(1, 17), rng.integers(17, 65), rng.integers(65, 513), rng.integers(513, 1501)][i % 4]
+        n = edges[i] if i < len(edges) else int(rng.integers(*classes[i % len(classes)]))
         X = Y = Z = 0
         for b in range(L - 1):  # de-interleave the Morton index of the depth-(L-1) node
             X |= ((int(P) >> (3 * b)) & 1) << b
