@@ -274,6 +274,17 @@ int svo_server_reset_stats(svo_server *srv);
 int svo_camera_from_yaw_pitch(const float position[3], float yaw, float pitch, float fov, float aspect,
                               svo_camera *out);
 
+/* ---- OBJ + MTL files (Scene::load_meshes, src/Scene.cpp:36-147, over tinyobj::LoadObj): the
+ * triangle soup for svo_scene_create.  *xyz: 9 floats per triangle, the file's positions (not yet
+ * normalised: pass normalize = 1 to svo_scene_create, as Scene::Create does); *rgb: per triangle,
+ * packUnorm4x8(vec4(Kd, 0)) & 0xffffff of its material (src/Scene.cpp:164).  Faces are grouped per
+ * material, largest group first; polygons are ear-clipped as tinyobj does.  Both arrays are
+ * allocated here and released with svo_obj_free.  Errors (unreadable file, no material, a face
+ * without a material, an index of 0 or out of range, no faces) return SVO_ERR_INVALID_ARGUMENT with
+ * the reason in msg (msgcap bytes, may be 0). */
+int svo_obj_load(const char *path, float **xyz, uint32_t **rgb, uint64_t *T, char *msg, uint64_t msgcap);
+void svo_obj_free(float *xyz, uint32_t *rgb);
+
 /* ---- synthetic scenes used by the tests and the benchmark (seeded, deterministic).
  * kind: 1 = UV spheres (64x32, 4032 triangles each), 2 = box room + rotated boxes + spheres,
  * 3 = spheres + thin slivers.  With tri_xyz == NULL only *T is returned (the triangle count). */

@@ -22,6 +22,7 @@ from .api import (  # noqa: F401
     build_batch,
     camera_look_at,
     gen_scene,
+    load_obj,
     march_batch,
     march_rays,
     march_rays_batch,

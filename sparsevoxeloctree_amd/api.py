@@ -22,6 +22,7 @@ Errors raise SvoError carrying the library's status code.
 """
 import ctypes as C
 import math
+import os
 
 import numpy as np
 
@@ -603,6 +604,26 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def load_obj(path):
+    """OBJ + MTL file -> (xyz (T, 9) float32 positions as in the file, rgb (T,) uint32 per-triangle
+    packUnorm4x8(Kd)), the input of Scene.Create(..., normalize=True) (Scene::load_meshes,
+    src/Scene.cpp:36-147)."""
+    xyz_p = C.POINTER(C.c_float)()
+    rgb_p = C.POINTER(C.c_uint32)()
+    T = C.c_uint64()
+    msg = C.create_string_buffer(512)
+    rc = lib.svo_obj_load(os.fsencode(path), C.byref(xyz_p), C.byref(rgb_p), C.byref(T), msg, len(msg))
+    if rc:
+        raise SvoError(rc, "load_obj %s: %s" % (path, msg.value.decode(errors="replace")))
+    try:
+        n = T.value
+        xyz = np.ctypeslib.as_array(xyz_p, shape=(n * 9,)).reshape(n, 9).copy() if n else np.zeros((0, 9), np.float32)
+        rgb = np.ctypeslib.as_array(rgb_p, shape=(n,)).copy() if n else np.zeros(0, np.uint32)
+    finally:
+        lib.svo_obj_free(xyz_p, rgb_p)
+    return xyz, rgb
 
 
 def gen_scene(kind, seed, param=0):

@@ -26,6 +26,7 @@ SOURCES = [
     "svo_util.hip",
     "svo_server.cpp",
     "scenegen.cpp",
+    "svo_obj.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

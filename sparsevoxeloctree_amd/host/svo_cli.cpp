@@ -18,15 +18,15 @@
 #include "svo.hpp"
 
 static int usage() {
-	std::fprintf(stderr, "usage: svo_cli (-scene KIND[,SEED[,PARAM]] | -tris FILE) -lvl N [-frame W H] [-no-beam] "
-	                     "[-save FILE]\n");
+	std::fprintf(stderr, "usage: svo_cli (-scene KIND[,SEED[,PARAM]] | -tris FILE | -obj FILE.obj) -lvl N "
+	                     "[-frame W H] [-no-beam] [-save FILE]\n");
 	return 2;
 }
 
 int main(int argc, char **argv) {
 	int kind = 0;
 	uint64_t seed = 1, param = 0;
-	std::string tris_path, save_path;
+	std::string tris_path, obj_path, save_path;
 	uint32_t level = 0, fw = 0, fh = 0;
 	bool beam = true;
 	for (int i = 1; i < argc; ++i) {
@@ -38,6 +38,8 @@ int main(int argc, char **argv) {
 			seed = s, param = p;
 		} else if (!std::strcmp(argv[i], "-tris") && i + 1 < argc) {
 			tris_path = argv[++i];
+		} else if (!std::strcmp(argv[i], "-obj") && i + 1 < argc) {
+			obj_path = argv[++i]; // the reference's LoaderThread input (src/LoaderThread.cpp:43-56)
 		} else if (!std::strcmp(argv[i], "-lvl") && i + 1 < argc) {
 			level = (uint32_t)std::atoi(argv[++i]);
 		} else if (!std::strcmp(argv[i], "-frame") && i + 2 < argc) {
@@ -50,7 +52,7 @@ int main(int argc, char **argv) {
 		} else
 			return usage();
 	}
-	if ((kind == 0) == tris_path.empty() || level == 0)
+	if ((kind != 0) + !tris_path.empty() + !obj_path.empty() != 1 || level == 0)
 		return usage();
 
 	std::vector<float> xyz;
@@ -62,6 +64,17 @@ int main(int argc, char **argv) {
 		xyz.resize(9 * T);
 		rgb.resize(T);
 		svo_gen_scene(kind, seed, param, xyz.data(), rgb.data(), &T);
+	} else if (!obj_path.empty()) {
+		float *px = nullptr;
+		uint32_t *pc = nullptr;
+		char msg[512] = {0};
+		if (svo_obj_load(obj_path.c_str(), &px, &pc, &T, msg, sizeof msg) != SVO_OK) {
+			std::fprintf(stderr, "%s: %s\n", obj_path.c_str(), msg);
+			return 1;
+		}
+		xyz.assign(px, px + 9 * T);
+		rgb.assign(pc, pc + T);
+		svo_obj_free(px, pc);
 	} else {
 		FILE *f = std::fopen(tris_path.c_str(), "rb");
 		if (!f || std::fread(&T, 8, 1, f) != 1) {

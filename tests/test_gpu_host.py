@@ -36,6 +36,21 @@ def test_cli_load_path_matches_oracle(tmp_path):
     assert re.search(r"frame 160x90 hits \d+", r.stdout)
 
 
+def test_cli_obj_input_matches_oracle(tmp_path):
+    # the reference's load path from an OBJ file: svo_cli -obj (svo_obj_load + Scene::Create)
+    import sparsevoxeloctree_amd as S
+
+    obj = os.path.join(ROOT, "tests", "golden", "obj", "shapes.obj")
+    out = tmp_path / "shapes.svo"
+    r = subprocess.run([CLI, "-obj", obj, "-lvl", "6", "-save", str(out)], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    xyz, rgb = S.load_obj(obj)
+    ref_words = oracle.build(oracle.voxelize(oracle.normalize(xyz), rgb, 6), 6)
+    with S.Context(0) as ctx:
+        assert np.array_equal(S.Octree.Load(ctx, out).words(), ref_words)
+
+
 def test_octree_file_round_trip(ctx, tmp_path):
     import sparsevoxeloctree_amd as S
     from svo_testutil import random_frags

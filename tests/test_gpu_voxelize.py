@@ -120,3 +120,33 @@ def test_voxelize_then_build_end_to_end(ctx, no_device_errors):
     b.CmdBuild()
     w = b.GetOctree().words()
     assert np.array_equal(w, oracle.build(oracle.voxelize(oracle.normalize(xyz), rgb, level), level))
+
+
+def test_obj_file_voxelize_and_build(ctx, no_device_errors, tmp_path):
+    # an OBJ scene (svo_obj_load, Scene::load_meshes) through normalisation, voxelization and the
+    # build: 16 spheres written with 4 materials, and the polygon fixture
+    import os
+
+    import sparsevoxeloctree_amd as S
+
+    xyz, rgb = S.gen_scene(1, 0x5EED0007, 16)
+    kd = np.array([[0.8, 0.1, 0.1], [0.1, 0.7, 0.2], [0.5, 0.5, 0.5], [0.2, 0.3, 1.0]])
+    (tmp_path / "s.mtl").write_text("".join("newmtl m%d\nKd %g %g %g\n" % ((m,) + tuple(kd[m])) for m in range(4)))
+    lines = ["mtllib s.mtl"]
+    for t in range(xyz.shape[0]):
+        if t % 4032 == 0:
+            lines.append("usemtl m%d" % ((t // 4032) % 4))
+        for c in range(3):
+            lines.append("v %.9g %.9g %.9g" % tuple(xyz[t, 3 * c:3 * c + 3]))
+        lines.append("f -3 -2 -1")
+    (tmp_path / "s.obj").write_text("\n".join(lines) + "\n")
+    here = os.path.dirname(os.path.abspath(__file__))
+    for path, level in ((str(tmp_path / "s.obj"), 9), (os.path.join(here, "golden", "obj", "shapes.obj"), 6)):
+        oxyz, orgb = S.load_obj(path)
+        print("obj %s: %d triangles, level %d" % (os.path.basename(path), oxyz.shape[0], level), flush=True)
+        _, vox, fr = gpu_voxelize(ctx, oxyz, orgb, level)
+        want = oracle.voxelize(oracle.normalize(oxyz), orgb, level)
+        assert np.array_equal(fr, want)
+        b = S.OctreeBuilder.Create(vox)
+        b.CmdBuild()
+        assert np.array_equal(b.GetOctree().words(), oracle.build(want, level))
