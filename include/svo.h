@@ -274,6 +274,19 @@ int svo_server_reset_stats(svo_server *srv);
 int svo_camera_from_yaw_pitch(const float position[3], float yaw, float pitch, float fov, float aspect,
                               svo_camera *out);
 
+/* ---- progressive path tracer (PathTracer::CmdRender, src/PathTracer.cpp:176-215;
+ * ConstColorPathTrace and the accumulation of shader/path_tracer.comp:32-60,111-132).  One sample
+ * per pixel: the camera ray through pixel + the frame's 2-D Sobol point (the pixel corner at
+ * spp 0), then up to bounce-1 cosine-weighted diffuse bounces (Octree_RayMarchLeaf, 5 arguments);
+ * a path that escapes adds sky (linear RGB) times the product of the leaf albedos it met.
+ * color_dev / albedo_dev / normal_dev: width*height float4 (w = 1), row-major, the running means
+ * after spp + 1 samples (spp = samples already in the buffers; spp 0 starts them, as a Reset).
+ * albedo_dev and normal_dev (first-hit albedo and face normal) may be NULL.  The sampling is seeded
+ * (seed) and replaces the reference's Sobol buffer and blue-noise table.  bounce: 1..64. */
+int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
+                   uint32_t bounce, uint32_t spp, const float sky[3], uint64_t seed, float *color_dev,
+                   float *albedo_dev, float *normal_dev);
+
 /* ---- OBJ + MTL files (Scene::load_meshes, src/Scene.cpp:36-147, over tinyobj::LoadObj): the
  * triangle soup for svo_scene_create.  *xyz: 9 floats per triangle, the file's positions (not yet
  * normalised: pass normalize = 1 to svo_scene_create, as Scene::Create does); *rgb: per triangle,

@@ -47,6 +47,7 @@ def load():
             "orc_march_rays": (None, [P, u64, u32, P, u64, i32, P, i32]),
             "orc_bounce_rays": (None, [P, u64, u64, P, P]),
             "orc_sincos": (None, [f32, P, P]),
+            "orc_path_trace": (None, [P, u64, u32, P, u32, u32, u32, u32, P, u64, P, P, P, i32]),
             "orc_stack_uninit_reads": (u64, []),
             "orc_reset_counters": (None, []),
         }
@@ -168,6 +169,19 @@ def bounce_rays(hits, seed):
     mask = np.empty(R, dtype=np.uint8)
     lib.orc_bounce_rays(_p(hits), R, seed, _p(rays), _p(mask))
     return rays, mask.astype(bool)
+
+
+def path_trace(words, level, cam, width, height, bounce, spp, sky, seed, color=None, albedo=None, normal=None,
+               threads=None):
+    """One progressive sample per pixel into float32 (height*width, 4) buffers (created when None and
+    returned as (color, albedo, normal))."""
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    bufs = [np.zeros((width * height, 4), np.float32) if b is None else b for b in (color, albedo, normal)]
+    sky = np.ascontiguousarray(sky, dtype=np.float32)
+    load().orc_path_trace(_p(words), words.shape[0], level, _p(cam), width, height, bounce, spp, _p(sky), seed,
+                       _p(bufs[0]), _p(bufs[1]), _p(bufs[2]), threads or nthreads())
+    return tuple(bufs)
 
 
 def sincos(phi):

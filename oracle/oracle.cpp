@@ -801,6 +801,95 @@ void orc_bounce_rays(const orc_hit *hits, uint64_t R, uint64_t seed, float *rays
 	}
 }
 
+// lowbias32 integer hash (C. Wellons' hash-prospector).
+static uint32_t hash32(uint32_t x) {
+	x ^= x >> 16;
+	x *= 0x7feb352du;
+	x ^= x >> 15;
+	x *= 0x846ca68bu;
+	x ^= x >> 16;
+	return x;
+}
+
+// ConstColorPathTrace + the running means of shader/path_tracer.comp:32-60,111-132, with the seeded
+// sampling that replaces the reference's Sobol buffer and blue-noise table: bounce cur of frame spp
+// uses the 2-D Sobol point spp XOR-shifted by hash32(key ^ cur) and rotated by a per-pixel hash.
+void orc_path_trace(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                    uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, uint64_t seed, float *color,
+                    float *albedo, float *normal, int nthreads) {
+	(void)level;
+	Octree oc{words, W};
+	uint64_t st = seed;
+	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
+	const uint32_t s0 = (uint32_t)z0, s1 = (uint32_t)(z0 >> 32), s2 = (uint32_t)z1, s3 = (uint32_t)(z1 >> 32);
+	const float kPi = 3.1415926535897932384626433832795f; // shader/util.glsl:4
+	const float jx = (float)(sobol0(spp) >> 8) * 5.9604644775390625e-08f,
+	            jy = (float)(sobol1(spp) >> 8) * 5.9604644775390625e-08f;
+	auto colour = [](uint32_t w, int c) { return (float)((w >> (8 * c)) & 0xffu) / 255.0f; }; // unpackUnorm4x8
+	auto face = [](uint32_t code) {
+		V3 n{0.0f, 0.0f, 0.0f};
+		const float sg = (code & 1u) ? 1.0f : -1.0f;
+		(code >> 1 == 0 ? n.x : code >> 1 == 1 ? n.y : n.z) = sg;
+		return n;
+	};
+	parallel_for((uint64_t)width * height, nthreads, [&](uint64_t b, uint64_t e) {
+		for (uint64_t kk = b; kk < e; ++kk) {
+			const uint32_t k = (uint32_t)kk, px = k % width, py = k / width;
+			V3 o = cam_vec(cam, 0);
+			V3 d = gen_ray(cam, ((float)px + jx) / (float)width, ((float)py + jy) / (float)height);
+			orc_hit h = march_leaf(oc, o, d, 1);
+			V3 rad{sky[0], sky[1], sky[2]}, alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
+			if (h.vz_flags & 0x80000000u) {
+				V3 acc{colour(h.leaf, 0), colour(h.leaf, 1), colour(h.leaf, 2)};
+				alb0 = acc;
+				uint32_t code = (h.vz_flags >> 24) & 7u;
+				nrm0 = face(code);
+				rad = V3{0.0f, 0.0f, 0.0f};
+				const uint32_t na = hash32(k ^ s2), nb = hash32(na ^ s3);
+				for (uint32_t cur = 1; cur < bounce; ++cur) {
+					const uint32_t su = (sobol0(spp) ^ hash32(s0 ^ cur)) + na, sv = (sobol1(spp) ^ hash32(s1 ^ cur)) + nb;
+					const float u = (float)(su >> 8) * 5.9604644775390625e-08f,
+					            v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+					// Diffuse_Sample + AlignDirection (shader/diffuse.glsl:7-14, shader/util.glsl:5-9)
+					const V3 nrm = face(code);
+					float rr = std::sqrt(u), phi = 2.0f * kPi * v;
+					float sn, cs;
+					sincos_poly(phi, sn, cs);
+					V3 dl{rr * cs, rr * sn, std::sqrt(1.0f - u)};
+					V3 ax = std::fabs(nrm.x) > 0.01f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+					V3 uu = normalize(cross(ax, nrm));
+					V3 vv = cross(nrm, uu);
+					o = V3{h.pos[0], h.pos[1], h.pos[2]};
+					d = V3{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
+					       dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+					h = march_leaf(oc, o, d, 1);
+					if (!(h.vz_flags & 0x80000000u)) {
+						rad = V3{acc.x * sky[0], acc.y * sky[1], acc.z * sky[2]};
+						break;
+					}
+					acc = V3{acc.x * colour(h.leaf, 0), acc.y * colour(h.leaf, 1), acc.z * colour(h.leaf, 2)};
+					code = (h.vz_flags >> 24) & 7u;
+				}
+			}
+			const float n = (float)spp, n1 = (float)(spp + 1u);
+			auto mean = [&](float *img, V3 v) {
+				if (!img)
+					return;
+				float *p = img + 4 * (uint64_t)k;
+				if (spp == 0u) {
+					p[0] = v.x, p[1] = v.y, p[2] = v.z;
+				} else {
+					p[0] = (p[0] * n + v.x) / n1, p[1] = (p[1] * n + v.y) / n1, p[2] = (p[2] * n + v.z) / n1;
+				}
+				p[3] = 1.0f;
+			};
+			mean(color, rad);
+			mean(albedo, alb0);
+			mean(normal, nrm0);
+		}
+	});
+}
+
 uint64_t orc_stack_uninit_reads(void) { return g_uninit_reads.load(); }
 void orc_reset_counters(void) { g_uninit_reads.store(0); }
 

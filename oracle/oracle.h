@@ -38,6 +38,13 @@ typedef struct orc_hit {
 	uint32_t nfetch;   /* executed octree loads */
 } orc_hit;
 
+/* ConstColorPathTrace with running means (shader/path_tracer.comp:32-60,111-132) and the seeded
+ * sampling of svo_path_trace: one sample per pixel into float4 color/albedo/normal (albedo and
+ * normal may be NULL); spp = samples already accumulated. */
+void orc_path_trace(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                    uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, uint64_t seed, float *color,
+                    float *albedo, float *normal, int nthreads);
+
 /* Scene::load_meshes normalisation to [-1,1]^3 (src/Scene.cpp:62-99). In place on xyz[3*nverts]. */
 void orc_normalize(float *xyz, uint64_t nverts);
 

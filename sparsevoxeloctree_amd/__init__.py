@@ -14,6 +14,7 @@ from .api import (  # noqa: F401
     Octree,
     OctreeBuilder,
     OctreeTracer,
+    PathTracer,
     Scene,
     Server,
     SvoError,

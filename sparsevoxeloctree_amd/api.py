@@ -413,6 +413,67 @@ class OctreeTracer:
         return self.beam.download(np.float32, bw * bh).reshape(bh, bw)
 
 
+class PathTracer:
+    """Progressive path tracing with a constant sky (PathTracer, src/PathTracer.cpp:176-215;
+    ConstColorPathTrace, shader/path_tracer.comp:32-60): each CmdRender adds one sample per pixel to
+    the running means of colour, first-hit albedo and first-hit normal (float4 per pixel)."""
+
+    def __init__(self, octree, camera):
+        self.octree, self.camera = octree, camera
+        self.ctx = octree.ctx
+        self.m_width, self.m_height = 1280, 720
+        self.m_bounce = 4  # kDefaultBounce, src/Config.hpp:25
+        self.m_sun_radiance = (5.0, 5.0, 5.0)  # kDefaultConstantColor, src/Config.hpp:28 (uConstColor)
+        self.seed = 0x5EED
+        self.spp = 0
+        self.color = self.albedo = self.normal = None
+
+    @staticmethod
+    def Create(octree, camera):
+        return PathTracer(octree, camera)
+
+    def Resize(self, width, height):
+        self.m_width, self.m_height = int(width), int(height)
+        self.color = None
+        self.spp = 0
+
+    def Reset(self):
+        """Starts the means again (the reference clears its target images and its Sobol state)."""
+        self.spp = 0
+
+    def _ensure(self):
+        n = self.m_width * self.m_height * 16
+        if self.color is None or self.color.nbytes != n:
+            self.color, self.albedo, self.normal = (self.ctx.alloc(n) for _ in range(3))
+            self.spp = 0
+
+    def CmdRender(self):
+        self._ensure()
+        cam = self.camera.uniform() if isinstance(self.camera, Camera) else self.camera
+        sky = (C.c_float * 3)(*self.m_sun_radiance)
+        self.ctx.check(
+            lib.svo_path_trace(self.ctx.h, self.octree.h, C.byref(cam), self.m_width, self.m_height, self.m_bounce,
+                               self.spp, sky, self.seed, self.color.ptr, self.albedo.ptr, self.normal.ptr),
+            "path trace",
+        )
+        self.spp += 1
+
+    def GetSampleCount(self):
+        return self.spp
+
+    def _get(self, buf):
+        return buf.download(np.float32, self.m_width * self.m_height * 4).reshape(self.m_height, self.m_width, 4)
+
+    def GetColor(self):
+        return self._get(self.color)
+
+    def GetAlbedo(self):
+        return self._get(self.albedo)
+
+    def GetNormal(self):
+        return self._get(self.normal)
+
+
 def march_rays(ctx, octree, rays, variant=_capi.SVO_MARCH_LEAF_ITER):
     """Marches explicit rays (R, 6) float32 in octree space; returns hit records."""
     rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 6)

@@ -324,7 +324,7 @@ int svo_ctx_destroy(svo_ctx *ctx) {
 		(void)hipStreamSynchronize(ctx->stream);
 	ctx->prof.reset();
 	for (StageTimer *t : {&ctx->t_normalize, &ctx->t_count, &ctx->t_fill, &ctx->t_build, &ctx->t_build2, &ctx->t_beam, &ctx->t_march,
-	                      &ctx->t_rays, &ctx->t_bounce})
+	                      &ctx->t_rays, &ctx->t_bounce, &ctx->t_path})
 		t->destroy();
 	if (ctx->d_scan_parts)
 		(void)hipFree(ctx->d_scan_parts);
@@ -993,6 +993,21 @@ int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t 
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
 	DeviceGuard g(ctx);
 	return bounce_rays(ctx, hits_dev, R, seed, rays_dev);
+}
+
+int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
+                   uint32_t bounce, uint32_t spp, const float sky[3], uint64_t seed, float *color_dev,
+                   float *albedo_dev, float *normal_dev) {
+	if (!ctx || !oct || !cam || !sky || !color_dev || width == 0 || height == 0)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (bounce < 1 || bounce > 64)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce must be 1..64");
+	if ((uint64_t)width * height > 0xFFFFFFFFull || spp == 0xFFFFFFFFu)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame or sample count too large");
+	if (!oct->words || oct->W < 8)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "octree is empty");
+	DeviceGuard g(ctx);
+	return path_trace(ctx, oct, cam, width, height, bounce, spp, sky, seed, color_dev, albedo_dev, normal_dev);
 }
 
 // ---- batched tracing

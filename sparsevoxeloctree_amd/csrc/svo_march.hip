@@ -693,6 +693,113 @@ __global__ void __launch_bounds__(kBlock)
 	r[5] = dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z;
 }
 
+// ---- progressive path tracing (ConstColorPathTrace + accumulation, shader/path_tracer.comp:32-60,
+// 111-132) --------------------------------------------------------------------------------------
+// lowbias32 integer hash (C. Wellons' hash-prospector).
+__device__ __forceinline__ uint32_t hash32(uint32_t x) {
+	x ^= x >> 16;
+	x *= 0x7feb352du;
+	x ^= x >> 15;
+	x *= 0x846ca68bu;
+	x ^= x >> 16;
+	return x;
+}
+
+// Diffuse_Sample + AlignDirection (shader/diffuse.glsl:7-14, shader/util.glsl:5-9) around the axis
+// normal of face code `code` (axis * 2 + positive), for the sample (u, v).
+__device__ __forceinline__ f3 diffuse_dir(uint32_t code, float u, float v, f3 &nrm) {
+	const float kPi = 3.1415926535897932384626433832795f; // shader/util.glsl:4
+	const float sg = (code & 1u) ? 1.0f : -1.0f;
+	nrm = f3{code >> 1 == 0u ? sg : 0.0f, code >> 1 == 1u ? sg : 0.0f, code >> 1 == 2u ? sg : 0.0f};
+	float rr = __builtin_sqrtf(u), phi = 2.0f * kPi * v;
+	float sn, cs;
+	sincos_poly(phi, sn, cs);
+	f3 dl{rr * cs, rr * sn, __builtin_sqrtf(1.0f - u)};
+	f3 ax = fabsf(nrm.x) > 0.01f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
+	f3 uu = normalize3(cross3(ax, nrm));
+	f3 vv = cross3(nrm, uu);
+	return f3{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
+	          dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+}
+
+struct PathArgs {
+	CamArgs cam;
+	uint32_t width, height, bounce, spp;
+	float sky[3];
+	uint32_t s0, s1, s2, s3; // sampling keys from the seed
+};
+
+// One sample per pixel of a progressive render: the camera ray through pixel + the frame's 2-D Sobol
+// point (the pixel corner at spp 0), then up to bounce-1 cosine-weighted diffuse bounces; a path
+// that escapes brings the constant sky colour times the product of the albedos it met.  Sampling
+// replaces the reference's running Sobol dimensions and 256^2 blue-noise table: bounce `cur` uses
+// the frame's 2-D Sobol point (index spp) XOR-shifted by a hash of cur, rotated (fract of the sum)
+// by a per-pixel hash, all in 32-bit integers.  The colour, first-hit albedo and first-hit normal
+// are running means over spp + 1 samples (float4 per pixel, w = 1).
+__global__ void __launch_bounds__(64)
+    path_trace_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a, uint32_t tiles_x, float4 *color,
+                      float4 *albedo, float4 *normal, uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * 64];
+	const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x, lane = threadIdx.x;
+	const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+	if (px >= a.width || py >= a.height)
+		return;
+	uint32_t err = 0u;
+	const uint32_t k = py * a.width + px;
+	const float jx = (float)(sobol0(a.spp) >> 8) * 5.9604644775390625e-08f,
+	            jy = (float)(sobol1(a.spp) >> 8) * 5.9604644775390625e-08f;
+	f3 o{a.cam.px, a.cam.py, a.cam.pz};
+	f3 d = gen_ray(a.cam, ((float)px + jx) / (float)a.width, ((float)py + jy) / (float)a.height);
+	uint4 h0, h1;
+	march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
+	f3 rad{a.sky[0], a.sky[1], a.sky[2]}, alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
+	if (h1.y & 0x80000000u) {
+		const float k255 = 255.0f; // unpackUnorm4x8: c / 255
+		f3 acc{(float)(h1.z & 0xffu) / k255, (float)((h1.z >> 8) & 0xffu) / k255,
+		       (float)((h1.z >> 16) & 0xffu) / k255};
+		alb0 = acc;
+		uint32_t code = (h1.y >> 24) & 7u;
+		{
+			const float sg = (code & 1u) ? 1.0f : -1.0f;
+			nrm0 = f3{code >> 1 == 0u ? sg : 0.0f, code >> 1 == 1u ? sg : 0.0f, code >> 1 == 2u ? sg : 0.0f};
+		}
+		rad = f3{0.0f, 0.0f, 0.0f};
+		const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3); // per-pixel rotation
+		for (uint32_t cur = 1; cur < a.bounce; ++cur) {
+			const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
+			const float u = (float)(su >> 8) * 5.9604644775390625e-08f, v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+			f3 nrm;
+			o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
+			d = diffuse_dir(code, u, v, nrm);
+			march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
+			if (!(h1.y & 0x80000000u)) {
+				rad = f3{acc.x * a.sky[0], acc.y * a.sky[1], acc.z * a.sky[2]};
+				break;
+			}
+			acc = f3{acc.x * ((float)(h1.z & 0xffu) / k255), acc.y * ((float)((h1.z >> 8) & 0xffu) / k255),
+			         acc.z * ((float)((h1.z >> 16) & 0xffu) / k255)};
+			code = (h1.y >> 24) & 7u;
+		}
+	}
+	// running means (shader/path_tracer.comp:126-133); spp 0 starts them (the reference clears first)
+	const float n = (float)a.spp, n1 = (float)(a.spp + 1u);
+	auto mean = [&](float4 *img, f3 v) {
+		if (!img)
+			return;
+		if (a.spp == 0u) {
+			img[k] = make_float4(v.x, v.y, v.z, 1.0f);
+			return;
+		}
+		const float4 p = img[k];
+		img[k] = make_float4((p.x * n + v.x) / n1, (p.y * n + v.y) / n1, (p.z * n + v.z) / n1, 1.0f);
+	};
+	mean(color, rad);
+	mean(albedo, alb0);
+	mean(normal, nrm0);
+	if (err)
+		atomicOr(err_out, err);
+}
+
 CamArgs cam_args(const svo_camera *c) {
 	return CamArgs{c->pos[0],  c->pos[1],  c->pos[2],  c->look[0], c->look[1], c->look[2],
 	               c->side[0], c->side[1], c->side[2], c->up[0],   c->up[1],   c->up[2]};
@@ -893,6 +1000,24 @@ int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, fl
 	ctx->t_bounce.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "bounce");
+	return SVO_OK;
+}
+
+// One progressive sample per pixel (PathTracer::CmdRender, src/PathTracer.cpp:196-215).
+int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t bounce,
+               uint32_t spp, const float sky[3], uint64_t seed, float *color, float *albedo, float *normal) {
+	uint64_t st = seed;
+	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
+	PathArgs a{cam_args(cam), w, h, bounce, spp, {sky[0], sky[1], sky[2]},
+	           (uint32_t)z0, (uint32_t)(z0 >> 32), (uint32_t)z1, (uint32_t)(z1 >> 32)};
+	const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
+	ctx->t_path.start(ctx->stream);
+	hipError_t e = launch(ctx, "path_trace", path_trace_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
+	                      (const uint32_t *)oct->words, oct->W, a, tiles_x, reinterpret_cast<float4 *>(color),
+	                      reinterpret_cast<float4 *>(albedo), reinterpret_cast<float4 *>(normal), ctx->d_err);
+	ctx->t_path.stop(ctx->stream);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "path_trace");
 	return SVO_OK;
 }
 

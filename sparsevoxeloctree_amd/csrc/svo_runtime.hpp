@@ -125,7 +125,7 @@ struct svo_ctx {
 	svo::BuildState *d_state = nullptr;
 	uint32_t *d_err = nullptr;
 	svo::BuildState *h_state = nullptr; // pinned mirror for read-back
-	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_build2, t_beam, t_march, t_rays, t_bounce;
+	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_build2, t_beam, t_march, t_rays, t_bounce, t_path;
 	svo::Staging stage;
 	int cu_count = 256;
 	// launch-shape autotuner: off by default; when on, a batched launch of a new shape class times
@@ -207,6 +207,8 @@ int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
                uint32_t *nfetch);
 int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits);
 int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays);
+int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t bounce,
+               uint32_t spp, const float sky[3], uint64_t seed, float *color, float *albedo, float *normal);
 int copy_device(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 // Device-wide exclusive scan of n (read from the device) uint32 counts; see svo_scan.hpp.

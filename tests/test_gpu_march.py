@@ -205,3 +205,32 @@ def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
         for k, (a, b) in enumerate(zip(outs[0], outs[other])):
             ok, msg = hits_equal(a, b)
             assert ok, "mode %d case %d: %s" % (other, k, msg)
+
+
+@pytest.mark.parametrize("bounce", [1, 2, 4])
+def test_path_trace_matches_oracle(ctx, no_device_errors, scene_octree, bounce):
+    # progressive path tracing (PathTracer::CmdRender): three samples per pixel accumulated in
+    # place, colour, first-hit albedo and normal bit-exact against the oracle after every sample
+    import sparsevoxeloctree_amd as S
+
+    octree, words, level, _ = scene_octree
+    W, H = 99, 61  # partial 8x8 tiles on both edges
+    cam = cameras()[1]
+    sky = (5.0, 4.0, 3.0)
+    pt = S.PathTracer.Create(octree, _camera_struct(cam))
+    pt.Resize(W, H)
+    pt.m_bounce = bounce
+    pt.m_sun_radiance = sky
+    pt.seed = 0x5EED0C0DE
+    ref = [np.zeros((W * H, 4), np.float32) for _ in range(3)]
+    for s in range(3):
+        pt.CmdRender()
+        oracle.path_trace(words, level, cam, W, H, bounce, s, np.array(sky, np.float32), pt.seed, *ref)
+        got = [pt.GetColor(), pt.GetAlbedo(), pt.GetNormal()]
+        for g, r, name in zip(got, ref, ("colour", "albedo", "normal")):
+            assert np.array_equal(g.reshape(-1, 4), r), "%s after sample %d" % (name, s)
+    assert pt.GetSampleCount() == 3
+    pt.Reset()
+    pt.CmdRender()  # a restart overwrites the means
+    first = oracle.path_trace(words, level, cam, W, H, bounce, 0, np.array(sky, np.float32), pt.seed)
+    assert np.array_equal(pt.GetColor().reshape(-1, 4), first[0])

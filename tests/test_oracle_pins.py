@@ -341,3 +341,83 @@ def test_bounce_directions_cosine_hemisphere():
     cosn = d[np.arange(6000), axis] * sign
     assert np.all(cosn >= 0.0)
     assert abs(cosn.mean() - 2.0 / 3.0) < 0.02  # E[cos] of a cosine-weighted hemisphere
+
+
+# ---- progressive path tracer (ConstColorPathTrace + running means) ---------------------------------
+def _pt_cam():
+    import sparsevoxeloctree_amd as S
+
+    return S.camera_look_at((2.7, 2.2, 0.4), (1.5, 1.5, 1.5), fov=math.pi / 3, aspect=1.5).uniform_array()
+
+
+def test_path_trace_empty_octree_is_the_sky():
+    sky = np.array([5.0, 2.5, 0.75], np.float32)
+    words = np.zeros(8, np.uint32)
+    col, alb, nrm = oracle.path_trace(words, 1, _pt_cam(), 24, 16, 4, 0, sky, 7)
+    assert np.array_equal(col[:, :3], np.tile(sky, (24 * 16, 1)))
+    assert not alb[:, :3].any() and not nrm[:, :3].any()
+    assert np.all(col[:, 3] == 1.0)
+
+
+def _solid_cube(level, colour):
+    res = 1 << level
+    g = np.arange(res)
+    x, y, z = np.meshgrid(g, g, g, indexing="ij")
+    fr = pack_frags(x.ravel(), y.ravel(), z.ravel(), np.full(res**3, colour))
+    return oracle.build(fr, level)
+
+
+@pytest.mark.parametrize("bounce", [1, 2, 5])
+def test_path_trace_solid_cube_closed_form(bounce):
+    # one colour everywhere: a hit pixel's sample is 0 (no path escaped) or a^k * sky after k
+    # albedo products, 1 <= k < bounce.  (Some bounces re-enter the voxel they leave: the marcher's
+    # EPS offset, 2^-48, is below the float spacing at octree coordinates in [1, 2], so o_pos lies
+    # on the hit face, octree.glsl:490-495; both sides reproduce that.)  Misses are the sky.
+    colour = 0x40A0C8
+    words = _solid_cube(3, colour)
+    sky = np.array([5.0, 4.0, 3.0], np.float32)
+    W, H = 40, 28
+    col, alb, nrm = oracle.path_trace(words, 3, _pt_cam(), W, H, bounce, 0, sky, 11)
+    hits = oracle.march_frame(words, 3, _pt_cam(), W, H)
+    hit = hit_mask(hits).reshape(-1)
+    assert hit.any() and (~hit).any()
+    a = np.array([(colour >> (8 * c)) & 0xFF for c in range(3)], np.float32) / np.float32(255)
+    allowed, acc = [np.zeros(3, np.float32)], a.copy()
+    for k in range(1, bounce):
+        allowed.append(acc * sky)
+        acc = acc * a
+    got = col[hit, :3]
+    match = np.zeros(len(got), bool)
+    for v in allowed:
+        match |= np.all(got == v, axis=1)
+    assert match.all()
+    if bounce > 1:
+        assert np.any(np.all(got == allowed[1], axis=1))  # paths that escape after one bounce
+    else:
+        assert not got.any()
+    assert np.array_equal(col[~hit, :3], np.tile(sky, ((~hit).sum(), 1)))
+    assert np.array_equal(alb[hit, :3], np.tile(a, (hit.sum(), 1)))
+    n = nrm[hit, :3]
+    assert np.all(np.abs(n).sum(axis=1) == 1.0) and np.all(np.isin(n, (-1.0, 0.0, 1.0)))
+
+
+def test_path_trace_running_mean():
+    # spp frames accumulated in place equal the mean of the frames' own samples
+    xyz, rgb = _ptscene()
+    words = oracle.build(oracle.voxelize(oracle.normalize(xyz), rgb, 6), 6)
+    sky = np.array([1.0, 1.0, 1.0], np.float32)
+    W, H, n = 32, 24, 5
+    acc = np.zeros((W * H, 4), np.float32)
+    singles = []
+    for s in range(n):
+        oracle.path_trace(words, 6, _pt_cam(), W, H, 4, s, sky, 3, color=acc)
+        one, _, _ = oracle.path_trace(words, 6, _pt_cam(), W, H, 4, s, sky, 3, color=np.zeros((W * H, 4), np.float32))
+        singles.append(one[:, :3].astype(np.float64) * (s + 1))  # (0 * s + sample) / (s + 1)
+    assert np.allclose(acc[:, :3], np.mean(singles, axis=0), rtol=1e-5, atol=1e-6)
+    assert np.any(np.std(singles, axis=0) > 0)  # the samples differ between frames
+
+
+def _ptscene():
+    import sparsevoxeloctree_amd as S
+
+    return S.gen_scene(1, 0x5EED0101, 6)
