@@ -756,6 +756,7 @@ def main():
     if cfg["bounce"]:
         line["bounce_rays"] = nb
         line["bounce_mrays_per_s"] = round(nb / (st["rays_ms"] * 1e-3) / 1e6, 3)
+        line["path_trace"] = path_trace_rate(S, ctx, octree, cam, cfg)
     mem = ctx.memory()
     line["peak_pool_bytes"] = mem["peak_in_use"]
     line["autotune"] = S.autotune(ctx, True).strip().splitlines()
@@ -777,6 +778,25 @@ def main():
             "ray_queries_per_s", "builds_rejected", "ray_queries_failed", "evictions", "capacity"))
     print(json.dumps(line))
     sys.stdout.flush()
+
+
+def path_trace_rate(S, ctx, octree, cam, cfg, bounce=4, samples=8):
+    """Progressive path tracing of the same frame (PathTracer::CmdRender, 4 bounces, constant sky):
+    full samples per pixel timed on the host clock around synchronised launches."""
+    pt = S.PathTracer.Create(octree, cam)
+    pt.Resize(cfg["width"], cfg["height"])
+    pt.m_bounce = bounce
+    for _ in range(2):
+        pt.CmdRender()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(samples):
+        pt.CmdRender()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / samples
+    paths = cfg["width"] * cfg["height"]
+    return {"frame": "%dx%d" % (cfg["width"], cfg["height"]), "bounce": bounce, "samples": samples,
+            "ms_per_sample": round(dt * 1e3, 4), "mpaths_per_s": round(paths / dt / 1e6, 2)}
 
 
 def summarise_extra(fn, args, cfg, keys):
