@@ -122,3 +122,21 @@ def test_batch_and_server_argument_checks():
     assert lib.svo_march_batch(None, None, None, 0, 8, 8, 0, None) == 1
     assert lib.svo_march_plan_destroy(None) == 1
     assert lib.svo_march_plan_set_shape(None, 0) == 1
+
+
+def test_path_trace_and_obj_argument_checks():
+    # argument errors before any device work (no GPU needed)
+    import ctypes
+
+    from sparsevoxeloctree_amd._capi import lib
+
+    sky = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
+    assert lib.svo_path_trace(None, None, None, 8, 8, 4, 0, sky, 1, None, None, None) == 1
+    xyz = ctypes.POINTER(ctypes.c_float)()
+    rgb = ctypes.POINTER(ctypes.c_uint32)()
+    T = ctypes.c_uint64()
+    assert lib.svo_obj_load(None, ctypes.byref(xyz), ctypes.byref(rgb), ctypes.byref(T), None, 0) == 1
+    msg = ctypes.create_string_buffer(128)
+    assert lib.svo_obj_load(b"/nonexistent/x.obj", ctypes.byref(xyz), ctypes.byref(rgb), ctypes.byref(T), msg, 128) == 1
+    assert b"cannot read" in msg.value and not xyz and not rgb and T.value == 0
+    lib.svo_obj_free(None, None)  # freeing nothing is allowed
