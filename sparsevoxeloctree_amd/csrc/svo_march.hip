@@ -727,7 +727,50 @@ struct PathArgs {
 	uint32_t width, height, bounce, spp;
 	float sky[3];
 	uint32_t s0, s1, s2, s3; // sampling keys from the seed
+	uint32_t tiles_x;
+	float4 *color, *albedo, *normal;
 };
+
+// The steps of one path, shared by both schedules so that they give the same bits.
+// Primary ray of pixel (px, py): through the pixel plus the frame's 2-D Sobol point.
+__device__ __forceinline__ void path_primary(const PathArgs &a, uint32_t px, uint32_t py, f3 &o, f3 &d) {
+	const float jx = (float)(sobol0(a.spp) >> 8) * 5.9604644775390625e-08f,
+	            jy = (float)(sobol1(a.spp) >> 8) * 5.9604644775390625e-08f;
+	o = f3{a.cam.px, a.cam.py, a.cam.pz};
+	d = gen_ray(a.cam, ((float)px + jx) / (float)a.width, ((float)py + jy) / (float)a.height);
+}
+// Bounce cur >= 1 of pixel k from a hit record: origin o_pos, cosine-weighted direction about the
+// hit face's normal.
+__device__ __forceinline__ void path_bounce(const PathArgs &a, uint32_t k, uint32_t cur, uint4 h0, uint32_t code, f3 &o,
+                                            f3 &d) {
+	const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3); // per-pixel rotation
+	const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
+	const float u = (float)(su >> 8) * 5.9604644775390625e-08f, v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+	f3 nrm;
+	o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
+	d = diffuse_dir(code, u, v, nrm);
+}
+__device__ __forceinline__ f3 leaf_albedo(uint32_t leaf) { // unpackUnorm4x8: c / 255
+	const float k255 = 255.0f;
+	return f3{(float)(leaf & 0xffu) / k255, (float)((leaf >> 8) & 0xffu) / k255, (float)((leaf >> 16) & 0xffu) / k255};
+}
+__device__ __forceinline__ f3 face_normal(uint32_t code) {
+	const float sg = (code & 1u) ? 1.0f : -1.0f;
+	return f3{code >> 1 == 0u ? sg : 0.0f, code >> 1 == 1u ? sg : 0.0f, code >> 1 == 2u ? sg : 0.0f};
+}
+// Running mean of one pixel (shader/path_tracer.comp:126-133); spp 0 starts it (the reference
+// clears its images first).
+__device__ __forceinline__ void path_mean(const PathArgs &a, float4 *img, uint32_t k, f3 v) {
+	if (!img)
+		return;
+	if (a.spp == 0u) {
+		img[k] = make_float4(v.x, v.y, v.z, 1.0f);
+		return;
+	}
+	const float n = (float)a.spp, n1 = (float)(a.spp + 1u);
+	const float4 p = img[k];
+	img[k] = make_float4((p.x * n + v.x) / n1, (p.y * n + v.y) / n1, (p.z * n + v.z) / n1, 1.0f);
+}
 
 // One sample per pixel of a progressive render: the camera ray through pixel + the frame's 2-D Sobol
 // point (the pixel corner at spp 0), then up to bounce-1 cosine-weighted diffuse bounces; a path
@@ -735,67 +778,141 @@ struct PathArgs {
 // replaces the reference's running Sobol dimensions and 256^2 blue-noise table: bounce `cur` uses
 // the frame's 2-D Sobol point (index spp) XOR-shifted by a hash of cur, rotated (fract of the sum)
 // by a per-pixel hash, all in 32-bit integers.  The colour, first-hit albedo and first-hit normal
-// are running means over spp + 1 samples (float4 per pixel, w = 1).
-__global__ void __launch_bounds__(64)
-    path_trace_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a, uint32_t tiles_x, float4 *color,
-                      float4 *albedo, float4 *normal, uint32_t *err_out) {
+// are running means over spp + 1 samples (float4 per pixel, w = 1).  This schedule: one path per
+// lane, 8x8-pixel tiles.
+__global__ void __launch_bounds__(64) path_trace_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a,
+                                                        uint32_t *err_out) {
 	__shared__ uint32_t s_stack[kStackSlots * 64];
-	const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x, lane = threadIdx.x;
+	const uint32_t tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x, lane = threadIdx.x;
 	const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
 	if (px >= a.width || py >= a.height)
 		return;
 	uint32_t err = 0u;
 	const uint32_t k = py * a.width + px;
-	const float jx = (float)(sobol0(a.spp) >> 8) * 5.9604644775390625e-08f,
-	            jy = (float)(sobol1(a.spp) >> 8) * 5.9604644775390625e-08f;
-	f3 o{a.cam.px, a.cam.py, a.cam.pz};
-	f3 d = gen_ray(a.cam, ((float)px + jx) / (float)a.width, ((float)py + jy) / (float)a.height);
+	f3 o, d;
+	path_primary(a, px, py, o, d);
 	uint4 h0, h1;
 	march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
 	f3 rad{a.sky[0], a.sky[1], a.sky[2]}, alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
 	if (h1.y & 0x80000000u) {
-		const float k255 = 255.0f; // unpackUnorm4x8: c / 255
-		f3 acc{(float)(h1.z & 0xffu) / k255, (float)((h1.z >> 8) & 0xffu) / k255,
-		       (float)((h1.z >> 16) & 0xffu) / k255};
+		f3 acc = leaf_albedo(h1.z);
 		alb0 = acc;
 		uint32_t code = (h1.y >> 24) & 7u;
-		{
-			const float sg = (code & 1u) ? 1.0f : -1.0f;
-			nrm0 = f3{code >> 1 == 0u ? sg : 0.0f, code >> 1 == 1u ? sg : 0.0f, code >> 1 == 2u ? sg : 0.0f};
-		}
+		nrm0 = face_normal(code);
 		rad = f3{0.0f, 0.0f, 0.0f};
-		const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3); // per-pixel rotation
 		for (uint32_t cur = 1; cur < a.bounce; ++cur) {
-			const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
-			const float u = (float)(su >> 8) * 5.9604644775390625e-08f, v = (float)(sv >> 8) * 5.9604644775390625e-08f;
-			f3 nrm;
-			o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
-			d = diffuse_dir(code, u, v, nrm);
+			path_bounce(a, k, cur, h0, code, o, d);
 			march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
 			if (!(h1.y & 0x80000000u)) {
 				rad = f3{acc.x * a.sky[0], acc.y * a.sky[1], acc.z * a.sky[2]};
 				break;
 			}
-			acc = f3{acc.x * ((float)(h1.z & 0xffu) / k255), acc.y * ((float)((h1.z >> 8) & 0xffu) / k255),
-			         acc.z * ((float)((h1.z >> 16) & 0xffu) / k255)};
+			const f3 al = leaf_albedo(h1.z);
+			acc = f3{acc.x * al.x, acc.y * al.y, acc.z * al.z};
 			code = (h1.y >> 24) & 7u;
 		}
 	}
-	// running means (shader/path_tracer.comp:126-133); spp 0 starts them (the reference clears first)
-	const float n = (float)a.spp, n1 = (float)(a.spp + 1u);
-	auto mean = [&](float4 *img, f3 v) {
-		if (!img)
-			return;
-		if (a.spp == 0u) {
-			img[k] = make_float4(v.x, v.y, v.z, 1.0f);
-			return;
+	path_mean(a, a.color, k, rad);
+	path_mean(a, a.albedo, k, alb0);
+	path_mean(a, a.normal, k, nrm0);
+	if (err)
+		atomicOr(err_out, err);
+}
+
+// The same paths on a persistent grid (march_persist's refill scheme): a lane whose path ends writes
+// its pixel and takes the next pixel of its wave's pool; a path that goes on starts its next ray
+// in place.  Pixels are numbered in 8x8 tiles and dealt to the waves in 64-pixel chunks.
+template <uint32_t kT>
+__global__ void __launch_bounds__(kT) path_persist(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a, uint64_t n,
+                                                   uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * kT];
+	uint32_t *stk = s_stack + threadIdx.x;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t wave = (uint64_t)blockIdx.x * (kT / 64u) + (threadIdx.x >> 6);
+	const uint64_t nwaves = (uint64_t)gridDim.x * (kT / 64u);
+	const uint64_t below = (1ull << lane) - 1ull;
+	March m;
+	f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f}, acc{0.0f, 0.0f, 0.0f};
+	uint32_t k = 0u, cur = 0u, iter = 0u, nfetch = 0u, err = 0u;
+	bool active = false;
+	uint64_t pool_next = 0, pool_end = 0, chunk = wave;
+	bool exhausted = false;
+	for (;;) {
+		const uint64_t idle = __ballot(!active);
+		const uint32_t nidle = (uint32_t)__popcll(idle);
+		if (nidle >= kRefill && (!exhausted || pool_next < pool_end)) {
+			if (pool_next == pool_end) {
+				const uint64_t lo = chunk * kChunk;
+				if (lo < n) {
+					pool_next = lo;
+					pool_end = lo + kChunk < n ? lo + kChunk : n;
+					chunk += nwaves;
+				} else {
+					exhausted = true;
+				}
+			}
+			if (pool_next < pool_end) {
+				if (!active) {
+					const uint64_t q = pool_next + (uint32_t)__popcll(idle & below);
+					const uint32_t tile = (uint32_t)(q >> 6), r = (uint32_t)q & 63u;
+					const uint32_t px = (tile % a.tiles_x) * 8u + (r & 7u), py = (tile / a.tiles_x) * 8u + (r >> 3);
+					if (q < pool_end && px < a.width && py < a.height) {
+						k = py * a.width + px;
+						cur = 0u;
+						path_primary(a, px, py, o, d);
+						march_init<true>(o, d, m);
+						iter = nfetch = 0u;
+						active = true;
+					}
+				}
+				pool_next = pool_next + nidle < pool_end ? pool_next + nidle : pool_end;
+			}
 		}
-		const float4 p = img[k];
-		img[k] = make_float4((p.x * n + v.x) / n1, (p.y * n + v.y) / n1, (p.z * n + v.z) / n1, 1.0f);
-	};
-	mean(color, rad);
-	mean(albedo, alb0);
-	mean(normal, nrm0);
+		if (__ballot(active) == 0ull) {
+			if (exhausted && pool_next == pool_end)
+				break;
+			continue;
+		}
+		bool marching = active;
+#pragma unroll 1
+		for (uint32_t t = 0; t < kTrips; ++t)
+			if (marching)
+				marching = march_trip<kT>(oct, W, m, stk, iter, nfetch, err);
+		if (active && !marching) {
+			uint4 h0, h1;
+			march_finish<1>(m, o, d, iter, nfetch, h0, h1);
+			const bool hit = (h1.y & 0x80000000u) != 0u;
+			bool done = true;
+			f3 rad{0.0f, 0.0f, 0.0f};
+			if (cur == 0u) {
+				f3 alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
+				if (hit) {
+					acc = alb0 = leaf_albedo(h1.z);
+					nrm0 = face_normal((h1.y >> 24) & 7u);
+					done = a.bounce <= 1u;
+				} else {
+					rad = f3{a.sky[0], a.sky[1], a.sky[2]};
+				}
+				path_mean(a, a.albedo, k, alb0);
+				path_mean(a, a.normal, k, nrm0);
+			} else if (!hit) {
+				rad = f3{acc.x * a.sky[0], acc.y * a.sky[1], acc.z * a.sky[2]};
+			} else {
+				const f3 al = leaf_albedo(h1.z);
+				acc = f3{acc.x * al.x, acc.y * al.y, acc.z * al.z};
+				done = cur + 1u >= a.bounce;
+			}
+			if (done) {
+				path_mean(a, a.color, k, rad);
+				active = false;
+			} else { // the next bounce of the same path
+				++cur;
+				path_bounce(a, k, cur, h0, (h1.y >> 24) & 7u, o, d);
+				march_init<true>(o, d, m);
+				iter = nfetch = 0u;
+			}
+		}
+	}
 	if (err)
 		atomicOr(err_out, err);
 }
@@ -1003,18 +1120,62 @@ int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, fl
 	return SVO_OK;
 }
 
-// One progressive sample per pixel (PathTracer::CmdRender, src/PathTracer.cpp:196-215).
+// One progressive sample per pixel (PathTracer::CmdRender, src/PathTracer.cpp:196-215): one path
+// per lane in 8x8 tiles (schedule 0) or persistent waves with per-lane refill (schedule 1),
+// autotuned like the marches; the results are identical.
 int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t bounce,
                uint32_t spp, const float sky[3], uint64_t seed, float *color, float *albedo, float *normal) {
 	uint64_t st = seed;
 	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
-	PathArgs a{cam_args(cam), w, h, bounce, spp, {sky[0], sky[1], sky[2]},
-	           (uint32_t)z0, (uint32_t)(z0 >> 32), (uint32_t)z1, (uint32_t)(z1 >> 32)};
 	const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
+	const PathArgs a{cam_args(cam),
+	                 w,
+	                 h,
+	                 bounce,
+	                 spp,
+	                 {sky[0], sky[1], sky[2]},
+	                 (uint32_t)z0,
+	                 (uint32_t)(z0 >> 32),
+	                 (uint32_t)z1,
+	                 (uint32_t)(z1 >> 32),
+	                 tiles_x,
+	                 reinterpret_cast<float4 *>(color),
+	                 reinterpret_cast<float4 *>(albedo),
+	                 reinterpret_cast<float4 *>(normal)};
+	auto run_with = [&](const PathArgs &pa, int mode) -> hipError_t {
+		if (mode == 1) {
+			const uint64_t n = (uint64_t)tiles_x * tiles_y * 64u;
+			uint64_t grid = (n + kBlock - 1) / kBlock;
+			const uint64_t cap = (uint64_t)ctx->cu_count * 4u; // 99 VGPRs: 4 waves per SIMD
+			grid = grid < cap ? grid : cap;
+			return launch(ctx, "path_trace", path_persist<kBlock>, dim3((uint32_t)(grid ? grid : 1)), dim3(kBlock), 0,
+			              (const uint32_t *)oct->words, oct->W, pa, n, ctx->d_err);
+		}
+		return launch(ctx, "path_trace", path_trace_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
+		              (const uint32_t *)oct->words, oct->W, pa, ctx->d_err);
+	};
+	int mode = ctx->march_mode == 1 ? 1 : 0;
+	if (ctx->march_mode < 0 && ctx->autotune) {
+		char key[96];
+		std::snprintf(key, sizeof key, "path_trace %ux%u level=%u bounce=%u", w, h, oct->level, bounce);
+		if (ctx->tune_choice.find(key) == ctx->tune_choice.end()) {
+			// the trials render sample 0 into scratch, so the caller's running means see one sample
+			float4 *scratch = nullptr;
+			if (ctx->pool->alloc((void **)&scratch, sizeof(float4) * (uint64_t)w * h) != hipSuccess)
+				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "path tracer tuning buffer");
+			PathArgs trial = a;
+			trial.spp = 0u;
+			trial.color = trial.albedo = trial.normal = scratch;
+			const int rc = tune(ctx, key, 2, kScheduleName, [&](int m) { return run_with(trial, m); }, mode);
+			ctx->pool->free(scratch); // stream-ordered reuse
+			if (rc)
+				return rc;
+		} else {
+			mode = ctx->tune_choice[key];
+		}
+	}
 	ctx->t_path.start(ctx->stream);
-	hipError_t e = launch(ctx, "path_trace", path_trace_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
-	                      (const uint32_t *)oct->words, oct->W, a, tiles_x, reinterpret_cast<float4 *>(color),
-	                      reinterpret_cast<float4 *>(albedo), reinterpret_cast<float4 *>(normal), ctx->d_err);
+	hipError_t e = run_with(a, mode);
 	ctx->t_path.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "path_trace");

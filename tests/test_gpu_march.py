@@ -207,10 +207,23 @@ def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
             assert ok, "mode %d case %d: %s" % (other, k, msg)
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("bounce", [1, 2, 4])
-def test_path_trace_matches_oracle(ctx, no_device_errors, scene_octree, bounce):
-    # progressive path tracing (PathTracer::CmdRender): three samples per pixel accumulated in
-    # place, colour, first-hit albedo and normal bit-exact against the oracle after every sample
+def test_path_trace_matches_oracle(ctx, no_device_errors, scene_octree, bounce, mode):
+    # progressive path tracing (PathTracer::CmdRender), one path per lane (mode 0) or persistent
+    # waves with per-lane refill (mode 1): three samples per pixel accumulated in place, colour,
+    # first-hit albedo and normal bit-exact against the oracle after every sample
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    ctx.check(lib.svo_march_mode(ctx.h, mode))
+    try:
+        _path_trace_case(scene_octree, bounce)
+    finally:
+        ctx.check(lib.svo_march_mode(ctx.h, -1))
+
+
+def _path_trace_case(scene_octree, bounce):
     import sparsevoxeloctree_amd as S
 
     octree, words, level, _ = scene_octree
