@@ -37,7 +37,7 @@ enum DeviceError : uint32_t {
 
 class Pool {
 public:
-	explicit Pool(int device) : m_device(device) {}
+	explicit Pool(int /*device: allocations follow the caller's current device*/) {}
 	~Pool();
 	hipError_t alloc(void **p, uint64_t bytes);
 	void free(void *p);
@@ -48,7 +48,6 @@ public:
 
 private:
 	static uint64_t round_size(uint64_t bytes);
-	int m_device;
 	std::multimap<uint64_t, void *> m_free;
 	std::unordered_map<void *, uint64_t> m_live;
 	uint64_t m_in_use = 0, m_cached = 0, m_peak = 0;
