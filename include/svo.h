@@ -287,6 +287,26 @@ int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, u
                    uint32_t bounce, uint32_t spp, const float sky[3], uint64_t seed, float *color_dev,
                    float *albedo_dev, float *normal_dev);
 
+/* ---- environment-map light (EnvironmentMap, src/EnvironmentMap.cpp:55-190; EnvMapPathTrace,
+ * shader/path_tracer.comp:62-106).  svo_envmap_create: width x height texels of linear RGB (4
+ * floats each, the 4th ignored, row 0 = the +y pole), as an equirectangular map.  The texel pdf
+ * (luminance * sin(theta), mean 1) and the alias table are built on the host, as the reference does,
+ * and kept in HBM.  A map with zero luminance everywhere is rejected.  svo_envmap_download returns
+ * the device tables: texels (4 floats: rgb, pdf) and alias ({prob, alias} per texel); either may be
+ * NULL. */
+typedef struct svo_envmap svo_envmap;
+int svo_envmap_create(svo_ctx *ctx, uint32_t width, uint32_t height, const float *rgba, svo_envmap **out);
+int svo_envmap_download(svo_ctx *ctx, const svo_envmap *env, float *texels, uint32_t *alias);
+int svo_envmap_destroy(svo_envmap *env);
+/* svo_path_trace with the environment-map light: a path that misses at once takes the map's
+ * radiance; every bounce adds a light sample (alias-table texel, occlusion ray) and, when its diffuse
+ * ray escapes, the map's radiance, both weighted by the power heuristic and scaled by multiplier;
+ * rotation turns the map about +y (radians).  One path per lane; same buffers and sampling as
+ * svo_path_trace.  Texture lookups are bilinear with repeat addressing, in fp32. */
+int svo_path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
+                       uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
+                       uint64_t seed, float *color_dev, float *albedo_dev, float *normal_dev);
+
 /* ---- OBJ + MTL files (Scene::load_meshes, src/Scene.cpp:36-147, over tinyobj::LoadObj): the
  * triangle soup for svo_scene_create.  *xyz: 9 floats per triangle, the file's positions (not yet
  * normalised: pass normalize = 1 to svo_scene_create, as Scene::Create does); *rgb: per triangle,

@@ -48,6 +48,13 @@ def load():
             "orc_bounce_rays": (None, [P, u64, u64, P, P]),
             "orc_sincos": (None, [f32, P, P]),
             "orc_path_trace": (None, [P, u64, u32, P, u32, u32, u32, u32, P, u64, P, P, P, i32]),
+            "orc_envmap_tables": (i32, [u32, u32, P, P, P]),
+            "orc_path_trace_env": (None, [P, u64, u32, P, u32, u32, u32, u32, P, P, u32, u32, f32, f32, u64, P, P, P,
+                                          i32]),
+            "orc_env_sample": (None, [P, P, u32, u32, f32, f32, f32, P, P, P]),
+            "orc_env_lookup": (None, [P, u32, u32, f32, P, P, P]),
+            "orc_atan2": (f32, [f32, f32]),
+            "orc_acos": (f32, [f32]),
             "orc_stack_uninit_reads": (u64, []),
             "orc_reset_counters": (None, []),
         }
@@ -182,6 +189,59 @@ def path_trace(words, level, cam, width, height, bounce, spp, sky, seed, color=N
     load().orc_path_trace(_p(words), words.shape[0], level, _p(cam), width, height, bounce, spp, _p(sky), seed,
                        _p(bufs[0]), _p(bufs[1]), _p(bufs[2]), threads or nthreads())
     return tuple(bufs)
+
+
+def envmap_tables(rgba):
+    """(texels (H, W, 4) rgb + pdf, alias (H, W, 2) uint32) for an (H, W, 4) float32 map, or None."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[:2]
+    t = np.zeros((h, w, 4), np.float32)
+    a = np.zeros((h, w, 2), np.uint32)
+    if not load().orc_envmap_tables(w, h, _p(rgba), _p(t), _p(a)):
+        return None
+    return t, a
+
+
+def path_trace_env(words, level, cam, width, height, bounce, spp, texels, alias, rotation, multiplier, seed,
+                   color=None, albedo=None, normal=None, threads=None):
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    texels = np.ascontiguousarray(texels, dtype=np.float32)
+    alias = np.ascontiguousarray(alias, dtype=np.uint32)
+    eh, ew = texels.shape[:2]
+    bufs = [np.zeros((width * height, 4), np.float32) if b is None else b for b in (color, albedo, normal)]
+    load().orc_path_trace_env(_p(words), words.shape[0], level, _p(cam), width, height, bounce, spp, _p(texels),
+                              _p(alias), ew, eh, rotation, multiplier, seed, _p(bufs[0]), _p(bufs[1]), _p(bufs[2]),
+                              threads or nthreads())
+    return tuple(bufs)
+
+
+def env_sample(texels, alias, rotation, u, v):
+    """(direction, radiance, pdf) of EnvMap_Sample at the 2-D point (u, v)."""
+    texels = np.ascontiguousarray(texels, dtype=np.float32)
+    alias = np.ascontiguousarray(alias, dtype=np.uint32)
+    h, w = texels.shape[:2]
+    d, l, p = np.empty(3, np.float32), np.empty(3, np.float32), np.empty(1, np.float32)
+    load().orc_env_sample(_p(texels), _p(alias), w, h, rotation, u, v, _p(d), _p(l), _p(p))
+    return d, l, p[0]
+
+
+def env_lookup(texels, rotation, d):
+    """(rgb + pdf texel filter, EnvMap_PDF) of one direction."""
+    texels = np.ascontiguousarray(texels, dtype=np.float32)
+    h, w = texels.shape[:2]
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    out, p = np.empty(4, np.float32), np.empty(1, np.float32)
+    load().orc_env_lookup(_p(texels), w, h, rotation, _p(d), _p(out), _p(p))
+    return out, p[0]
+
+
+def atan2(y, x):
+    return load().orc_atan2(y, x)
+
+
+def acos(x):
+    return load().orc_acos(x)
 
 
 def sincos(phi):

@@ -890,6 +890,251 @@ void orc_path_trace(const uint32_t *words, uint64_t W, uint32_t level, const flo
 	});
 }
 
+// ---- environment-map light (EnvMapPathTrace, shader/path_tracer.comp:62-106) ----------------------
+// weigh_hdr_image + generate_alias_table (src/EnvironmentMap.cpp:104-188); false without light.
+int orc_envmap_tables(uint32_t w, uint32_t h, const float *rgba, float *texels, uint32_t *alias) {
+	const uint64_t n = (uint64_t)w * h;
+	std::vector<double> wt(n);
+	double sum = 0.0;
+	for (uint32_t j = 0; j < h; ++j) {
+		const double sin_theta = std::sin(M_PI * ((double)j + 0.5) / (double)h);
+		for (uint32_t i = 0; i < w; ++i) {
+			const float *c = rgba + 4 * ((uint64_t)j * w + i);
+			wt[(uint64_t)j * w + i] = (double)((0.2126f * c[0] + 0.7152f * c[1]) + 0.0722f * c[2]) * sin_theta;
+			sum += wt[(uint64_t)j * w + i];
+		}
+	}
+	if (!(sum > 0.0))
+		return 0;
+	const double mul = (double)n / sum;
+	for (uint64_t i = 0; i < n; ++i) {
+		wt[i] *= mul;
+		for (int c = 0; c < 3; ++c)
+			texels[4 * i + c] = rgba[4 * i + c];
+		texels[4 * i + 3] = (float)wt[i];
+	}
+	std::vector<uint32_t> small, large; // the reference's two stacks
+	for (uint64_t i = 0; i < n; ++i)
+		(wt[i] < 1.0 ? small : large).push_back((uint32_t)i);
+	for (uint64_t i = 0; i < 2 * n; ++i)
+		alias[i] = 0u;
+	while (!small.empty() && !large.empty()) {
+		const uint32_t s = small.back(), l = large.back();
+		small.pop_back();
+		const double p = std::min(std::max(wt[s] * 4294967296.0, 0.0), 4294967295.0);
+		alias[2 * s] = (uint32_t)p;
+		alias[2 * s + 1] = l;
+		wt[l] -= 1.0 - wt[s];
+		if (wt[l] < 1.0) {
+			large.pop_back();
+			small.push_back(l);
+		}
+	}
+	for (uint32_t l : large)
+		alias[2 * l] = alias[2 * l + 1] = 0xFFFFFFFFu;
+	for (uint32_t t : small)
+		alias[2 * t] = alias[2 * t + 1] = 0xFFFFFFFFu;
+	return 1;
+}
+
+// atan on [0, 1] by pi/4 + atan((t-1)/(t+1)) above tan(pi/8) and a fitted odd polynomial; the
+// product's device function performs the same operations.
+static float atan01(float t) {
+	const bool big = t > 0.41421356f;
+	const float x = big ? (t - 1.0f) / (t + 1.0f) : t, z = x * x;
+	float p = -0x1.0a311cp-4f;
+	p = p * z + 0x1.b8b1c6p-4f;
+	p = p * z + -0x1.24286cp-3f;
+	p = p * z + 0x1.999774p-3f;
+	p = p * z + -0x1.555554p-2f;
+	const float r = x + (x * z) * p;
+	return big ? 0x1.921fb6p-1f + r : r;
+}
+static float atan2_poly(float y, float x) {
+	const float ax = std::fabs(x), ay = std::fabs(y), mx = gmax(ax, ay), mn = gmin(ax, ay);
+	float r = atan01(mx > 0.0f ? mn / mx : 0.0f);
+	r = ay > ax ? 0x1.921fb6p+0f - r : r;
+	r = x < 0.0f ? 0x1.921fb6p+1f - r : r;
+	return y < 0.0f ? -r : r;
+}
+static float acos_poly(float x) { return atan2_poly(std::sqrt(gmax(1.0f - x * x, 0.0f)), x); }
+float orc_atan2(float y, float x) { return atan2_poly(y, x); }
+float orc_acos(float x) { return acos_poly(x); }
+
+namespace {
+struct Env {
+	const float *tex; // 4 per texel
+	const uint32_t *alias;
+	uint32_t w, h;
+	float rot, mult;
+	void texture(float u, float v, float out[4]) const { // bilinear, repeat
+		const float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+		const float fx = std::floor(x), fy = std::floor(y), a = x - fx, b = y - fy;
+		auto wrap = [](float f, uint32_t n) -> uint32_t {
+			if (!(f > -1e9f && f < 1e9f))
+				return 0u;
+			const int64_t i = (int64_t)f % (int64_t)n;
+			return (uint32_t)(i < 0 ? i + n : i);
+		};
+		const uint32_t i0 = wrap(fx, w), i1 = wrap(fx + 1.0f, w), j0 = wrap(fy, h), j1 = wrap(fy + 1.0f, h);
+		const float *t00 = tex + 4 * ((uint64_t)j0 * w + i0), *t10 = tex + 4 * ((uint64_t)j0 * w + i1);
+		const float *t01 = tex + 4 * ((uint64_t)j1 * w + i0), *t11 = tex + 4 * ((uint64_t)j1 * w + i1);
+		const float a1 = 1.0f - a, b1 = 1.0f - b;
+		for (int c = 0; c < 4; ++c)
+			out[c] = (t00[c] * a1 + t10[c] * a) * b1 + (t01[c] * a1 + t11[c] * a) * b;
+	}
+	void lookup(V3 d, float out[4]) const {
+		const float kPi = 0x1.921fb6p+1f;
+		texture(((atan2_poly(d.x, d.z) + rot) / kPi) * 0.5f, acos_poly(d.y) / kPi, out);
+	}
+	float pdf(V3 d) const {
+		const float kTwoPiPi = (2.0f * 0x1.921fb6p+1f) * 0x1.921fb6p+1f;
+		float t[4];
+		lookup(d, t);
+		return t[3] / (kTwoPiPi * std::sqrt(1.0f - d.y * d.y));
+	}
+	V3 sample(float su, float sv, V3 &light, float &p) const {
+		const float kPi = 0x1.921fb6p+1f, kTwoPiPi = (2.0f * kPi) * kPi;
+		const uint32_t sz = w * h;
+		uint32_t idx = (uint32_t)(su * (float)sz);
+		idx = idx < sz - 1u ? idx : sz - 1u;
+		const uint32_t prob = (uint32_t)(sv * 4294967296.0f);
+		const uint32_t q = prob > alias[2 * idx] ? alias[2 * idx + 1] : idx;
+		const float ux = ((float)(q % w) + 0.5f) / (float)w, uy = ((float)(q / w) + 0.5f) / (float)h;
+		const float phi = (2.0f * kPi) * ux - rot, theta = kPi * uy;
+		float st, ct, sp, cp;
+		sincos_poly(theta, st, ct);
+		sincos_poly(phi, sp, cp);
+		const float *t = tex + 4 * (uint64_t)q;
+		light = V3{t[0], t[1], t[2]};
+		p = t[3] / (kTwoPiPi * st);
+		return V3{st * sp, ct, st * cp};
+	}
+};
+} // namespace
+
+void orc_env_sample(const float *texels, const uint32_t *alias, uint32_t w, uint32_t h, float rotation, float u,
+                    float v, float *dir, float *light, float *pdf) {
+	const Env env{texels, alias, w, h, rotation, 1.0f};
+	V3 l;
+	const V3 d = env.sample(u, v, l, *pdf);
+	dir[0] = d.x, dir[1] = d.y, dir[2] = d.z;
+	light[0] = l.x, light[1] = l.y, light[2] = l.z;
+}
+void orc_env_lookup(const float *texels, uint32_t w, uint32_t h, float rotation, const float *dir, float *out,
+                    float *pdf) {
+	const Env env{texels, nullptr, w, h, rotation, 1.0f};
+	const V3 d{dir[0], dir[1], dir[2]};
+	env.lookup(d, out);
+	*pdf = env.pdf(d);
+}
+
+void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                        uint32_t height, uint32_t bounce, uint32_t spp, const float *texels, const uint32_t *alias,
+                        uint32_t ew, uint32_t eh, float rotation, float multiplier, uint64_t seed, float *color,
+                        float *albedo, float *normal, int nthreads) {
+	(void)level;
+	Octree oc{words, W};
+	const Env env{texels, alias, ew, eh, rotation, multiplier};
+	uint64_t st = seed;
+	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
+	const uint32_t s0 = (uint32_t)z0, s1 = (uint32_t)(z0 >> 32), s2 = (uint32_t)z1, s3 = (uint32_t)(z1 >> 32);
+	const float kPi = 0x1.921fb6p+1f, kBsdf = 1.0f / kPi;
+	const float jx = (float)(sobol0(spp) >> 8) * 5.9604644775390625e-08f,
+	            jy = (float)(sobol1(spp) >> 8) * 5.9604644775390625e-08f;
+	auto colour = [](uint32_t w, int c) { return (float)((w >> (8 * c)) & 0xffu) / 255.0f; };
+	auto face = [](uint32_t code) {
+		V3 n{0.0f, 0.0f, 0.0f};
+		const float sg = (code & 1u) ? 1.0f : -1.0f;
+		(code >> 1 == 0 ? n.x : code >> 1 == 1 ? n.y : n.z) = sg;
+		return n;
+	};
+	auto mis = [](float a, float b) {
+		const float t = a * a;
+		return t / (b * b + t);
+	};
+	parallel_for((uint64_t)width * height, nthreads, [&](uint64_t b, uint64_t e) {
+		for (uint64_t kk = b; kk < e; ++kk) {
+			const uint32_t k = (uint32_t)kk, px = k % width, py = k / width;
+			V3 o = cam_vec(cam, 0);
+			V3 d = gen_ray(cam, ((float)px + jx) / (float)width, ((float)py + jy) / (float)height);
+			orc_hit h = march_leaf(oc, o, d, 1);
+			V3 rad{0.0f, 0.0f, 0.0f}, alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
+			if (!(h.vz_flags & 0x80000000u)) {
+				float t[4];
+				env.lookup(d, t);
+				rad = V3{t[0], t[1], t[2]};
+			} else {
+				V3 acc{colour(h.leaf, 0), colour(h.leaf, 1), colour(h.leaf, 2)};
+				alb0 = acc;
+				uint32_t code = (h.vz_flags >> 24) & 7u;
+				nrm0 = face(code);
+				const uint32_t na = hash32(k ^ s2), nb = hash32(na ^ s3);
+				for (uint32_t cur = 1; cur < bounce; ++cur) {
+					const uint32_t su = (sobol0(spp) ^ hash32(s0 ^ cur)) + na, sv = (sobol1(spp) ^ hash32(s1 ^ cur)) + nb;
+					const float u = (float)(su >> 8) * 5.9604644775390625e-08f,
+					            v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+					o = V3{h.pos[0], h.pos[1], h.pos[2]};
+					const V3 nrm = face(code);
+					{
+						V3 light;
+						float lpdf;
+						const V3 dl = env.sample(u, v, light, lpdf);
+						const float ndd = (nrm.x * dl.x + nrm.y * dl.y) + nrm.z * dl.z;
+						if (ndd > 0.0f && !std::isnan(lpdf) && lpdf > 0.0f &&
+						    !(march_leaf(oc, o, dl, 2).vz_flags & 0x80000000u)) {
+							const float bpdf = ndd / kPi, wgt = mis(lpdf, bpdf);
+							rad = V3{rad.x + ((((acc.x * light.x) * multiplier) * wgt) * ndd) * kBsdf / lpdf,
+							         rad.y + ((((acc.y * light.y) * multiplier) * wgt) * ndd) * kBsdf / lpdf,
+							         rad.z + ((((acc.z * light.z) * multiplier) * wgt) * ndd) * kBsdf / lpdf};
+						}
+					}
+					// Diffuse_Sample + AlignDirection
+					float rr = std::sqrt(u), phi = 2.0f * kPi * v;
+					float sn, cs;
+					sincos_poly(phi, sn, cs);
+					V3 dl{rr * cs, rr * sn, std::sqrt(1.0f - u)};
+					V3 ax = std::fabs(nrm.x) > 0.01f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+					V3 uu = normalize(cross(ax, nrm));
+					V3 vv = cross(nrm, uu);
+					d = V3{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
+					       dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+					const float bpdf = std::sqrt(1.0f - u) / kPi;
+					h = march_leaf(oc, o, d, 1);
+					if (!(h.vz_flags & 0x80000000u)) {
+						const float lpdf = env.pdf(d);
+						if (!std::isnan(lpdf) && lpdf > 0.0f) {
+							float t[4];
+							env.lookup(d, t);
+							const float wgt = mis(bpdf, lpdf);
+							rad = V3{rad.x + ((acc.x * t[0]) * multiplier) * wgt, rad.y + ((acc.y * t[1]) * multiplier) * wgt,
+							         rad.z + ((acc.z * t[2]) * multiplier) * wgt};
+						}
+						break;
+					}
+					acc = V3{acc.x * colour(h.leaf, 0), acc.y * colour(h.leaf, 1), acc.z * colour(h.leaf, 2)};
+					code = (h.vz_flags >> 24) & 7u;
+				}
+			}
+			const float n = (float)spp, n1 = (float)(spp + 1u);
+			auto mean = [&](float *img, V3 v) {
+				if (!img)
+					return;
+				float *p = img + 4 * (uint64_t)k;
+				if (spp == 0u) {
+					p[0] = v.x, p[1] = v.y, p[2] = v.z;
+				} else {
+					p[0] = (p[0] * n + v.x) / n1, p[1] = (p[1] * n + v.y) / n1, p[2] = (p[2] * n + v.z) / n1;
+				}
+				p[3] = 1.0f;
+			};
+			mean(color, rad);
+			mean(albedo, alb0);
+			mean(normal, nrm0);
+		}
+	});
+}
+
 uint64_t orc_stack_uninit_reads(void) { return g_uninit_reads.load(); }
 void orc_reset_counters(void) { g_uninit_reads.store(0); }
 

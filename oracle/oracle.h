@@ -45,6 +45,22 @@ void orc_path_trace(const uint32_t *words, uint64_t W, uint32_t level, const flo
                     uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, uint64_t seed, float *color,
                     float *albedo, float *normal, int nthreads);
 
+/* Environment-map tables (src/EnvironmentMap.cpp:104-188): texels 4 floats (rgb, pdf) and alias
+ * {prob, alias} per texel from rgba (4 floats per texel); 0 when the map has no light. */
+int orc_envmap_tables(uint32_t w, uint32_t h, const float *rgba, float *texels, uint32_t *alias);
+/* EnvMapPathTrace with the sampling of svo_path_trace_env; buffers as orc_path_trace. */
+void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
+                        uint32_t height, uint32_t bounce, uint32_t spp, const float *texels, const uint32_t *alias,
+                        uint32_t ew, uint32_t eh, float rotation, float multiplier, uint64_t seed, float *color,
+                        float *albedo, float *normal, int nthreads);
+/* EnvMap_Sample (direction, radiance, pdf) and EnvMap_Radiance + EnvMap_PDF of one direction. */
+void orc_env_sample(const float *texels, const uint32_t *alias, uint32_t w, uint32_t h, float rotation, float u,
+                    float v, float *dir, float *light, float *pdf);
+void orc_env_lookup(const float *texels, uint32_t w, uint32_t h, float rotation, const float *dir, float *out,
+                    float *pdf);
+float orc_atan2(float y, float x);
+float orc_acos(float x);
+
 /* Scene::load_meshes normalisation to [-1,1]^3 (src/Scene.cpp:62-99). In place on xyz[3*nverts]. */
 void orc_normalize(float *xyz, uint64_t nverts);
 

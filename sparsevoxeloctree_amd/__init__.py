@@ -10,6 +10,7 @@ from .api import (  # noqa: F401
     Camera,
     Context,
     DeviceArray,
+    EnvironmentMap,
     MarchPlan,
     Octree,
     OctreeBuilder,

@@ -171,6 +171,10 @@ _SIGS = {
     "svo_camera_from_yaw_pitch": (i32, [C.POINTER(C.c_float), f32, f32, f32, f32, C.POINTER(svo_camera)]),
     "svo_gen_scene": (i32, [i32, u64, u64, P, P, C.POINTER(u64)]),
     "svo_path_trace": (i32, [P, P, C.POINTER(svo_camera), u32, u32, u32, u32, C.POINTER(C.c_float), u64, P, P, P]),
+    "svo_envmap_create": (i32, [P, u32, u32, P, PP]),
+    "svo_envmap_download": (i32, [P, P, P, P]),
+    "svo_envmap_destroy": (i32, [P]),
+    "svo_path_trace_env": (i32, [P, P, C.POINTER(svo_camera), u32, u32, u32, u32, P, f32, f32, u64, P, P, P]),
     "svo_obj_load": (i32, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_uint32)),
                            C.POINTER(u64), C.c_char_p, u64]),
     "svo_obj_free": (None, [C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),

@@ -413,6 +413,48 @@ class OctreeTracer:
         return self.beam.download(np.float32, bw * bh).reshape(bh, bw)
 
 
+class EnvironmentMap:
+    """An equirectangular environment map in HBM with its sampling tables (EnvironmentMap,
+    src/EnvironmentMap.cpp:55-190): rgba (H, W, 4) or (H, W, 3) float32 linear radiance."""
+
+    def __init__(self, ctx, rgba):
+        rgba = np.asarray(rgba, dtype=np.float32)
+        if rgba.ndim != 3 or rgba.shape[2] not in (3, 4):
+            raise ValueError("environment map must be (H, W, 3|4)")
+        if rgba.shape[2] == 3:
+            rgba = np.concatenate([rgba, np.zeros(rgba.shape[:2] + (1,), np.float32)], axis=2)
+        rgba = np.ascontiguousarray(rgba)
+        self.ctx = ctx
+        self.m_height, self.m_width = rgba.shape[:2]
+        self.m_rotation = 0.0
+        self.m_multiplier = 1.0
+        self.h = C.c_void_p()
+        ctx.check(lib.svo_envmap_create(ctx.h, self.m_width, self.m_height, ptr(rgba), C.byref(self.h)), "envmap")
+
+    @staticmethod
+    def Create(ctx, rgba):
+        return EnvironmentMap(ctx, rgba)
+
+    def tables(self):
+        """(texels (H, W, 4): rgb + pdf, alias (H, W, 2) uint32 {prob, alias}) from the device."""
+        n = self.m_width * self.m_height
+        t = np.empty(4 * n, np.float32)
+        a = np.empty(2 * n, np.uint32)
+        self.ctx.check(lib.svo_envmap_download(self.ctx.h, self.h, ptr(t), ptr(a)), "envmap download")
+        return t.reshape(self.m_height, self.m_width, 4), a.reshape(self.m_height, self.m_width, 2)
+
+    def close(self):
+        if self.h:
+            lib.svo_envmap_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
 class PathTracer:
     """Progressive path tracing with a constant sky (PathTracer, src/PathTracer.cpp:176-215;
     ConstColorPathTrace, shader/path_tracer.comp:32-60): each CmdRender adds one sample per pixel to
@@ -427,6 +469,7 @@ class PathTracer:
         self.seed = 0x5EED
         self.spp = 0
         self.color = self.albedo = self.normal = None
+        self.m_env_map = None  # an EnvironmentMap: EnvMapPathTrace instead of the constant colour
 
     @staticmethod
     def Create(octree, camera):
@@ -450,12 +493,17 @@ class PathTracer:
     def CmdRender(self):
         self._ensure()
         cam = self.camera.uniform() if isinstance(self.camera, Camera) else self.camera
-        sky = (C.c_float * 3)(*self.m_sun_radiance)
-        self.ctx.check(
-            lib.svo_path_trace(self.ctx.h, self.octree.h, C.byref(cam), self.m_width, self.m_height, self.m_bounce,
-                               self.spp, sky, self.seed, self.color.ptr, self.albedo.ptr, self.normal.ptr),
-            "path trace",
-        )
+        env = self.m_env_map
+        if env is not None:
+            rc = lib.svo_path_trace_env(self.ctx.h, self.octree.h, C.byref(cam), self.m_width, self.m_height,
+                                        self.m_bounce, self.spp, env.h, env.m_rotation, env.m_multiplier, self.seed,
+                                        self.color.ptr, self.albedo.ptr, self.normal.ptr)
+        else:
+            sky = (C.c_float * 3)(*self.m_sun_radiance)
+            rc = lib.svo_path_trace(self.ctx.h, self.octree.h, C.byref(cam), self.m_width, self.m_height,
+                                    self.m_bounce, self.spp, sky, self.seed, self.color.ptr, self.albedo.ptr,
+                                    self.normal.ptr)
+        self.ctx.check(rc, "path trace")
         self.spp += 1
 
     def GetSampleCount(self):

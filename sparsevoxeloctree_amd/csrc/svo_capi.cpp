@@ -1010,6 +1010,141 @@ int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, u
 	return path_trace(ctx, oct, cam, width, height, bounce, spp, sky, seed, color_dev, albedo_dev, normal_dev);
 }
 
+// ---- environment maps (EnvironmentMap::Reset, src/EnvironmentMap.cpp:55-190) ----------------------
+namespace {
+// weigh_hdr_image (src/EnvironmentMap.cpp:104-128): luminance * sin(theta) per texel, normalised to
+// a mean of 1 (the texel's pdf, kept in alpha); generate_alias_table (:130-188): the two-stack
+// alias construction, probabilities as 32-bit fractions.  false if the map has no light.
+bool envmap_tables(uint32_t w, uint32_t h, const float *rgba, std::vector<float> &texels,
+                   std::vector<uint32_t> &alias) {
+	const uint64_t n = (uint64_t)w * h;
+	std::vector<double> wt(n);
+	double sum = 0.0;
+	for (uint32_t j = 0; j < h; ++j) {
+		const double sin_theta = std::sin(M_PI * ((double)j + 0.5) / (double)h);
+		for (uint32_t i = 0; i < w; ++i) {
+			const float *c = rgba + 4 * ((uint64_t)j * w + i);
+			const float lum = (0.2126f * c[0] + 0.7152f * c[1]) + 0.0722f * c[2];
+			wt[(uint64_t)j * w + i] = (double)lum * sin_theta;
+			sum += wt[(uint64_t)j * w + i];
+		}
+	}
+	if (!(sum > 0.0))
+		return false;
+	const double mul = (double)n / sum;
+	texels.resize(4 * n);
+	for (uint64_t i = 0; i < n; ++i) {
+		wt[i] *= mul;
+		texels[4 * i + 0] = rgba[4 * i + 0];
+		texels[4 * i + 1] = rgba[4 * i + 1];
+		texels[4 * i + 2] = rgba[4 * i + 2];
+		texels[4 * i + 3] = (float)wt[i];
+	}
+	alias.assign(2 * n, 0u);
+	std::vector<uint32_t> stack(n); // small entries grow from the front, large ones from the back
+	uint64_t small = 0, large = n;
+	for (uint64_t i = 0; i < n; ++i) {
+		if (wt[i] < 1.0)
+			stack[small++] = (uint32_t)i;
+		else
+			stack[--large] = (uint32_t)i;
+	}
+	while (small > 0 && large < n) {
+		const uint32_t s = stack[--small], l = stack[large];
+		const double p = wt[s] * 4294967296.0;
+		alias[2 * s] = (uint32_t)(p < 0.0 ? 0.0 : p > 4294967295.0 ? 4294967295.0 : p);
+		alias[2 * s + 1] = l;
+		wt[l] -= 1.0 - wt[s];
+		if (wt[l] < 1.0) {
+			++large;
+			stack[small++] = l;
+		}
+	}
+	while (large < n) {
+		const uint32_t l = stack[large++];
+		alias[2 * l] = alias[2 * l + 1] = 0xFFFFFFFFu;
+	}
+	while (small > 0) {
+		const uint32_t t = stack[--small];
+		alias[2 * t] = alias[2 * t + 1] = 0xFFFFFFFFu;
+	}
+	return true;
+}
+} // namespace
+
+int svo_envmap_create(svo_ctx *ctx, uint32_t width, uint32_t height, const float *rgba, svo_envmap **out) {
+	if (!ctx || !rgba || !out || width == 0 || height == 0)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	if ((uint64_t)width * height > 0xFFFFFFFFull / 16u)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "environment map too large");
+	std::vector<float> texels;
+	std::vector<uint32_t> alias;
+	if (!envmap_tables(width, height, rgba, texels, alias))
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "environment map has no light (zero luminance)");
+	DeviceGuard g(ctx);
+	svo_envmap *e = new (std::nothrow) svo_envmap();
+	if (!e)
+		return SVO_ERR_OUT_OF_MEMORY;
+	e->ctx = ctx;
+	e->width = width;
+	e->height = height;
+	if (ctx->pool->alloc((void **)&e->texels, sizeof(float) * texels.size()) != hipSuccess ||
+	    ctx->pool->alloc((void **)&e->alias, sizeof(uint32_t) * alias.size()) != hipSuccess) {
+		svo_envmap_destroy(e);
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "environment map allocation failed");
+	}
+	int rc = svo_upload(ctx, e->texels, texels.data(), sizeof(float) * texels.size());
+	if (!rc)
+		rc = svo_upload(ctx, e->alias, alias.data(), sizeof(uint32_t) * alias.size());
+	if (rc) {
+		svo_envmap_destroy(e);
+		return rc;
+	}
+	*out = e;
+	return SVO_OK;
+}
+
+int svo_envmap_download(svo_ctx *ctx, const svo_envmap *env, float *texels, uint32_t *alias) {
+	if (!ctx || !env || (!texels && !alias))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	const uint64_t n = (uint64_t)env->width * env->height;
+	int rc = texels ? svo_download(ctx, texels, env->texels, sizeof(float) * 4 * n) : SVO_OK;
+	if (!rc && alias)
+		rc = svo_download(ctx, alias, env->alias, sizeof(uint32_t) * 2 * n);
+	return rc;
+}
+
+int svo_envmap_destroy(svo_envmap *env) {
+	if (!env)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(env->ctx);
+	(void)hipStreamSynchronize(env->ctx->stream); // paths in flight may still read it
+	env->ctx->pool->free(env->texels);
+	env->ctx->pool->free(env->alias);
+	delete env;
+	return SVO_OK;
+}
+
+int svo_path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
+                       uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
+                       uint64_t seed, float *color_dev, float *albedo_dev, float *normal_dev) {
+	if (!ctx || !oct || !cam || !env || !color_dev || width == 0 || height == 0)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (bounce < 1 || bounce > 64)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce must be 1..64");
+	if ((uint64_t)width * height > 0xFFFFFFFFull || spp == 0xFFFFFFFFu)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame or sample count too large");
+	if (!oct->words || oct->W < 8)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "octree is empty");
+	if (env->ctx != ctx)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "environment map belongs to another context");
+	DeviceGuard g(ctx);
+	return path_trace_env(ctx, oct, cam, width, height, bounce, spp, env, rotation, multiplier, seed, color_dev,
+	                      albedo_dev, normal_dev);
+}
+
 // ---- batched tracing
 namespace {
 int check_frame_batch(svo_ctx *ctx, const svo_octree *const *octs, const svo_camera *cams, uint32_t n,

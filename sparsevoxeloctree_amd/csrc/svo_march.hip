@@ -819,6 +819,166 @@ __global__ void __launch_bounds__(64) path_trace_kernel(const uint32_t *__restri
 		atomicOr(err_out, err);
 }
 
+// ---- environment-map light (EnvMapPathTrace, shader/path_tracer.comp:62-106;
+// shader/environment_map.glsl; shader/mis.glsl) -----------------------------------------------
+// atan on [0, 1]: atan(t) = pi/4 + atan((t-1)/(t+1)) above tan(pi/8), then an odd polynomial
+// x + x^3 P(x^2) fitted (least squares, float coefficients) to atan on [0, tan(pi/8)]; max error
+// 9e-8.  Own function, identical on host and device, since libm and device math differ.
+__device__ __forceinline__ float atan01(float t) {
+	const bool big = t > 0.41421356f;
+	const float x = big ? (t - 1.0f) / (t + 1.0f) : t, z = x * x;
+	float p = -0x1.0a311cp-4f;
+	p = p * z + 0x1.b8b1c6p-4f;
+	p = p * z + -0x1.24286cp-3f;
+	p = p * z + 0x1.999774p-3f;
+	p = p * z + -0x1.555554p-2f;
+	const float r = x + (x * z) * p;
+	return big ? 0x1.921fb6p-1f + r : r;
+}
+__device__ __forceinline__ float atan2_poly(float y, float x) { // GLSL atan(y, x); atan2(0, 0) = 0
+	const float ax = fabsf(x), ay = fabsf(y), mx = gmax(ax, ay), mn = gmin(ax, ay);
+	float r = atan01(mx > 0.0f ? mn / mx : 0.0f);
+	r = ay > ax ? 0x1.921fb6p+0f - r : r;
+	r = x < 0.0f ? 0x1.921fb6p+1f - r : r;
+	return y < 0.0f ? -r : r;
+}
+__device__ __forceinline__ float acos_poly(float x) { return atan2_poly(__builtin_sqrtf(gmax(1.0f - x * x, 0.0f)), x); }
+
+struct EnvArgs {
+	const float4 *texels; // rgb + pdf
+	const uint2 *alias;   // {prob, alias}
+	uint32_t width, height;
+	float rotation, multiplier;
+};
+
+// texture(): bilinear, repeat addressing in both directions (src/EnvironmentMap.cpp:10), in fp32.
+__device__ __forceinline__ float4 env_texture(const EnvArgs &e, float u, float v) {
+	const float x = u * (float)e.width - 0.5f, y = v * (float)e.height - 0.5f;
+	const float fx = floorf(x), fy = floorf(y);
+	const float a = x - fx, b = y - fy;
+	auto wrap = [](float f, uint32_t n) -> uint32_t {
+		if (!(f > -1e9f && f < 1e9f))
+			return 0u; // not a finite coordinate
+		const int64_t i = (int64_t)f % (int64_t)n;
+		return (uint32_t)(i < 0 ? i + n : i);
+	};
+	const uint32_t i0 = wrap(fx, e.width), i1 = wrap(fx + 1.0f, e.width);
+	const uint32_t j0 = wrap(fy, e.height), j1 = wrap(fy + 1.0f, e.height);
+	const float4 t00 = e.texels[j0 * e.width + i0], t10 = e.texels[j0 * e.width + i1];
+	const float4 t01 = e.texels[j1 * e.width + i0], t11 = e.texels[j1 * e.width + i1];
+	const float a1 = 1.0f - a, b1 = 1.0f - b;
+	auto mix2 = [&](float p00, float p10, float p01, float p11) {
+		return (p00 * a1 + p10 * a) * b1 + (p01 * a1 + p11 * a) * b;
+	};
+	return make_float4(mix2(t00.x, t10.x, t01.x, t11.x), mix2(t00.y, t10.y, t01.y, t11.y),
+	                   mix2(t00.z, t10.z, t01.z, t11.z), mix2(t00.w, t10.w, t01.w, t11.w));
+}
+__device__ __forceinline__ float4 env_lookup(const EnvArgs &e, f3 d) { // EnvMap_Dir2Coord + texture
+	const float kPi = 0x1.921fb6p+1f;
+	const float u = ((atan2_poly(d.x, d.z) + e.rotation) / kPi) * 0.5f, v = acos_poly(d.y) / kPi;
+	return env_texture(e, u, v);
+}
+__device__ __forceinline__ float env_pdf(const EnvArgs &e, f3 d) { // EnvMap_PDF
+	const float kTwoPiPi = (2.0f * 0x1.921fb6p+1f) * 0x1.921fb6p+1f;
+	return env_lookup(e, d).w / (kTwoPiPi * __builtin_sqrtf(1.0f - d.y * d.y));
+}
+// EnvMap_Sample: texel by the alias table, its centre's direction, radiance and pdf.
+__device__ __forceinline__ f3 env_sample(const EnvArgs &e, float su, float sv, f3 &light, float &pdf) {
+	const float kPi = 0x1.921fb6p+1f, kTwoPiPi = (2.0f * kPi) * kPi;
+	const uint32_t sz = e.width * e.height;
+	uint32_t idx = (uint32_t)(su * (float)sz);
+	idx = idx < sz - 1u ? idx : sz - 1u;
+	const uint32_t prob = (uint32_t)(sv * 4294967296.0f);
+	const uint2 al = e.alias[idx];
+	const uint32_t p = prob > al.x ? al.y : idx;
+	const float ux = ((float)(p % e.width) + 0.5f) / (float)e.width, uy = ((float)(p / e.width) + 0.5f) / (float)e.height;
+	const float phi = (2.0f * kPi) * ux - e.rotation, theta = kPi * uy;
+	float st, ct, sp, cp;
+	sincos_poly(theta, st, ct);
+	sincos_poly(phi, sp, cp);
+	const float4 t = e.texels[p];
+	light = f3{t.x, t.y, t.z};
+	pdf = t.w / (kTwoPiPi * st);
+	return f3{st * sp, ct, st * cp};
+}
+__device__ __forceinline__ float mis_power(float a, float b) { // MIS_PowerHeuristic
+	const float t = a * a;
+	return t / (b * b + t);
+}
+
+// EnvMapPathTrace per pixel, one path per lane in 8x8 tiles (the same sampling as the constant-sky
+// paths; the light sample reuses the bounce's 2-D point, as the reference does).
+__global__ void __launch_bounds__(64) path_trace_env_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a,
+                                                            EnvArgs e, uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * 64];
+	const uint32_t tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x, lane = threadIdx.x;
+	const uint32_t px = tx * 8u + (lane & 7u), py = ty * 8u + (lane >> 3);
+	if (px >= a.width || py >= a.height)
+		return;
+	const float kPi = 0x1.921fb6p+1f, kBsdf = 1.0f / kPi;
+	uint32_t err = 0u;
+	const uint32_t k = py * a.width + px;
+	f3 o, d;
+	path_primary(a, px, py, o, d);
+	uint4 h0, h1;
+	march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
+	f3 rad{0.0f, 0.0f, 0.0f}, alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
+	if (!(h1.y & 0x80000000u)) {
+		const float4 t = env_lookup(e, d);
+		rad = f3{t.x, t.y, t.z};
+	} else {
+		f3 acc = leaf_albedo(h1.z);
+		alb0 = acc;
+		uint32_t code = (h1.y >> 24) & 7u;
+		nrm0 = face_normal(code);
+		const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3);
+		for (uint32_t cur = 1; cur < a.bounce; ++cur) {
+			const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
+			const float u = (float)(su >> 8) * 5.9604644775390625e-08f, v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+			o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
+			const f3 nrm = face_normal(code);
+			{ // light sample
+				f3 light;
+				float lpdf;
+				const f3 dl = env_sample(e, u, v, light, lpdf);
+				const float ndd = (nrm.x * dl.x + nrm.y * dl.y) + nrm.z * dl.z;
+				if (ndd > 0.0f && !__builtin_isnan(lpdf) && lpdf > 0.0f) {
+					uint4 q0, q1;
+					march_leaf<2, 64>(oct, W, o, dl, s_stack + lane, err, q0, q1);
+					if (!(q1.y & 0x80000000u)) {
+						const float bpdf = ndd / kPi, w = mis_power(lpdf, bpdf);
+						rad = f3{rad.x + ((((acc.x * light.x) * e.multiplier) * w) * ndd) * kBsdf / lpdf,
+						         rad.y + ((((acc.y * light.y) * e.multiplier) * w) * ndd) * kBsdf / lpdf,
+						         rad.z + ((((acc.z * light.z) * e.multiplier) * w) * ndd) * kBsdf / lpdf};
+					}
+				}
+			}
+			f3 n2;
+			d = diffuse_dir(code, u, v, n2);
+			const float bpdf = __builtin_sqrtf(1.0f - u) / kPi;
+			march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
+			if (!(h1.y & 0x80000000u)) {
+				const float lpdf = env_pdf(e, d);
+				if (!__builtin_isnan(lpdf) && lpdf > 0.0f) {
+					const float4 t = env_lookup(e, d);
+					const float w = mis_power(bpdf, lpdf);
+					rad = f3{rad.x + ((acc.x * t.x) * e.multiplier) * w, rad.y + ((acc.y * t.y) * e.multiplier) * w,
+					         rad.z + ((acc.z * t.z) * e.multiplier) * w};
+				}
+				break;
+			}
+			const f3 al = leaf_albedo(h1.z);
+			acc = f3{acc.x * al.x, acc.y * al.y, acc.z * al.z};
+			code = (h1.y >> 24) & 7u;
+		}
+	}
+	path_mean(a, a.color, k, rad);
+	path_mean(a, a.albedo, k, alb0);
+	path_mean(a, a.normal, k, nrm0);
+	if (err)
+		atomicOr(err_out, err);
+}
+
 // The same paths on a persistent grid (march_persist's refill scheme): a lane whose path ends writes
 // its pixel and takes the next pixel of its wave's pool; a path that goes on starts its next ray
 // in place.  Pixels are numbered in 8x8 tiles and dealt to the waves in 64-pixel chunks.
@@ -1179,6 +1339,27 @@ int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
 	ctx->t_path.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "path_trace");
+	return SVO_OK;
+}
+
+// One progressive sample per pixel with the environment-map light (EnvMapPathTrace).
+int path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
+                   uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
+                   uint64_t seed, float *color, float *albedo, float *normal) {
+	uint64_t st = seed;
+	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
+	const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
+	const PathArgs a{cam_args(cam), w, h, bounce, spp, {0.0f, 0.0f, 0.0f}, (uint32_t)z0, (uint32_t)(z0 >> 32),
+	                 (uint32_t)z1, (uint32_t)(z1 >> 32), tiles_x, reinterpret_cast<float4 *>(color),
+	                 reinterpret_cast<float4 *>(albedo), reinterpret_cast<float4 *>(normal)};
+	const EnvArgs e{reinterpret_cast<const float4 *>(env->texels), reinterpret_cast<const uint2 *>(env->alias),
+	                env->width, env->height, rotation, multiplier};
+	ctx->t_path.start(ctx->stream);
+	hipError_t err = launch(ctx, "path_trace_env", path_trace_env_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
+	                        (const uint32_t *)oct->words, oct->W, a, e, ctx->d_err);
+	ctx->t_path.stop(ctx->stream);
+	if (err != hipSuccess)
+		return hip_check(ctx, err, "path_trace_env");
 	return SVO_OK;
 }
 

@@ -168,6 +168,15 @@ struct svo_octree {
 	bool pending;            // build enqueued, not yet finished
 };
 
+// An environment map (EnvironmentMap, src/EnvironmentMap.cpp:57-70): texels rgb + sampling pdf
+// (the reference keeps the pdf in alpha) and the alias table, in HBM.
+struct svo_envmap {
+	svo_ctx *ctx;
+	uint32_t width, height;
+	float *texels;    // 4 floats per texel
+	uint32_t *alias;  // {prob, alias} per texel
+};
+
 namespace svo {
 
 // Shared helpers for the entry points.
@@ -208,6 +217,9 @@ int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t 
 int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays);
 int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t bounce,
                uint32_t spp, const float sky[3], uint64_t seed, float *color, float *albedo, float *normal);
+int path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
+                   uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
+                   uint64_t seed, float *color, float *albedo, float *normal);
 int copy_device(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 // Device-wide exclusive scan of n (read from the device) uint32 counts; see svo_scan.hpp.

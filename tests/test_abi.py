@@ -140,3 +140,9 @@ def test_path_trace_and_obj_argument_checks():
     assert lib.svo_obj_load(b"/nonexistent/x.obj", ctypes.byref(xyz), ctypes.byref(rgb), ctypes.byref(T), msg, 128) == 1
     assert b"cannot read" in msg.value and not xyz and not rgb and T.value == 0
     lib.svo_obj_free(None, None)  # freeing nothing is allowed
+    env = ctypes.c_void_p()
+    rgba = (ctypes.c_float * 32)()
+    assert lib.svo_envmap_create(None, 4, 2, rgba, ctypes.byref(env)) == 1 and not env
+    assert lib.svo_envmap_download(None, None, None, None) == 1
+    assert lib.svo_envmap_destroy(None) == 1
+    assert lib.svo_path_trace_env(None, None, None, 8, 8, 4, 0, None, 0.0, 1.0, 1, None, None, None) == 1
