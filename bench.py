@@ -757,6 +757,9 @@ def main():
         line["bounce_rays"] = nb
         line["bounce_mrays_per_s"] = round(nb / (st["rays_ms"] * 1e-3) / 1e6, 3)
         line["path_trace"] = path_trace_rate(S, ctx, octree, cam, cfg)
+        env = synthetic_envmap(S, ctx)
+        line["path_trace_env"] = dict(path_trace_rate(S, ctx, octree, cam, cfg, env=env), envmap="2048x1024")
+        env.close()
     mem = ctx.memory()
     line["peak_pool_bytes"] = mem["peak_in_use"]
     line["autotune"] = S.autotune(ctx, True).strip().splitlines()
@@ -780,12 +783,23 @@ def main():
     sys.stdout.flush()
 
 
-def path_trace_rate(S, ctx, octree, cam, cfg, bounce=4, samples=8):
-    """Progressive path tracing of the same frame (PathTracer::CmdRender, 4 bounces, constant sky):
-    full samples per pixel timed on the host clock around synchronised launches."""
+def synthetic_envmap(S, ctx, width=2048, height=1024, seed=0x5EED0E):
+    """An equirectangular sky of smooth random radiance with a small bright sun (float32 RGBA)."""
+    rng = np.random.default_rng(seed)
+    rgba = np.ones((height, width, 4), np.float32)
+    rgba[..., :3] = rng.uniform(0.3, 1.0, (height, width, 3))
+    rgba[height // 5:height // 5 + 4, width // 3:width // 3 + 4, :3] *= 500.0
+    return S.EnvironmentMap.Create(ctx, rgba)
+
+
+def path_trace_rate(S, ctx, octree, cam, cfg, bounce=4, samples=8, env=None):
+    """Progressive path tracing of the same frame (PathTracer::CmdRender, 4 bounces, constant sky or
+    the environment-map light): full samples per pixel timed on the host clock around synchronised
+    launches."""
     pt = S.PathTracer.Create(octree, cam)
     pt.Resize(cfg["width"], cfg["height"])
     pt.m_bounce = bounce
+    pt.m_env_map = env
     for _ in range(2):
         pt.CmdRender()
     ctx.synchronize()

@@ -908,7 +908,7 @@ __device__ __forceinline__ float mis_power(float a, float b) { // MIS_PowerHeuri
 
 // EnvMapPathTrace per pixel, one path per lane in 8x8 tiles (the same sampling as the constant-sky
 // paths; the light sample reuses the bounce's 2-D point, as the reference does).
-__global__ void __launch_bounds__(64) path_trace_env_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) path_trace_env_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a,
                                                             EnvArgs e, uint32_t *err_out) {
 	__shared__ uint32_t s_stack[kStackSlots * 64];
 	const uint32_t tx = blockIdx.x % a.tiles_x, ty = blockIdx.x / a.tiles_x, lane = threadIdx.x;
