@@ -1077,6 +1077,152 @@ __global__ void __launch_bounds__(kT) path_persist(const uint32_t *__restrict__ 
 		atomicOr(err_out, err);
 }
 
+// The environment-map paths on the persistent grid of path_persist.  A bounce takes up to two
+// marches of one lane: the occlusion ray towards the light sample (only when the sample faces the
+// surface; its weighted radiance waits in `pend` until the ray is known to escape), then the BSDF
+// ray.  Every value is computed with the operations of path_trace_env_kernel, in the same order.
+constexpr uint32_t kPersistEnvBlocksPerCu = 4; // 120 VGPRs: 4 waves of 64 per SIMD
+template <uint32_t kT>
+__global__ void __launch_bounds__(kT) path_persist_env(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a,
+                                                       EnvArgs e, uint64_t n, uint32_t *err_out) {
+	__shared__ uint32_t s_stack[kStackSlots * kT];
+	uint32_t *stk = s_stack + threadIdx.x;
+	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t wave = (uint64_t)blockIdx.x * (kT / 64u) + (threadIdx.x >> 6);
+	const uint64_t nwaves = (uint64_t)gridDim.x * (kT / 64u);
+	const uint64_t below = (1ull << lane) - 1ull;
+	const float kPi = 0x1.921fb6p+1f, kBsdf = 1.0f / kPi;
+	March m;
+	f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f}, acc{0.0f, 0.0f, 0.0f}, rad{0.0f, 0.0f, 0.0f}, pend{0.0f, 0.0f, 0.0f};
+	float u = 0.0f, v = 0.0f, bpdf = 0.0f;
+	uint32_t k = 0u, cur = 0u, code = 0u, iter = 0u, nfetch = 0u, err = 0u;
+	bool active = false, shadow = false;
+	uint64_t pool_next = 0, pool_end = 0, chunk = wave;
+	bool exhausted = false;
+	for (;;) {
+		const uint64_t idle = __ballot(!active);
+		const uint32_t nidle = (uint32_t)__popcll(idle);
+		if (nidle >= kRefill && (!exhausted || pool_next < pool_end)) {
+			if (pool_next == pool_end) {
+				const uint64_t lo = chunk * kChunk;
+				if (lo < n) {
+					pool_next = lo;
+					pool_end = lo + kChunk < n ? lo + kChunk : n;
+					chunk += nwaves;
+				} else {
+					exhausted = true;
+				}
+			}
+			if (pool_next < pool_end) {
+				if (!active) {
+					const uint64_t q = pool_next + (uint32_t)__popcll(idle & below);
+					const uint32_t tile = (uint32_t)(q >> 6), r = (uint32_t)q & 63u;
+					const uint32_t px = (tile % a.tiles_x) * 8u + (r & 7u), py = (tile / a.tiles_x) * 8u + (r >> 3);
+					if (q < pool_end && px < a.width && py < a.height) {
+						k = py * a.width + px;
+						cur = 0u;
+						shadow = false;
+						rad = f3{0.0f, 0.0f, 0.0f};
+						path_primary(a, px, py, o, d);
+						march_init<true>(o, d, m);
+						iter = nfetch = 0u;
+						active = true;
+					}
+				}
+				pool_next = pool_next + nidle < pool_end ? pool_next + nidle : pool_end;
+			}
+		}
+		if (__ballot(active) == 0ull) {
+			if (exhausted && pool_next == pool_end)
+				break;
+			continue;
+		}
+		bool marching = active;
+#pragma unroll 1
+		for (uint32_t t = 0; t < kTrips; ++t)
+			if (marching)
+				marching = march_trip<kT>(oct, W, m, stk, iter, nfetch, err);
+		if (!active || marching)
+			continue;
+		uint4 h0, h1;
+		if (shadow) { // occlusion ray done: keep the light term if it escaped, then the BSDF ray
+			march_finish<2>(m, o, d, iter, nfetch, h0, h1);
+			if (!(h1.y & 0x80000000u))
+				rad = f3{rad.x + pend.x, rad.y + pend.y, rad.z + pend.z};
+			shadow = false;
+			f3 n2;
+			d = diffuse_dir(code, u, v, n2);
+			bpdf = __builtin_sqrtf(1.0f - u) / kPi;
+			march_init<true>(o, d, m);
+			iter = nfetch = 0u;
+			continue;
+		}
+		march_finish<1>(m, o, d, iter, nfetch, h0, h1);
+		const bool hit = (h1.y & 0x80000000u) != 0u;
+		bool done = true;
+		if (cur == 0u) {
+			f3 alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
+			if (hit) {
+				acc = alb0 = leaf_albedo(h1.z);
+				code = (h1.y >> 24) & 7u;
+				nrm0 = face_normal(code);
+				done = a.bounce <= 1u;
+			} else {
+				const float4 t = env_lookup(e, d);
+				rad = f3{t.x, t.y, t.z};
+			}
+			path_mean(a, a.albedo, k, alb0);
+			path_mean(a, a.normal, k, nrm0);
+		} else if (!hit) {
+			const float lpdf = env_pdf(e, d);
+			if (!__builtin_isnan(lpdf) && lpdf > 0.0f) {
+				const float4 t = env_lookup(e, d);
+				const float w = mis_power(bpdf, lpdf);
+				rad = f3{rad.x + ((acc.x * t.x) * e.multiplier) * w, rad.y + ((acc.y * t.y) * e.multiplier) * w,
+				         rad.z + ((acc.z * t.z) * e.multiplier) * w};
+			}
+		} else {
+			const f3 al = leaf_albedo(h1.z);
+			acc = f3{acc.x * al.x, acc.y * al.y, acc.z * al.z};
+			code = (h1.y >> 24) & 7u;
+			done = cur + 1u >= a.bounce;
+		}
+		if (done) {
+			path_mean(a, a.color, k, rad);
+			active = false;
+			continue;
+		}
+		// bounce cur + 1 from this hit: the light sample's occlusion ray first when it faces the face
+		++cur;
+		const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3);
+		const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
+		u = (float)(su >> 8) * 5.9604644775390625e-08f;
+		v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+		o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
+		const f3 nrm = face_normal(code);
+		f3 light;
+		float lpdf;
+		const f3 dl = env_sample(e, u, v, light, lpdf);
+		const float ndd = (nrm.x * dl.x + nrm.y * dl.y) + nrm.z * dl.z;
+		if (ndd > 0.0f && !__builtin_isnan(lpdf) && lpdf > 0.0f) {
+			const float bp = ndd / kPi, w = mis_power(lpdf, bp);
+			pend = f3{((((acc.x * light.x) * e.multiplier) * w) * ndd) * kBsdf / lpdf,
+			          ((((acc.y * light.y) * e.multiplier) * w) * ndd) * kBsdf / lpdf,
+			          ((((acc.z * light.z) * e.multiplier) * w) * ndd) * kBsdf / lpdf};
+			d = dl;
+			shadow = true;
+		} else {
+			f3 n2;
+			d = diffuse_dir(code, u, v, n2);
+			bpdf = __builtin_sqrtf(1.0f - u) / kPi;
+		}
+		march_init<true>(o, d, m);
+		iter = nfetch = 0u;
+	}
+	if (err)
+		atomicOr(err_out, err);
+}
+
 CamArgs cam_args(const svo_camera *c) {
 	return CamArgs{c->pos[0],  c->pos[1],  c->pos[2],  c->look[0], c->look[1], c->look[2],
 	               c->side[0], c->side[1], c->side[2], c->up[0],   c->up[1],   c->up[2]};
@@ -1354,9 +1500,40 @@ int path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, u
 	                 reinterpret_cast<float4 *>(albedo), reinterpret_cast<float4 *>(normal)};
 	const EnvArgs e{reinterpret_cast<const float4 *>(env->texels), reinterpret_cast<const uint2 *>(env->alias),
 	                env->width, env->height, rotation, multiplier};
+	auto run_with = [&](const PathArgs &pa, int mode) -> hipError_t {
+		if (mode == 1) {
+			const uint64_t n = (uint64_t)tiles_x * tiles_y * 64u;
+			uint64_t grid = (n + kBlock - 1) / kBlock;
+			const uint64_t cap = (uint64_t)ctx->cu_count * kPersistEnvBlocksPerCu;
+			grid = grid < cap ? grid : cap;
+			return launch(ctx, "path_trace_env", path_persist_env<kBlock>, dim3((uint32_t)(grid ? grid : 1)),
+			              dim3(kBlock), 0, (const uint32_t *)oct->words, oct->W, pa, e, n, ctx->d_err);
+		}
+		return launch(ctx, "path_trace_env", path_trace_env_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
+		              (const uint32_t *)oct->words, oct->W, pa, e, ctx->d_err);
+	};
+	int mode = ctx->march_mode == 1 ? 1 : 0;
+	if (ctx->march_mode < 0 && ctx->autotune) {
+		char key[96];
+		std::snprintf(key, sizeof key, "path_trace_env %ux%u level=%u bounce=%u", w, h, oct->level, bounce);
+		if (ctx->tune_choice.find(key) == ctx->tune_choice.end()) {
+			// the trials render sample 0 into scratch, so the caller's running means see one sample
+			float4 *scratch = nullptr;
+			if (ctx->pool->alloc((void **)&scratch, sizeof(float4) * (uint64_t)w * h) != hipSuccess)
+				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "path tracer tuning buffer");
+			PathArgs trial = a;
+			trial.spp = 0u;
+			trial.color = trial.albedo = trial.normal = scratch;
+			const int rc = tune(ctx, key, 2, kScheduleName, [&](int md) { return run_with(trial, md); }, mode);
+			ctx->pool->free(scratch); // stream-ordered reuse
+			if (rc)
+				return rc;
+		} else {
+			mode = ctx->tune_choice[key];
+		}
+	}
 	ctx->t_path.start(ctx->stream);
-	hipError_t err = launch(ctx, "path_trace_env", path_trace_env_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
-	                        (const uint32_t *)oct->words, oct->W, a, e, ctx->d_err);
+	hipError_t err = run_with(a, mode);
 	ctx->t_path.stop(ctx->stream);
 	if (err != hipSuccess)
 		return hip_check(ctx, err, "path_trace_env");

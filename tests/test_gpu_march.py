@@ -257,11 +257,23 @@ def _env_rgba(h, w, seed):
     return rgba
 
 
+@pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("bounce", [1, 2, 4])
-def test_path_trace_env_matches_oracle(ctx, no_device_errors, scene_octree, bounce):
-    # EnvMapPathTrace (shader/path_tracer.comp:62-106): the environment map's tables built by the
-    # library equal the oracle's, and three accumulated samples of colour, albedo and normal are
-    # bit-exact against the oracle after every sample (light samples with occlusion rays, MIS)
+def test_path_trace_env_matches_oracle(ctx, no_device_errors, scene_octree, bounce, mode):
+    # EnvMapPathTrace (shader/path_tracer.comp:62-106), one path per lane (mode 0) or persistent
+    # waves (mode 1): the environment map's tables built by the library equal the oracle's, and
+    # three accumulated samples of colour, albedo and normal are bit-exact against the oracle after
+    # every sample (light samples with occlusion rays, MIS)
+    from sparsevoxeloctree_amd._capi import lib
+
+    ctx.check(lib.svo_march_mode(ctx.h, mode))
+    try:
+        _path_trace_env_case(ctx, scene_octree, bounce)
+    finally:
+        ctx.check(lib.svo_march_mode(ctx.h, -1))
+
+
+def _path_trace_env_case(ctx, scene_octree, bounce):
     import sparsevoxeloctree_amd as S
 
     octree, words, level, _ = scene_octree
