@@ -983,7 +983,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) pa
 // its pixel and takes the next pixel of its wave's pool; a path that goes on starts its next ray
 // in place.  Pixels are numbered in 8x8 tiles and dealt to the waves in 64-pixel chunks.
 template <uint32_t kT>
-__global__ void __launch_bounds__(kT) path_persist(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a, uint64_t n,
+__global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) path_persist(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a, uint64_t n,
                                                    uint32_t *err_out) {
 	__shared__ uint32_t s_stack[kStackSlots * kT];
 	uint32_t *stk = s_stack + threadIdx.x;
@@ -1452,7 +1452,7 @@ int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
 		if (mode == 1) {
 			const uint64_t n = (uint64_t)tiles_x * tiles_y * 64u;
 			uint64_t grid = (n + kBlock - 1) / kBlock;
-			const uint64_t cap = (uint64_t)ctx->cu_count * 4u; // 99 VGPRs: 4 waves per SIMD
+			const uint64_t cap = (uint64_t)ctx->cu_count * 5u; // 96 VGPRs: 5 waves per SIMD
 			grid = grid < cap ? grid : cap;
 			return launch(ctx, "path_trace", path_persist<kBlock>, dim3((uint32_t)(grid ? grid : 1)), dim3(kBlock), 0,
 			              (const uint32_t *)oct->words, oct->W, pa, n, ctx->d_err);
