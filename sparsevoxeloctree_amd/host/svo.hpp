@@ -10,6 +10,8 @@
 //   Octree         src/Octree.hpp:18-33
 //   Camera         src/Camera.hpp:14-38 (yaw / pitch / fov / aspect -> uniform block)
 //   OctreeTracer   src/OctreeTracer.hpp:15-70
+//   EnvironmentMap src/EnvironmentMap.hpp:20-33 (tables from linear RGBA texels; no EXR reader)
+//   PathTracer     src/PathTracer.hpp (Reset / CmdRender, m_bounce; light from Lighting)
 //
 // Header-only; link with libsvo.so.
 #pragma once
@@ -295,6 +297,104 @@ private:
 	uint32_t m_width = 0, m_height = 0;
 	svo_hit *m_hits = nullptr;
 	float *m_beam = nullptr;
+};
+
+// EnvironmentMap: an equirectangular map of linear RGBA texels with its sampling tables in HBM
+// (src/EnvironmentMap.cpp:99-188); m_rotation and m_multiplier as in the reference.
+class EnvironmentMap {
+public:
+	float m_rotation = 0.0f, m_multiplier = 1.0f;
+
+	static std::shared_ptr<EnvironmentMap> Create(const std::shared_ptr<Context> &ctx, uint32_t width, uint32_t height,
+	                                              const float *rgba) {
+		auto e = std::shared_ptr<EnvironmentMap>(new EnvironmentMap());
+		e->m_ctx = ctx;
+		if (!ctx->Check(svo_envmap_create(ctx->h, width, height, rgba, &e->m_h)))
+			return nullptr;
+		return e;
+	}
+	~EnvironmentMap() {
+		if (m_h)
+			svo_envmap_destroy(m_h);
+	}
+	const svo_envmap *Handle() const { return m_h; }
+
+private:
+	EnvironmentMap() = default;
+	std::shared_ptr<Context> m_ctx;
+	svo_envmap *m_h = nullptr;
+};
+
+// PathTracer: progressive samples into device running means of colour, first-hit albedo and
+// normal, float4 per pixel (PathTracer::Reset / CmdRender, src/PathTracer.cpp:176-215).  The
+// light is m_env_map when set (EnvMapPathTrace), else the constant m_sun_radiance.
+class PathTracer {
+public:
+	uint32_t m_bounce = 4;                      // kDefaultBounce, src/Config.hpp:25
+	float m_sun_radiance[3] = {5.0f, 5.0f, 5.0f}; // kDefaultConstantColor, src/Config.hpp:28
+	uint64_t m_seed = 0x5EED;
+	std::shared_ptr<EnvironmentMap> m_env_map;
+
+	static std::shared_ptr<PathTracer> Create(const std::shared_ptr<Octree> &octree,
+	                                          const std::shared_ptr<Camera> &camera) {
+		auto t = std::shared_ptr<PathTracer>(new PathTracer());
+		t->m_octree = octree;
+		t->m_camera = camera;
+		return t->Resize(1280, 720) ? t : nullptr;
+	}
+	~PathTracer() { release(); }
+	bool Resize(uint32_t width, uint32_t height) {
+		release();
+		m_width = width;
+		m_height = height;
+		m_spp = 0;
+		auto &ctx = m_octree->GetContextPtr();
+		const uint64_t bytes = (uint64_t)width * height * 4 * sizeof(float);
+		for (float *&b : m_images)
+			if (!ctx->Check(svo_malloc(ctx->h, bytes, (void **)&b)))
+				return false;
+		return true;
+	}
+	void Reset() { m_spp = 0; } // the next sample overwrites the means
+	bool CmdRender() {
+		auto &ctx = m_octree->GetContextPtr();
+		svo_camera c = m_camera->fetch_uniform_data();
+		const int rc =
+		    m_env_map ? svo_path_trace_env(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_bounce, m_spp,
+		                                   m_env_map->Handle(), m_env_map->m_rotation, m_env_map->m_multiplier, m_seed,
+		                                   m_images[0], m_images[1], m_images[2])
+		              : svo_path_trace(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_bounce, m_spp,
+		                               m_sun_radiance, m_seed, m_images[0], m_images[1], m_images[2]);
+		if (!ctx->Check(rc))
+			return false;
+		++m_spp;
+		return true;
+	}
+	uint32_t GetSampleCount() const { return m_spp; }
+	// image 0 = colour, 1 = albedo, 2 = normal: width * height float4
+	std::vector<float> Download(int image) const {
+		std::vector<float> out((size_t)m_width * m_height * 4);
+		auto &ctx = m_octree->GetContextPtr();
+		ctx->Check(svo_download(ctx->h, out.data(), m_images[image], out.size() * sizeof(float)));
+		return out;
+	}
+
+private:
+	PathTracer() = default;
+	void release() {
+		if (!m_octree)
+			return;
+		auto &ctx = m_octree->GetContextPtr();
+		for (float *&b : m_images) {
+			if (b)
+				svo_free(ctx->h, b);
+			b = nullptr;
+		}
+	}
+	std::shared_ptr<Octree> m_octree;
+	std::shared_ptr<Camera> m_camera;
+	uint32_t m_width = 0, m_height = 0, m_spp = 0;
+	float *m_images[3] = {nullptr, nullptr, nullptr};
 };
 
 } // namespace svo

@@ -3,11 +3,15 @@
 // Octree::Update, then optionally one primary frame through OctreeTracer.
 //
 //   svo_cli -scene KIND[,SEED[,PARAM]] -lvl N [-frame W H] [-no-beam] [-save FILE]
+//           [-pt SAMPLES BOUNCE [-env FILE] [-pt-out FILE]]
 //   svo_cli -tris FILE.bin -lvl N ...      (FILE.bin: uint64 T, T*9 float32 xyz, T uint32 rgb)
 //
 // Prints the reference's timing line, "FINISHED in {total} ms (Voxelize {v} ms, Octree building
 // {b} ms)" (src/LoaderThread.cpp:92-97), measured on the device, then the fragment count and the
-// octree's used range.  -save writes the octree with svo_octree_save.
+// octree's used range.  -save writes the octree with svo_octree_save.  -pt renders SAMPLES
+// progressive path-traced samples of the -frame view through PathTracer (constant sky, or the
+// environment map in FILE: uint32 width, height, then width*height float32 RGBA) and -pt-out
+// writes the colour means (width*height float32 RGBA).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -19,15 +23,15 @@
 
 static int usage() {
 	std::fprintf(stderr, "usage: svo_cli (-scene KIND[,SEED[,PARAM]] | -tris FILE | -obj FILE.obj) -lvl N "
-	                     "[-frame W H] [-no-beam] [-save FILE]\n");
+	                     "[-frame W H] [-no-beam] [-save FILE] [-pt SAMPLES BOUNCE [-env FILE] [-pt-out FILE]]\n");
 	return 2;
 }
 
 int main(int argc, char **argv) {
 	int kind = 0;
 	uint64_t seed = 1, param = 0;
-	std::string tris_path, obj_path, save_path;
-	uint32_t level = 0, fw = 0, fh = 0;
+	std::string tris_path, obj_path, save_path, env_path, pt_out;
+	uint32_t level = 0, fw = 0, fh = 0, pt_samples = 0, pt_bounce = 4;
 	bool beam = true;
 	for (int i = 1; i < argc; ++i) {
 		if (!std::strcmp(argv[i], "-scene") && i + 1 < argc) {
@@ -49,6 +53,13 @@ int main(int argc, char **argv) {
 			beam = false;
 		} else if (!std::strcmp(argv[i], "-save") && i + 1 < argc) {
 			save_path = argv[++i];
+		} else if (!std::strcmp(argv[i], "-pt") && i + 2 < argc) {
+			pt_samples = (uint32_t)std::atoi(argv[++i]);
+			pt_bounce = (uint32_t)std::atoi(argv[++i]);
+		} else if (!std::strcmp(argv[i], "-env") && i + 1 < argc) {
+			env_path = argv[++i];
+		} else if (!std::strcmp(argv[i], "-pt-out") && i + 1 < argc) {
+			pt_out = argv[++i];
 		} else
 			return usage();
 	}
@@ -149,6 +160,63 @@ int main(int argc, char **argv) {
 		svo_timing t2 = ctx->Timing();
 		std::printf("frame %ux%u hits %llu mean_iter %.2f beam %.3f ms march %.3f ms\n", fw, fh,
 		            (unsigned long long)nhit, (double)iters / hits.size(), beam ? t2.beam_ms : 0.0, t2.march_ms);
+		if (pt_samples) {
+			auto pt = svo::PathTracer::Create(octree, camera);
+			if (!pt || !pt->Resize(fw, fh)) {
+				std::fprintf(stderr, "path tracer: %s\n", ctx->error.c_str());
+				return 1;
+			}
+			pt->m_bounce = pt_bounce;
+			if (!env_path.empty()) {
+				FILE *f = std::fopen(env_path.c_str(), "rb");
+				uint32_t wh[2] = {0, 0};
+				std::vector<float> rgba;
+				bool ok = f && std::fread(wh, sizeof(uint32_t), 2, f) == 2 && wh[0] && wh[1] &&
+				          (uint64_t)wh[0] * wh[1] <= (1ull << 28);
+				if (ok) {
+					rgba.resize((size_t)wh[0] * wh[1] * 4);
+					ok = std::fread(rgba.data(), sizeof(float), rgba.size(), f) == rgba.size();
+				}
+				if (f)
+					std::fclose(f);
+				if (!ok) {
+					std::fprintf(stderr, "cannot read environment map %s\n", env_path.c_str());
+					return 1;
+				}
+				pt->m_env_map = svo::EnvironmentMap::Create(ctx, wh[0], wh[1], rgba.data());
+				if (!pt->m_env_map) {
+					std::fprintf(stderr, "environment map: %s\n", ctx->error.c_str());
+					return 1;
+				}
+			}
+			const auto t0 = std::chrono::steady_clock::now();
+			for (uint32_t s = 0; s < pt_samples; ++s)
+				if (!pt->CmdRender()) {
+					std::fprintf(stderr, "path tracer: %s\n", ctx->error.c_str());
+					return 1;
+				}
+			ctx->Synchronize();
+			const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+			std::vector<float> col = pt->Download(0);
+			double mean[3] = {0.0, 0.0, 0.0};
+			for (size_t k = 0; k < col.size(); k += 4)
+				for (int c = 0; c < 3; ++c)
+					mean[c] += col[k + c];
+			const double npx = (double)fw * fh;
+			std::printf("path_trace %s samples %u bounce %u mean %.6f %.6f %.6f %.3f ms/sample\n",
+			            pt->m_env_map ? "env" : "sky", pt->GetSampleCount(), pt_bounce, mean[0] / npx, mean[1] / npx,
+			            mean[2] / npx, ms / pt_samples);
+			if (!pt_out.empty()) {
+				FILE *f = std::fopen(pt_out.c_str(), "wb");
+				const bool ok = f && std::fwrite(col.data(), sizeof(float), col.size(), f) == col.size();
+				if (f)
+					std::fclose(f);
+				if (!ok) {
+					std::fprintf(stderr, "cannot write %s\n", pt_out.c_str());
+					return 1;
+				}
+			}
+		}
 	}
 	return 0;
 }

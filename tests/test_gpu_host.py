@@ -36,6 +36,41 @@ def test_cli_load_path_matches_oracle(tmp_path):
     assert re.search(r"frame 160x90 hits \d+", r.stdout)
 
 
+@pytest.mark.parametrize("light", ["sky", "env"])
+def test_cli_path_tracer_matches_oracle(tmp_path, light):
+    # svo::PathTracer (+ svo::EnvironmentMap) through svo_cli -pt: the colour means after three
+    # samples equal the oracle's, bit for bit
+    import sparsevoxeloctree_amd as S
+
+    W, H, bounce, n = 96, 64, 3, 3
+    out = tmp_path / "colour.bin"
+    cmd = [CLI, "-scene", "1,11,6", "-lvl", "7", "-frame", str(W), str(H), "-pt", str(n), str(bounce), "-pt-out",
+           str(out)]
+    rng = np.random.default_rng(4)
+    rgba = np.ones((16, 32, 4), np.float32)
+    rgba[..., :3] = rng.uniform(0.2, 2.0, (16, 32, 3))
+    if light == "env":
+        env = tmp_path / "env.bin"
+        env.write_bytes(np.array([32, 16], np.uint32).tobytes() + rgba.tobytes())
+        cmd += ["-env", str(env)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    assert re.search(r"path_trace %s samples 3 bounce 3 mean" % light, r.stdout)
+    got = np.fromfile(out, np.float32).reshape(-1, 4)
+    xyz, rgb = S.gen_scene(1, 11, 6)
+    words = oracle.build(oracle.voxelize(oracle.normalize(xyz), rgb, 7), 7)
+    cam = S.Camera((1.5, 1.5, 1.04), 0.0, 0.0, aspect=W / H).uniform_array()
+    ref = np.zeros((W * H, 4), np.float32)
+    tex, alias = oracle.envmap_tables(rgba)
+    for s in range(n):
+        if light == "env":
+            oracle.path_trace_env(words, 7, cam, W, H, bounce, s, tex, alias, 0.0, 1.0, 0x5EED, color=ref)
+        else:
+            oracle.path_trace(words, 7, cam, W, H, bounce, s, np.full(3, 5.0, np.float32), 0x5EED, color=ref)
+    assert np.array_equal(got, ref)
+
+
 def test_cli_obj_input_matches_oracle(tmp_path):
     # the reference's load path from an OBJ file: svo_cli -obj (svo_obj_load + Scene::Create)
     import sparsevoxeloctree_amd as S
