@@ -302,6 +302,47 @@ def _path_trace_env_case(ctx, scene_octree, bounce):
     env.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("shape,bounce,rotation,multiplier", [((1, 1), 3, 0.0, 1.0), ((3, 5), 16, -2.5, 2.0),
+                                                             ((4, 8), 2, 7.0, 0.0)])
+def test_path_trace_env_edge_cases(ctx, no_device_errors, scene_octree, mode, shape, bounce, rotation, multiplier):
+    # a one-texel map (one alias entry, every direction the same texel), an odd-sized map, the
+    # reference's largest bounce count (kMaxBounce = 16, src/Config.hpp:26), rotations beyond
+    # +-pi and a zero multiplier: colour, albedo and normal bit-exact against the oracle
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    octree, words, level, _ = scene_octree
+    rng = np.random.default_rng(sum(shape) + bounce)
+    rgba = np.ones(shape + (4,), np.float32)
+    rgba[..., :3] = rng.uniform(0.05, 3.0, shape + (3,))
+    env = S.EnvironmentMap.Create(ctx, rgba)
+    env.m_rotation, env.m_multiplier = rotation, multiplier
+    rtex, ralias = oracle.envmap_tables(rgba)
+    tex, alias = env.tables()
+    assert np.array_equal(tex, rtex) and np.array_equal(alias, ralias)
+    W, H = 67, 45
+    cam = cameras()[2]
+    ctx.check(lib.svo_march_mode(ctx.h, mode))
+    try:
+        pt = S.PathTracer.Create(octree, _camera_struct(cam))
+        pt.Resize(W, H)
+        pt.m_bounce = bounce
+        pt.m_env_map = env
+        pt.seed = 0xED6E + bounce
+        ref = [np.zeros((W * H, 4), np.float32) for _ in range(3)]
+        for s in range(2):
+            pt.CmdRender()
+            oracle.path_trace_env(words, level, cam, W, H, bounce, s, rtex, ralias, rotation, multiplier, pt.seed, *ref)
+            got = [pt.GetColor(), pt.GetAlbedo(), pt.GetNormal()]
+            for g, r, name in zip(got, ref, ("colour", "albedo", "normal")):
+                assert np.array_equal(g.reshape(-1, 4), r), "%s after sample %d" % (name, s)
+    finally:
+        ctx.check(lib.svo_march_mode(ctx.h, -1))
+    assert np.isfinite(ref[0]).all()
+    env.close()
+
+
 def test_envmap_rejects_a_dark_map(ctx):
     import sparsevoxeloctree_amd as S
 

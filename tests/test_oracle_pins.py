@@ -471,6 +471,8 @@ def test_envmap_tables_define_the_pdf():
     np.add.at(implied, al[~full], 1.0 - keep[~full])
     assert np.allclose(implied, tex[..., 3].ravel(), atol=1e-6)
     assert oracle.envmap_tables(np.zeros((4, 8, 4), np.float32)) is None  # no light
+    one, one_alias = oracle.envmap_tables(np.array([[[0.5, 2.0, 1.0, 0.0]]], np.float32))
+    assert one[0, 0, 3] == 1.0 and np.all(one_alias == 0xFFFFFFFF)  # a single texel takes every draw
 
 
 def _np_env_lookup(rgb, d, rot):
