@@ -208,7 +208,7 @@ def test_march_schedules_identical(ctx, no_device_errors, scene_octree, beam):
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("bounce", [1, 2, 4])
+@pytest.mark.parametrize("bounce", [1, 2, 4, 16])
 def test_path_trace_matches_oracle(ctx, no_device_errors, scene_octree, bounce, mode):
     # progressive path tracing (PathTracer::CmdRender), one path per lane (mode 0) or persistent
     # waves with per-lane refill (mode 1): three samples per pixel accumulated in place, colour,
@@ -340,6 +340,38 @@ def test_path_trace_env_edge_cases(ctx, no_device_errors, scene_octree, mode, sh
     finally:
         ctx.check(lib.svo_march_mode(ctx.h, -1))
     assert np.isfinite(ref[0]).all()
+    env.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("frame", [(1, 1), (9, 1), (1, 13)])
+def test_path_trace_tiny_frames(ctx, no_device_errors, scene_octree, mode, frame):
+    # frames narrower than one 8x8 tile: only the lanes inside the frame write, for both lights
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    octree, words, level, _ = scene_octree
+    W, H = frame
+    cam = cameras()[0]
+    rgba = _env_rgba(8, 16, 3)
+    env = S.EnvironmentMap.Create(ctx, rgba)
+    rtex, ralias = oracle.envmap_tables(rgba)
+    ctx.check(lib.svo_march_mode(ctx.h, mode))
+    try:
+        for use_env in (False, True):
+            pt = S.PathTracer.Create(octree, _camera_struct(cam))
+            pt.Resize(W, H)
+            pt.m_env_map = env if use_env else None
+            pt.CmdRender()
+            if use_env:
+                ref = oracle.path_trace_env(words, level, cam, W, H, pt.m_bounce, 0, rtex, ralias, 0.0, 1.0, pt.seed)
+            else:
+                ref = oracle.path_trace(words, level, cam, W, H, pt.m_bounce, 0,
+                                        np.array(pt.m_sun_radiance, np.float32), pt.seed)
+            assert np.array_equal(pt.GetColor().reshape(-1, 4), ref[0])
+            assert np.array_equal(pt.GetNormal().reshape(-1, 4), ref[2])
+    finally:
+        ctx.check(lib.svo_march_mode(ctx.h, -1))
     env.close()
 
 
