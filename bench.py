@@ -100,7 +100,8 @@ def run_step(S, ctx, scene, cfg, cam, bufs):
     R = w * h
     ctx.check(lib.svo_march(ctx.h, octree.h, C_cam(cam), w, h, 1, bufs["hits"].ptr, bufs["beam"].ptr), "march")
     if cfg["bounce"]:
-        ctx.check(lib.svo_bounce_rays(ctx.h, bufs["hits"].ptr, R, cfg["seed"], bufs["rays"].ptr), "bounce")
+        # bounce 1 of path-tracer sample 0 (Sobol dimensions 2-3 + blue noise, shader/path_tracer.comp:42)
+        ctx.check(lib.svo_bounce_rays(ctx.h, bufs["hits"].ptr, w, h, 0, 1, bufs["rays"].ptr), "bounce")
         ctx.check(lib.svo_march_rays(ctx.h, octree.h, bufs["rays"].ptr, R, 1, bufs["hits2"].ptr), "bounce march")
     ctx.synchronize()
     wall = time.perf_counter() - t0

@@ -49,6 +49,7 @@ typedef struct svo_ctx svo_ctx;
 typedef struct svo_scene svo_scene;       /* triangles resident in HBM (Scene) */
 typedef struct svo_fraglist svo_fraglist; /* voxel fragment list in HBM (Voxelizer) */
 typedef struct svo_octree svo_octree;     /* octree words in HBM (OctreeBuilder / Octree) */
+typedef struct svo_envmap svo_envmap;     /* environment map + sampling tables in HBM (EnvironmentMap) */
 
 /* Camera uniform block {vec4 pos, look, side, up} (src/Camera.hpp:28-30, src/Camera.cpp:77-95). */
 typedef struct svo_camera {
@@ -174,6 +175,14 @@ int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t 
 int svo_octree_info(const svo_octree *oct, uint32_t *level, uint64_t *W, const uint32_t **words_dev);
 int svo_octree_download(svo_ctx *ctx, const svo_octree *oct, uint32_t *out, uint64_t cap_words);
 int svo_octree_destroy(svo_octree *oct);
+/* Hand-off of a built octree to another context's stream (OctreeBuilder::CmdTransferOctreeOwnership
+ * and Octree::CmdTransferOwnership, src/OctreeBuilder.hpp:45, src/Octree.hpp:31; used by
+ * src/LoaderThread.cpp:78-82,99-105 to pass the octree from the loader queue to the main queue).
+ * Records a HIP event after the work enqueued so far on the context that built `oct` and makes
+ * dst's stream wait for it, so work enqueued on dst afterwards sees the finished octree; the host
+ * does not wait.  Both contexts must be on one device; the octree's memory stays with the context
+ * that built it (free it there). */
+int svo_octree_transfer(const svo_octree *oct, svo_ctx *dst);
 /* Octree files (the reference keeps octrees only in device memory; this is new): a 24-byte header
  * {char magic[8] = "SVOCTRE1", uint32 level, uint32 0, uint64 W} followed by W little-endian
  * words in canonical order.  svo_octree_load returns an octree ready to march. */
@@ -192,6 +201,16 @@ int svo_march(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32
  * words each beam cell loaded. */
 int svo_march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
                    float *beam_dev, uint32_t *nfetch_dev);
+/* The tracer's shaded image (shader/octree_tracer.frag:18-47, OctreeTracer::m_view_type,
+ * src/OctreeTracer.hpp:15) from a frame of hit records (svo_march's hits_dev) and the camera they
+ * were marched with.  view_type: 0 = diffuse, pow(albedo, 1/2.2); 1 = normal * 0.5 + 0.5;
+ * 2 = position, pos - 1; 3 = iterations, Heat(iter / 128) = sin(clamp(x,0,1) * 3 - (1,2,3)) * 0.5 +
+ * 0.5.  A miss has pos = 1, normal = 0 and, as colour, the light in the ray's direction: the
+ * constant colour light[3], or env's radiance turned by env_rotation when env is not NULL (without
+ * the multiplier, as the reference's Light()).  rgba8_dev: width*height uint32, RGBA8 UNORM (R in the
+ * low byte, alpha 255), each channel clamped to [0, 1] and rounded to nearest. */
+int svo_shade(svo_ctx *ctx, const svo_hit *hits_dev, const svo_camera *cam, uint32_t width, uint32_t height,
+              int view_type, const float light[3], const svo_envmap *env, float env_rotation, uint32_t *rgba8_dev);
 /* Explicit rays (path-tracer bounces, shader/path_tracer.comp:32-60): rays_dev = R x {o.xyz, d.xyz}
  * floats in octree space; variant = SVO_MARCH_*. */
 int svo_march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays_dev, uint64_t R, int variant,
@@ -232,10 +251,15 @@ int svo_autotune(svo_ctx *ctx, int on);
 int svo_march_mode(svo_ctx *ctx, int mode);
 int svo_autotune_report(svo_ctx *ctx, char *buf, uint64_t cap, uint64_t *len);
 
-/* One cosine-weighted diffuse bounce per hit (Diffuse_Sample + AlignDirection,
- * shader/diffuse.glsl:7-14, shader/util.glsl:5-9) with a seeded, XOR-scrambled 2-D Sobol sample
- * per ray index.  rays_dev = R x 6 floats; a ray whose hit bit is clear gets o = 0, d = -1. */
-int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t seed, float *rays_dev);
+/* Bounce `cur` (1..31) of path-tracer sample `spp` for a width x height frame of primary hits (ray
+ * k = pixel (k % width, k / width)), as shader/path_tracer.comp:42-44 draws it: o = o_pos,
+ * d = Diffuse_Sample(normal, fract(Sobol_GetVec2(cur) + noise)) (shader/diffuse.glsl:7-14,
+ * shader/util.glsl:5-9), where Sobol_GetVec2 reads the Sobol state after spp advances
+ * (shader/sobol.comp, src/Sobol.cpp:45-62) and noise is the pixel's texel of the 256x256 blue-noise
+ * texture (src/Noise.inl, fetched at pixel & 255).  rays_dev = width*height x 6 floats; a ray whose
+ * hit bit is clear gets o = 0, d = -1. */
+int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint32_t width, uint32_t height, uint32_t spp,
+                    uint32_t cur, float *rays_dev);
 
 /* ---- serving loop (new; the reference's nearest analogue is its loader thread, which builds one
  * octree on a side queue and hands it to the renderer, src/LoaderThread.cpp:15-116,
@@ -275,17 +299,25 @@ int svo_camera_from_yaw_pitch(const float position[3], float yaw, float pitch, f
                               svo_camera *out);
 
 /* ---- progressive path tracer (PathTracer::CmdRender, src/PathTracer.cpp:176-215;
- * ConstColorPathTrace and the accumulation of shader/path_tracer.comp:32-60,111-132).  One sample
- * per pixel: the camera ray through pixel + the frame's 2-D Sobol point (the pixel corner at
- * spp 0), then up to bounce-1 cosine-weighted diffuse bounces (Octree_RayMarchLeaf, 5 arguments);
- * a path that escapes adds sky (linear RGB) times the product of the leaf albedos it met.
- * color_dev / albedo_dev / normal_dev: width*height float4 (w = 1), row-major, the running means
- * after spp + 1 samples (spp = samples already in the buffers; spp 0 starts them, as a Reset).
- * albedo_dev and normal_dev (first-hit albedo and face normal) may be NULL.  The sampling is seeded
- * (seed) and replaces the reference's Sobol buffer and blue-noise table.  bounce: 1..64. */
+ * ConstColorPathTrace and main of shader/path_tracer.comp:32-60,111-132).  Sample `spp` of every
+ * pixel, with the reference's sampler: the Sobol state after spp advances of shader/sobol.comp
+ * (2 * (bounce + 1) dimensions, src/PathTracer.cpp:184), the primary ray through
+ * pixel + Sobol_GetVec2(0) (the pixel corner at spp 0), bounce cur sampled by
+ * fract(Sobol_GetVec2(cur) + noise(pixel & 255)) from the 256x256 blue-noise texture; up to
+ * bounce-1 cosine-weighted diffuse bounces (Octree_RayMarchLeaf, 5 arguments); a path that escapes
+ * adds sky (linear RGB) times the product of the leaf albedos it met.  The buffers hold the
+ * running means after spp + 1 samples, in the reference's image formats (src/PathTracer.cpp:8-16):
+ *   color_dev   width*height float4 (RGBA32F, a = 1)
+ *   albedo_dev  width*height uint32, RGBA8 UNORM (R in the low byte, a = 255): first-hit albedo
+ *   normal_dev  width*height uint32, RGBA8 SNORM (a = 127): first-hit face normal
+ * imageLoad / imageStore of the 8-bit formats: byte / 255 (UNORM), max(signed byte / 127, -1)
+ * (SNORM); stores round the clamped value times 255 (127) to nearest, halves up.  spp = samples
+ * already in the buffers; spp 0 starts them, as after the reference's Reset clears its images.
+ * albedo_dev and normal_dev may be NULL.  bounce: 1..31 (the Sobol table has 64 dimensions; the
+ * reference's UI allows 2..16, src/Config.hpp:24-26). */
 int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
-                   uint32_t bounce, uint32_t spp, const float sky[3], uint64_t seed, float *color_dev,
-                   float *albedo_dev, float *normal_dev);
+                   uint32_t bounce, uint32_t spp, const float sky[3], float *color_dev, uint32_t *albedo_dev,
+                   uint32_t *normal_dev);
 
 /* ---- environment-map light (EnvironmentMap, src/EnvironmentMap.cpp:55-190; EnvMapPathTrace,
  * shader/path_tracer.comp:62-106).  svo_envmap_create: width x height texels of linear RGB (4
@@ -294,18 +326,18 @@ int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, u
  * and kept in HBM.  A map with zero luminance everywhere is rejected.  svo_envmap_download returns
  * the device tables: texels (4 floats: rgb, pdf) and alias ({prob, alias} per texel); either may be
  * NULL. */
-typedef struct svo_envmap svo_envmap;
 int svo_envmap_create(svo_ctx *ctx, uint32_t width, uint32_t height, const float *rgba, svo_envmap **out);
 int svo_envmap_download(svo_ctx *ctx, const svo_envmap *env, float *texels, uint32_t *alias);
 int svo_envmap_destroy(svo_envmap *env);
-/* svo_path_trace with the environment-map light: a path that misses at once takes the map's
- * radiance; every bounce adds a light sample (alias-table texel, occlusion ray) and, when its diffuse
- * ray escapes, the map's radiance, both weighted by the power heuristic and scaled by multiplier;
- * rotation turns the map about +y (radians).  One path per lane; same buffers and sampling as
- * svo_path_trace.  Texture lookups are bilinear with repeat addressing, in fp32. */
+/* svo_path_trace with the environment-map light (EnvMapPathTrace, shader/path_tracer.comp:62-106): a
+ * path that misses at once takes the map's radiance; every bounce adds a light sample (alias-table
+ * texel, occlusion ray) and, when its diffuse ray escapes, the map's radiance, both weighted by the
+ * power heuristic and scaled by multiplier; rotation turns the map about +y (radians).  The light
+ * sample and the diffuse direction use the same 2-D sample, as the reference does.  Same buffers
+ * and sampler as svo_path_trace.  Texture lookups are bilinear with repeat addressing, in fp32. */
 int svo_path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
                        uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
-                       uint64_t seed, float *color_dev, float *albedo_dev, float *normal_dev);
+                       float *color_dev, uint32_t *albedo_dev, uint32_t *normal_dev);
 
 /* ---- OBJ + MTL files (Scene::load_meshes, src/Scene.cpp:36-147, over tinyobj::LoadObj): the
  * triangle soup for svo_scene_create.  *xyz: 9 floats per triangle, the file's positions (not yet

@@ -45,16 +45,18 @@ def load():
             "orc_march_frame": (None, [P, u64, u32, P, u32, u32, P, P, i32]),
             "orc_march_frame_rows": (None, [P, u64, u32, P, u32, u32, u32, u32, P, P, i32]),
             "orc_march_rays": (None, [P, u64, u32, P, u64, i32, P, i32]),
-            "orc_bounce_rays": (None, [P, u64, u64, P, P]),
+            "orc_bounce_rays": (None, [P, u32, u32, u32, u32, P, P, P, P]),
+            "orc_sobol_state": (None, [P, u32, u32, P]),
             "orc_sincos": (None, [f32, P, P]),
-            "orc_path_trace": (None, [P, u64, u32, P, u32, u32, u32, u32, P, u64, P, P, P, i32]),
+            "orc_path_trace": (None, [P, u64, u32, P, u32, u32, u32, u32, P, P, P, P, P, P, i32]),
             "orc_envmap_tables": (i32, [u32, u32, P, P, P]),
-            "orc_path_trace_env": (None, [P, u64, u32, P, u32, u32, u32, u32, P, P, u32, u32, f32, f32, u64, P, P, P,
-                                          i32]),
+            "orc_path_trace_env": (None, [P, u64, u32, P, u32, u32, u32, u32, P, P, u32, u32, f32, f32, P, P, P, P,
+                                          P, i32]),
             "orc_env_sample": (None, [P, P, u32, u32, f32, f32, f32, P, P, P]),
             "orc_env_lookup": (None, [P, u32, u32, f32, P, P, P]),
             "orc_atan2": (f32, [f32, f32]),
             "orc_acos": (f32, [f32]),
+            "orc_shade": (None, [P, P, u32, u32, i32, P, P, u32, u32, f32, P]),
             "orc_stack_uninit_reads": (u64, []),
             "orc_reset_counters": (None, []),
         }
@@ -168,27 +170,62 @@ def march_rays(words, level, rays, variant=0, threads=None):
     return hits
 
 
-def bounce_rays(hits, seed):
+_TABLES = None
+DATA_DIR = os.path.join(os.path.dirname(HERE), "sparsevoxeloctree_amd", "data")
+
+
+def sampler_tables():
+    """(Sobol generator matrices uint32[64, 32], blue noise uint8[256 * 256 * 2]): the data tables of the
+    reference's path-tracer sampler, read from the files the product library is built from (their
+    bytes are checked against the reference's own tables by tests/test_oracle_pins.py)."""
+    global _TABLES
+    if _TABLES is None:
+        sob = np.fromfile(os.path.join(DATA_DIR, "sobol_matrices.bin"), dtype="<u4").reshape(64, 32)
+        noise = np.fromfile(os.path.join(DATA_DIR, "blue_noise_rg8.bin"), dtype=np.uint8)
+        assert noise.size == 256 * 256 * 2
+        _TABLES = (np.ascontiguousarray(sob), np.ascontiguousarray(noise))
+    return _TABLES
+
+
+def sobol_state(n, dims=64, mat=None):
+    """The Sobol state after n advances from zero (sequential, as shader/sobol.comp)."""
+    mat = sampler_tables()[0] if mat is None else np.ascontiguousarray(mat, dtype=np.uint32)
+    x = np.zeros(dims, np.uint32)
+    load().orc_sobol_state(_p(mat), dims, n, _p(x))
+    return x
+
+
+def bounce_rays(hits, width, height, spp, cur=1):
+    """Bounce `cur` of sample spp for a width x height frame of primary hits: (rays (R, 6), hit mask)."""
     lib = load()
     hits = np.ascontiguousarray(hits.reshape(-1))
     R = hits.shape[0]
+    assert R == width * height
+    sob, noise = sampler_tables()
     rays = np.empty((R, 6), dtype=np.float32)
     mask = np.empty(R, dtype=np.uint8)
-    lib.orc_bounce_rays(_p(hits), R, seed, _p(rays), _p(mask))
+    lib.orc_bounce_rays(_p(hits), width, height, spp, cur, _p(sob), _p(noise), _p(rays), _p(mask))
     return rays, mask.astype(bool)
 
 
-def path_trace(words, level, cam, width, height, bounce, spp, sky, seed, color=None, albedo=None, normal=None,
+def _pt_buffers(width, height, color, albedo, normal):
+    return (np.zeros((width * height, 4), np.float32) if color is None else color,
+            np.zeros(width * height, np.uint32) if albedo is None else albedo,
+            np.zeros(width * height, np.uint32) if normal is None else normal)
+
+
+def path_trace(words, level, cam, width, height, bounce, spp, sky, color=None, albedo=None, normal=None,
                threads=None):
-    """One progressive sample per pixel into float32 (height*width, 4) buffers (created when None and
-    returned as (color, albedo, normal))."""
+    """Sample spp of every pixel accumulated into color (float32 (height*width, 4)), albedo (RGBA8
+    UNORM, uint32 per pixel) and normal (RGBA8 SNORM); created when None; returns the three."""
     words = np.ascontiguousarray(words, dtype=np.uint32)
     cam = np.ascontiguousarray(cam, dtype=np.float32)
-    bufs = [np.zeros((width * height, 4), np.float32) if b is None else b for b in (color, albedo, normal)]
+    bufs = _pt_buffers(width, height, color, albedo, normal)
     sky = np.ascontiguousarray(sky, dtype=np.float32)
-    load().orc_path_trace(_p(words), words.shape[0], level, _p(cam), width, height, bounce, spp, _p(sky), seed,
-                       _p(bufs[0]), _p(bufs[1]), _p(bufs[2]), threads or nthreads())
-    return tuple(bufs)
+    sob, noise = sampler_tables()
+    load().orc_path_trace(_p(words), words.shape[0], level, _p(cam), width, height, bounce, spp, _p(sky), _p(sob),
+                          _p(noise), _p(bufs[0]), _p(bufs[1]), _p(bufs[2]), threads or nthreads())
+    return bufs
 
 
 def envmap_tables(rgba):
@@ -202,18 +239,19 @@ def envmap_tables(rgba):
     return t, a
 
 
-def path_trace_env(words, level, cam, width, height, bounce, spp, texels, alias, rotation, multiplier, seed,
+def path_trace_env(words, level, cam, width, height, bounce, spp, texels, alias, rotation, multiplier,
                    color=None, albedo=None, normal=None, threads=None):
     words = np.ascontiguousarray(words, dtype=np.uint32)
     cam = np.ascontiguousarray(cam, dtype=np.float32)
     texels = np.ascontiguousarray(texels, dtype=np.float32)
     alias = np.ascontiguousarray(alias, dtype=np.uint32)
     eh, ew = texels.shape[:2]
-    bufs = [np.zeros((width * height, 4), np.float32) if b is None else b for b in (color, albedo, normal)]
+    bufs = _pt_buffers(width, height, color, albedo, normal)
+    sob, noise = sampler_tables()
     load().orc_path_trace_env(_p(words), words.shape[0], level, _p(cam), width, height, bounce, spp, _p(texels),
-                              _p(alias), ew, eh, rotation, multiplier, seed, _p(bufs[0]), _p(bufs[1]), _p(bufs[2]),
-                              threads or nthreads())
-    return tuple(bufs)
+                              _p(alias), ew, eh, rotation, multiplier, _p(sob), _p(noise), _p(bufs[0]), _p(bufs[1]),
+                              _p(bufs[2]), threads or nthreads())
+    return bufs
 
 
 def env_sample(texels, alias, rotation, u, v):
@@ -248,6 +286,18 @@ def sincos(phi):
     s, c = C.c_float(), C.c_float()
     load().orc_sincos(phi, C.byref(s), C.byref(c))
     return s.value, c.value
+
+
+def shade(hits, cam, width, height, view, light=(5.0, 5.0, 5.0), texels=None, rotation=0.0):
+    """RGBA8 UNORM texels (uint32 per pixel) of the tracer's view-type shading of a hit frame."""
+    hits = np.ascontiguousarray(hits.reshape(-1))
+    cam = np.ascontiguousarray(cam, dtype=np.float32)
+    light = np.ascontiguousarray(light, dtype=np.float32)
+    out = np.empty(width * height, np.uint32)
+    eh, ew = (0, 0) if texels is None else texels.shape[:2]
+    tex = None if texels is None else np.ascontiguousarray(texels, dtype=np.float32)
+    load().orc_shade(_p(hits), _p(cam), width, height, view, _p(light), _p(tex), ew, eh, rotation, _p(out))
+    return out
 
 
 def stack_uninit_reads():

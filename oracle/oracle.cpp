@@ -476,27 +476,88 @@ uint32_t beam_dim(uint32_t x) { return (x + 7u) / 8u + 1u; } // src/OctreeTracer
 // ---------------------------------------------------------------------------------------------
 // Bounce sampling.
 // ---------------------------------------------------------------------------------------------
-uint64_t splitmix64(uint64_t &s) {
-	uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-	return z ^ (z >> 31);
-}
-uint32_t sobol0(uint32_t n) { // van der Corput: bit reversal
-	uint32_t r = 0;
-	for (int k = 0; k < 32; ++k)
-		r |= ((n >> k) & 1u) << (31 - k);
-	return r;
-}
-uint32_t sobol1(uint32_t n) { // primitive polynomial x + 1: v_{k+1} = v_k ^ (v_k >> 1)
-	uint32_t v = 0x80000000u, r = 0;
-	for (int k = 0; k < 32; ++k) {
-		if ((n >> k) & 1u)
-			r ^= v;
-		v ^= v >> 1;
+// The reference's path-tracer sampler (shader/path_tracer.comp:42,72,113-117, shader/sobol.glsl,
+// shader/sobol.comp, src/Sobol.cpp:52-62, src/PathTracer.cpp:24-28).
+//   * Sobol state: `dims` 32-bit words, all zero after Sobol::Reset; each CmdNext (after each
+//     frame, src/PathTracer.cpp:214) XORs column c = findLSB(~n) of every dimension's matrix into
+//     it and increments the counter n (shader/sobol.comp:78-86).  Restated here as that sequential
+//     advance (the device computes the same state from the Gray code of n).
+//   * Sobol_GetVec2(i) = vec2(x[2i], x[2i+1]) / 4294967296.0.
+//   * noise = texelFetch(uNoise, pixel & 0xff).xy of a 256x256 RG8 UNORM texture: byte / 255.
+//   * bounce cur's sample = fract(Sobol_GetVec2(cur) + noise); the primary ray goes through
+//     pixel + Sobol_GetVec2(0).
+struct Sampler {
+	uint32_t x[64];
+	const uint8_t *noise;
+	void init(const uint32_t *mat, const uint8_t *noise_tab, uint32_t dims, uint32_t n) {
+		noise = noise_tab;
+		for (uint32_t d = 0; d < 64; ++d)
+			x[d] = 0u;
+		for (uint32_t i = 0; i < n; ++i) { // shader/sobol.comp:82-85
+			uint32_t c = 0;
+			while ((i >> c) & 1u) // c = findLSB(~i), the lowest zero bit of i (i < 2^32 - 1)
+				++c;
+			for (uint32_t d = 0; d < dims; ++d)
+				x[d] ^= mat[d * 32 + c];
+		}
 	}
-	return r;
+	static float to_unit(uint32_t v) { return (float)v / 4294967296.0f; }
+	void vec2(uint32_t i, float &u, float &v) const { u = to_unit(x[2 * i]), v = to_unit(x[2 * i + 1]); }
+	void pixel_noise(uint32_t px, uint32_t py, float &nu, float &nv) const {
+		const uint8_t *t = noise + (((py & 255u) << 8) | (px & 255u)) * 2u;
+		nu = (float)t[0] / 255.0f, nv = (float)t[1] / 255.0f;
+	}
+	void bounce(uint32_t cur, float nu, float nv, float &u, float &v) const { // fract(Sobol_GetVec2(cur) + noise)
+		float a, b;
+		vec2(cur, a, b);
+		const float s = a + nu, t = b + nv;
+		u = s - std::floor(s), v = t - std::floor(t);
+	}
+};
+
+// RGBA8 UNORM / SNORM texels of the path tracer's albedo and normal images (src/PathTracer.cpp:13-
+// 16): imageLoad gives byte / 255 (UNORM) or max(signed byte / 127, -1) (SNORM); imageStore rounds
+// the clamped value times 255 (or 127) to nearest, halves up.
+uint32_t sat_u(float f) { return f >= 1.0f ? (f < 4294967296.0f ? (uint32_t)f : 0xFFFFFFFFu) : 0u; }
+uint32_t unorm8(float f) {
+	f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+	return sat_u(std::floor(f * 255.0f + 0.5f));
 }
+uint32_t snorm8(float f) {
+	f = f < -1.0f ? -1.0f : (f > 1.0f ? 1.0f : f);
+	return (uint32_t)((int32_t)std::floor(f * 127.0f + 0.5f) & 0xff);
+}
+float unorm8_f(uint32_t c) { return (float)(c & 0xffu) / 255.0f; }
+float snorm8_f(uint32_t c) {
+	const float v = (float)(int8_t)(uint8_t)(c & 0xffu) / 127.0f;
+	return v < -1.0f ? -1.0f : v;
+}
+// Running mean of a pixel (shader/path_tracer.comp:123-131): (old * spp + v) / (spp + 1); spp 0
+// starts from the cleared image.
+void mean_f4(float *img, uint64_t k, uint32_t spp, const float v[3]) {
+	if (!img)
+		return;
+	float *p = img + 4 * k;
+	const float n = (float)spp, n1 = (float)(spp + 1u);
+	for (int c = 0; c < 3; ++c)
+		p[c] = spp == 0u ? v[c] : (p[c] * n + v[c]) / n1;
+	p[3] = 1.0f;
+}
+void mean_rgba8(uint32_t *img, uint64_t k, uint32_t spp, const float v[3], bool snorm) {
+	if (!img)
+		return;
+	const uint32_t old = spp == 0u ? 0u : img[k];
+	const float n = (float)spp, n1 = (float)(spp + 1u);
+	uint32_t out = snorm ? 127u << 24 : 255u << 24; // alpha 1
+	for (int c = 0; c < 3; ++c) {
+		const uint32_t b = old >> (8 * c);
+		const float o = snorm ? snorm8_f(b) : unorm8_f(b);
+		const float m = (o * n + v[c]) / n1;
+		out |= (snorm ? snorm8(m) : unorm8(m)) << (8 * c);
+	}
+	img[k] = out;
+}
+
 
 // sin/cos: quadrant reduction with a two-constant split of pi/2 and Taylor polynomials on
 // [-pi/4, pi/4]; all in binary32 with a fixed operation order.
@@ -761,11 +822,36 @@ void orc_march_rays(const uint32_t *words, uint64_t W, uint32_t level, const flo
 
 void orc_sincos(float phi, float *s, float *c) { sincos_poly(phi, *s, *c); }
 
-void orc_bounce_rays(const orc_hit *hits, uint64_t R, uint64_t seed, float *rays, uint8_t *mask) {
-	uint64_t st = seed;
-	uint64_t z = splitmix64(st);
-	uint32_t s0 = (uint32_t)z, s1 = (uint32_t)(z >> 32);
+// Diffuse_Sample + AlignDirection (shader/diffuse.glsl:7-14, shader/util.glsl:5-9) about the axis
+// normal of face code `code` (axis * 2 + positive) for the sample (u, v).
+static V3 diffuse_dir(uint32_t code, float u, float v) {
 	const float kPi = 3.1415926535897932384626433832795f; // shader/util.glsl:4
+	V3 nrm{0.0f, 0.0f, 0.0f};
+	const float sg = (code & 1u) ? 1.0f : -1.0f;
+	(code >> 1 == 0 ? nrm.x : code >> 1 == 1 ? nrm.y : nrm.z) = sg;
+	float rr = std::sqrt(u), phi = 2.0f * kPi * v;
+	float sn, cs;
+	sincos_poly(phi, sn, cs);
+	V3 dl{rr * cs, rr * sn, std::sqrt(1.0f - u)};
+	V3 ax = std::fabs(nrm.x) > 0.01f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+	V3 uu = normalize(cross(ax, nrm));
+	V3 vv = cross(nrm, uu);
+	return V3{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
+	          dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+}
+
+void orc_sobol_state(const uint32_t *mat, uint32_t dims, uint32_t n, uint32_t *x) {
+	Sampler sm;
+	sm.init(mat, nullptr, dims, n);
+	for (uint32_t d = 0; d < dims; ++d)
+		x[d] = sm.x[d];
+}
+
+void orc_bounce_rays(const orc_hit *hits, uint32_t width, uint32_t height, uint32_t spp, uint32_t cur,
+                     const uint32_t *sobol_mat, const uint8_t *noise, float *rays, uint8_t *mask) {
+	Sampler sm;
+	sm.init(sobol_mat, noise, 2 * (cur + 1), spp);
+	const uint64_t R = (uint64_t)width * height;
 	for (uint64_t k = 0; k < R; ++k) {
 		float *r = rays + 6 * k;
 		const orc_hit &h = hits[k];
@@ -778,114 +864,73 @@ void orc_bounce_rays(const orc_hit *hits, uint64_t R, uint64_t seed, float *rays
 		}
 		if (mask)
 			mask[k] = 1;
-		uint32_t n = (uint32_t)k;
-		float u = (float)((sobol0(n) ^ s0) >> 8) * 5.9604644775390625e-08f; // * 2^-24
-		float v = (float)((sobol1(n) ^ s1) >> 8) * 5.9604644775390625e-08f;
-		uint32_t code = (h.vz_flags >> 24) & 7u;
-		V3 nrm{0.0f, 0.0f, 0.0f};
-		float sg = (code & 1u) ? 1.0f : -1.0f;
-		(code >> 1 == 0 ? nrm.x : code >> 1 == 1 ? nrm.y : nrm.z) = sg;
-		// Diffuse_Sample (shader/diffuse.glsl:7-14)
-		float rr = std::sqrt(u), phi = 2.0f * kPi * v;
-		float sn, cs;
-		sincos_poly(phi, sn, cs);
-		V3 dl{rr * cs, rr * sn, std::sqrt(1.0f - u)};
-		// AlignDirection (shader/util.glsl:5-9)
-		V3 ax = std::fabs(nrm.x) > 0.01f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
-		V3 uu = normalize(cross(ax, nrm));
-		V3 vv = cross(nrm, uu);
-		V3 d{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
-		     dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+		const uint32_t px = (uint32_t)(k % width), py = (uint32_t)(k / width);
+		float nu, nv, u, v;
+		sm.pixel_noise(px, py, nu, nv);
+		sm.bounce(cur, nu, nv, u, v);
+		const V3 d = diffuse_dir((h.vz_flags >> 24) & 7u, u, v);
 		r[0] = h.pos[0], r[1] = h.pos[1], r[2] = h.pos[2];
 		r[3] = d.x, r[4] = d.y, r[5] = d.z;
 	}
 }
 
-// lowbias32 integer hash (C. Wellons' hash-prospector).
-static uint32_t hash32(uint32_t x) {
-	x ^= x >> 16;
-	x *= 0x7feb352du;
-	x ^= x >> 15;
-	x *= 0x846ca68bu;
-	x ^= x >> 16;
-	return x;
+static V3 face_normal(uint32_t code) {
+	V3 n{0.0f, 0.0f, 0.0f};
+	const float sg = (code & 1u) ? 1.0f : -1.0f;
+	(code >> 1 == 0 ? n.x : code >> 1 == 1 ? n.y : n.z) = sg;
+	return n;
+}
+static float leaf_colour(uint32_t w, int c) { return (float)((w >> (8 * c)) & 0xffu) / 255.0f; } // unpackUnorm4x8
+
+// Stores one sample of pixel k (shader/path_tracer.comp:123-131).
+static void store_sample(uint64_t k, uint32_t spp, V3 rad, V3 alb, V3 nrm, float *color, uint32_t *albedo,
+                         uint32_t *normal) {
+	const float c[3] = {rad.x, rad.y, rad.z}, a[3] = {alb.x, alb.y, alb.z}, n[3] = {nrm.x, nrm.y, nrm.z};
+	mean_f4(color, k, spp, c);
+	mean_rgba8(albedo, k, spp, a, false);
+	mean_rgba8(normal, k, spp, n, true);
 }
 
-// ConstColorPathTrace + the running means of shader/path_tracer.comp:32-60,111-132, with the seeded
-// sampling that replaces the reference's Sobol buffer and blue-noise table: bounce cur of frame spp
-// uses the 2-D Sobol point spp XOR-shifted by hash32(key ^ cur) and rotated by a per-pixel hash.
+// ConstColorPathTrace + main (shader/path_tracer.comp:32-60,111-132) for sample index spp.
 void orc_path_trace(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
-                    uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, uint64_t seed, float *color,
-                    float *albedo, float *normal, int nthreads) {
+                    uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, const uint32_t *sobol_mat,
+                    const uint8_t *noise, float *color, uint32_t *albedo, uint32_t *normal, int nthreads) {
 	(void)level;
 	Octree oc{words, W};
-	uint64_t st = seed;
-	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
-	const uint32_t s0 = (uint32_t)z0, s1 = (uint32_t)(z0 >> 32), s2 = (uint32_t)z1, s3 = (uint32_t)(z1 >> 32);
-	const float kPi = 3.1415926535897932384626433832795f; // shader/util.glsl:4
-	const float jx = (float)(sobol0(spp) >> 8) * 5.9604644775390625e-08f,
-	            jy = (float)(sobol1(spp) >> 8) * 5.9604644775390625e-08f;
-	auto colour = [](uint32_t w, int c) { return (float)((w >> (8 * c)) & 0xffu) / 255.0f; }; // unpackUnorm4x8
-	auto face = [](uint32_t code) {
-		V3 n{0.0f, 0.0f, 0.0f};
-		const float sg = (code & 1u) ? 1.0f : -1.0f;
-		(code >> 1 == 0 ? n.x : code >> 1 == 1 ? n.y : n.z) = sg;
-		return n;
-	};
+	Sampler sm;
+	sm.init(sobol_mat, noise, (bounce + 1) * 2, spp); // src/PathTracer.cpp:184
+	float jx, jy;
+	sm.vec2(0, jx, jy);
 	parallel_for((uint64_t)width * height, nthreads, [&](uint64_t b, uint64_t e) {
 		for (uint64_t kk = b; kk < e; ++kk) {
 			const uint32_t k = (uint32_t)kk, px = k % width, py = k / width;
+			float nu, nv;
+			sm.pixel_noise(px, py, nu, nv);
 			V3 o = cam_vec(cam, 0);
 			V3 d = gen_ray(cam, ((float)px + jx) / (float)width, ((float)py + jy) / (float)height);
 			orc_hit h = march_leaf(oc, o, d, 1);
 			V3 rad{sky[0], sky[1], sky[2]}, alb0{0.0f, 0.0f, 0.0f}, nrm0{0.0f, 0.0f, 0.0f};
 			if (h.vz_flags & 0x80000000u) {
-				V3 acc{colour(h.leaf, 0), colour(h.leaf, 1), colour(h.leaf, 2)};
+				V3 acc{leaf_colour(h.leaf, 0), leaf_colour(h.leaf, 1), leaf_colour(h.leaf, 2)};
 				alb0 = acc;
 				uint32_t code = (h.vz_flags >> 24) & 7u;
-				nrm0 = face(code);
+				nrm0 = face_normal(code);
 				rad = V3{0.0f, 0.0f, 0.0f};
-				const uint32_t na = hash32(k ^ s2), nb = hash32(na ^ s3);
 				for (uint32_t cur = 1; cur < bounce; ++cur) {
-					const uint32_t su = (sobol0(spp) ^ hash32(s0 ^ cur)) + na, sv = (sobol1(spp) ^ hash32(s1 ^ cur)) + nb;
-					const float u = (float)(su >> 8) * 5.9604644775390625e-08f,
-					            v = (float)(sv >> 8) * 5.9604644775390625e-08f;
-					// Diffuse_Sample + AlignDirection (shader/diffuse.glsl:7-14, shader/util.glsl:5-9)
-					const V3 nrm = face(code);
-					float rr = std::sqrt(u), phi = 2.0f * kPi * v;
-					float sn, cs;
-					sincos_poly(phi, sn, cs);
-					V3 dl{rr * cs, rr * sn, std::sqrt(1.0f - u)};
-					V3 ax = std::fabs(nrm.x) > 0.01f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
-					V3 uu = normalize(cross(ax, nrm));
-					V3 vv = cross(nrm, uu);
+					float u, v;
+					sm.bounce(cur, nu, nv, u, v);
 					o = V3{h.pos[0], h.pos[1], h.pos[2]};
-					d = V3{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
-					       dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+					d = diffuse_dir(code, u, v);
 					h = march_leaf(oc, o, d, 1);
 					if (!(h.vz_flags & 0x80000000u)) {
 						rad = V3{acc.x * sky[0], acc.y * sky[1], acc.z * sky[2]};
 						break;
 					}
-					acc = V3{acc.x * colour(h.leaf, 0), acc.y * colour(h.leaf, 1), acc.z * colour(h.leaf, 2)};
+					acc = V3{acc.x * leaf_colour(h.leaf, 0), acc.y * leaf_colour(h.leaf, 1), acc.z * leaf_colour(h.leaf, 2)};
 					code = (h.vz_flags >> 24) & 7u;
 				}
 			}
-			const float n = (float)spp, n1 = (float)(spp + 1u);
-			auto mean = [&](float *img, V3 v) {
-				if (!img)
-					return;
-				float *p = img + 4 * (uint64_t)k;
-				if (spp == 0u) {
-					p[0] = v.x, p[1] = v.y, p[2] = v.z;
-				} else {
-					p[0] = (p[0] * n + v.x) / n1, p[1] = (p[1] * n + v.y) / n1, p[2] = (p[2] * n + v.z) / n1;
-				}
-				p[3] = 1.0f;
-			};
-			mean(color, rad);
-			mean(albedo, alb0);
-			mean(normal, nrm0);
+			store_sample(k, spp, rad, alb0, nrm0, color, albedo, normal);
 		}
 	});
 }
@@ -1031,24 +1076,16 @@ void orc_env_lookup(const float *texels, uint32_t w, uint32_t h, float rotation,
 
 void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
                         uint32_t height, uint32_t bounce, uint32_t spp, const float *texels, const uint32_t *alias,
-                        uint32_t ew, uint32_t eh, float rotation, float multiplier, uint64_t seed, float *color,
-                        float *albedo, float *normal, int nthreads) {
+                        uint32_t ew, uint32_t eh, float rotation, float multiplier, const uint32_t *sobol_mat,
+                        const uint8_t *noise, float *color, uint32_t *albedo, uint32_t *normal, int nthreads) {
 	(void)level;
 	Octree oc{words, W};
 	const Env env{texels, alias, ew, eh, rotation, multiplier};
-	uint64_t st = seed;
-	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
-	const uint32_t s0 = (uint32_t)z0, s1 = (uint32_t)(z0 >> 32), s2 = (uint32_t)z1, s3 = (uint32_t)(z1 >> 32);
+	Sampler sm;
+	sm.init(sobol_mat, noise, (bounce + 1) * 2, spp);
+	float jx, jy;
+	sm.vec2(0, jx, jy);
 	const float kPi = 0x1.921fb6p+1f, kBsdf = 1.0f / kPi;
-	const float jx = (float)(sobol0(spp) >> 8) * 5.9604644775390625e-08f,
-	            jy = (float)(sobol1(spp) >> 8) * 5.9604644775390625e-08f;
-	auto colour = [](uint32_t w, int c) { return (float)((w >> (8 * c)) & 0xffu) / 255.0f; };
-	auto face = [](uint32_t code) {
-		V3 n{0.0f, 0.0f, 0.0f};
-		const float sg = (code & 1u) ? 1.0f : -1.0f;
-		(code >> 1 == 0 ? n.x : code >> 1 == 1 ? n.y : n.z) = sg;
-		return n;
-	};
 	auto mis = [](float a, float b) {
 		const float t = a * a;
 		return t / (b * b + t);
@@ -1056,6 +1093,8 @@ void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const
 	parallel_for((uint64_t)width * height, nthreads, [&](uint64_t b, uint64_t e) {
 		for (uint64_t kk = b; kk < e; ++kk) {
 			const uint32_t k = (uint32_t)kk, px = k % width, py = k / width;
+			float nu, nv;
+			sm.pixel_noise(px, py, nu, nv);
 			V3 o = cam_vec(cam, 0);
 			V3 d = gen_ray(cam, ((float)px + jx) / (float)width, ((float)py + jy) / (float)height);
 			orc_hit h = march_leaf(oc, o, d, 1);
@@ -1065,17 +1104,15 @@ void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const
 				env.lookup(d, t);
 				rad = V3{t[0], t[1], t[2]};
 			} else {
-				V3 acc{colour(h.leaf, 0), colour(h.leaf, 1), colour(h.leaf, 2)};
+				V3 acc{leaf_colour(h.leaf, 0), leaf_colour(h.leaf, 1), leaf_colour(h.leaf, 2)};
 				alb0 = acc;
 				uint32_t code = (h.vz_flags >> 24) & 7u;
-				nrm0 = face(code);
-				const uint32_t na = hash32(k ^ s2), nb = hash32(na ^ s3);
+				nrm0 = face_normal(code);
 				for (uint32_t cur = 1; cur < bounce; ++cur) {
-					const uint32_t su = (sobol0(spp) ^ hash32(s0 ^ cur)) + na, sv = (sobol1(spp) ^ hash32(s1 ^ cur)) + nb;
-					const float u = (float)(su >> 8) * 5.9604644775390625e-08f,
-					            v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+					float u, v;
+					sm.bounce(cur, nu, nv, u, v);
 					o = V3{h.pos[0], h.pos[1], h.pos[2]};
-					const V3 nrm = face(code);
+					const V3 nrm = face_normal(code);
 					{
 						V3 light;
 						float lpdf;
@@ -1089,16 +1126,7 @@ void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const
 							         rad.z + ((((acc.z * light.z) * multiplier) * wgt) * ndd) * kBsdf / lpdf};
 						}
 					}
-					// Diffuse_Sample + AlignDirection
-					float rr = std::sqrt(u), phi = 2.0f * kPi * v;
-					float sn, cs;
-					sincos_poly(phi, sn, cs);
-					V3 dl{rr * cs, rr * sn, std::sqrt(1.0f - u)};
-					V3 ax = std::fabs(nrm.x) > 0.01f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
-					V3 uu = normalize(cross(ax, nrm));
-					V3 vv = cross(nrm, uu);
-					d = V3{dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x, dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y,
-					       dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
+					d = diffuse_dir(code, u, v);
 					const float bpdf = std::sqrt(1.0f - u) / kPi;
 					h = march_leaf(oc, o, d, 1);
 					if (!(h.vz_flags & 0x80000000u)) {
@@ -1112,27 +1140,57 @@ void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const
 						}
 						break;
 					}
-					acc = V3{acc.x * colour(h.leaf, 0), acc.y * colour(h.leaf, 1), acc.z * colour(h.leaf, 2)};
+					acc = V3{acc.x * leaf_colour(h.leaf, 0), acc.y * leaf_colour(h.leaf, 1), acc.z * leaf_colour(h.leaf, 2)};
 					code = (h.vz_flags >> 24) & 7u;
 				}
 			}
-			const float n = (float)spp, n1 = (float)(spp + 1u);
-			auto mean = [&](float *img, V3 v) {
-				if (!img)
-					return;
-				float *p = img + 4 * (uint64_t)k;
-				if (spp == 0u) {
-					p[0] = v.x, p[1] = v.y, p[2] = v.z;
-				} else {
-					p[0] = (p[0] * n + v.x) / n1, p[1] = (p[1] * n + v.y) / n1, p[2] = (p[2] * n + v.z) / n1;
-				}
-				p[3] = 1.0f;
-			};
-			mean(color, rad);
-			mean(albedo, alb0);
-			mean(normal, nrm0);
+			store_sample(k, spp, rad, alb0, nrm0, color, albedo, normal);
 		}
 	});
+}
+
+// The reference tracer's shading of a primary frame (shader/octree_tracer.frag:18-47), into RGBA8
+// UNORM texels.  texels may be NULL (constant light).
+void orc_shade(const orc_hit *hits, const float *cam, uint32_t width, uint32_t height, int view, const float *light,
+               const float *texels, uint32_t ew, uint32_t eh, float rotation, uint32_t *out) {
+	const Env env{texels, nullptr, ew, eh, rotation, 1.0f};
+	for (uint64_t k = 0; k < (uint64_t)width * height; ++k) {
+		const orc_hit &h = hits[k];
+		const bool hit = (h.vz_flags & 0x80000000u) != 0u;
+		float c[3];
+		if (view == 3) { // Heat(iter / 128.0)
+			float x = (float)h.iter / 128.0f;
+			x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+			for (int i = 0; i < 3; ++i) {
+				float sn, cs;
+				sincos_poly(x * 3.0f - (float)(i + 1), sn, cs);
+				c[i] = sn * 0.5f + 0.5f;
+			}
+		} else if (view == 2) { // pos - 1.0; a miss has pos = 1
+			for (int i = 0; i < 3; ++i)
+				c[i] = hit ? h.pos[i] - 1.0f : 0.0f;
+		} else if (view == 1) { // normal * 0.5 + 0.5; a miss has normal = 0
+			const V3 n = hit ? face_normal((h.vz_flags >> 24) & 7u) : V3{0.0f, 0.0f, 0.0f};
+			c[0] = n.x * 0.5f + 0.5f, c[1] = n.y * 0.5f + 0.5f, c[2] = n.z * 0.5f + 0.5f;
+		} else { // pow(color, 1 / 2.2); a miss takes Light(d)
+			float col[3];
+			if (hit) {
+				for (int i = 0; i < 3; ++i)
+					col[i] = leaf_colour(h.leaf, i);
+			} else if (texels) {
+				const uint32_t px = (uint32_t)(k % width), py = (uint32_t)(k / width);
+				const V3 d = gen_ray(cam, (float)px / (float)width, (float)py / (float)height);
+				float t[4];
+				env.lookup(d, t);
+				col[0] = t[0], col[1] = t[1], col[2] = t[2];
+			} else {
+				col[0] = light[0], col[1] = light[1], col[2] = light[2];
+			}
+			for (int i = 0; i < 3; ++i)
+				c[i] = std::pow(col[i], 1.0f / 2.2f);
+		}
+		out[k] = unorm8(c[0]) | unorm8(c[1]) << 8 | unorm8(c[2]) << 16 | 0xFF000000u;
+	}
 }
 
 uint64_t orc_stack_uninit_reads(void) { return g_uninit_reads.load(); }

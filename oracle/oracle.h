@@ -38,21 +38,27 @@ typedef struct orc_hit {
 	uint32_t nfetch;   /* executed octree loads */
 } orc_hit;
 
-/* ConstColorPathTrace with running means (shader/path_tracer.comp:32-60,111-132) and the seeded
- * sampling of svo_path_trace: one sample per pixel into float4 color/albedo/normal (albedo and
- * normal may be NULL); spp = samples already accumulated. */
+/* The path tracer's sampler (shader/sobol.comp:78-86, shader/sobol.glsl:9-10, src/Sobol.cpp:52-62):
+ * the Sobol state after n advances from zero, dims words.  mat = the 64 x 32 generator table. */
+void orc_sobol_state(const uint32_t *mat, uint32_t dims, uint32_t n, uint32_t *x);
+
+/* ConstColorPathTrace with running means (shader/path_tracer.comp:32-60,111-132) for sample index
+ * spp (the Sobol counter): the primary ray through pixel + Sobol_GetVec2(0), bounce cur sampled by
+ * fract(Sobol_GetVec2(cur) + blue noise of pixel & 255).  color: float4 per pixel; albedo / normal:
+ * RGBA8 UNORM / RGBA8 SNORM per pixel (either may be NULL); spp = samples already accumulated.
+ * sobol_mat: 64 x 32 uint32; noise: 256 x 256 x 2 bytes. */
 void orc_path_trace(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
-                    uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, uint64_t seed, float *color,
-                    float *albedo, float *normal, int nthreads);
+                    uint32_t height, uint32_t bounce, uint32_t spp, const float *sky, const uint32_t *sobol_mat,
+                    const uint8_t *noise, float *color, uint32_t *albedo, uint32_t *normal, int nthreads);
 
 /* Environment-map tables (src/EnvironmentMap.cpp:104-188): texels 4 floats (rgb, pdf) and alias
  * {prob, alias} per texel from rgba (4 floats per texel); 0 when the map has no light. */
 int orc_envmap_tables(uint32_t w, uint32_t h, const float *rgba, float *texels, uint32_t *alias);
-/* EnvMapPathTrace with the sampling of svo_path_trace_env; buffers as orc_path_trace. */
+/* EnvMapPathTrace with the same sampler and buffers as orc_path_trace. */
 void orc_path_trace_env(const uint32_t *words, uint64_t W, uint32_t level, const float *cam, uint32_t width,
                         uint32_t height, uint32_t bounce, uint32_t spp, const float *texels, const uint32_t *alias,
-                        uint32_t ew, uint32_t eh, float rotation, float multiplier, uint64_t seed, float *color,
-                        float *albedo, float *normal, int nthreads);
+                        uint32_t ew, uint32_t eh, float rotation, float multiplier, const uint32_t *sobol_mat,
+                        const uint8_t *noise, float *color, uint32_t *albedo, uint32_t *normal, int nthreads);
 /* EnvMap_Sample (direction, radiance, pdf) and EnvMap_Radiance + EnvMap_PDF of one direction. */
 void orc_env_sample(const float *texels, const uint32_t *alias, uint32_t w, uint32_t h, float rotation, float u,
                     float v, float *dir, float *light, float *pdf);
@@ -101,13 +107,21 @@ void orc_march_frame_rows(const uint32_t *words, uint64_t W, uint32_t level, con
 void orc_march_rays(const uint32_t *words, uint64_t W, uint32_t level, const float *rays, uint64_t R, int variant,
                     orc_hit *hits, int nthreads);
 
-/* One cosine-weighted diffuse bounce from primary hits (shader/path_tracer.comp:38-44,
- * shader/diffuse.glsl:7-14, shader/util.glsl:5-9) with the seeded Sobol sample of pixel i.
- * rays[6*R]; a miss gets o = d = 0 and is flagged in mask[i] = 0 (mask may be NULL). */
-void orc_bounce_rays(const orc_hit *hits, uint64_t R, uint64_t seed, float *rays, uint8_t *mask);
+/* Bounce cur of sample spp from the primary hits of a width x height frame (shader/path_tracer.comp:
+ * 42-44, shader/diffuse.glsl:7-14, shader/util.glsl:5-9): o = o_pos, d = Diffuse_Sample(normal,
+ * fract(Sobol_GetVec2(cur) + noise(pixel & 255))).  rays[6*width*height]; a miss gets o = 0, d = -1
+ * and mask[i] = 0 (mask may be NULL). */
+void orc_bounce_rays(const orc_hit *hits, uint32_t width, uint32_t height, uint32_t spp, uint32_t cur,
+                     const uint32_t *sobol_mat, const uint8_t *noise, float *rays, uint8_t *mask);
 
 /* sin / cos of phi used by the bounce sampler (own polynomial; identical on the device). */
 void orc_sincos(float phi, float *s, float *c);
+
+/* The tracer's shading (shader/octree_tracer.frag:18-47) of a frame of hits into RGBA8 UNORM texels;
+ * view 0..3 = diffuse, normal, position, iteration heat; misses take light[3], or the environment
+ * map's radiance (texels != NULL, rotation; no multiplier).  pow is the host libm's. */
+void orc_shade(const orc_hit *hits, const float *cam, uint32_t width, uint32_t height, int view, const float *light,
+               const float *texels, uint32_t ew, uint32_t eh, float rotation, uint32_t *out);
 
 /* Number of stack reads of never-written entries seen since the last reset (must stay 0). */
 uint64_t orc_stack_uninit_reads(void);

@@ -21,6 +21,7 @@ from .api import (  # noqa: F401
     SvoError,
     Voxelizer,
     autotune,
+    bounce_rays,
     build_batch,
     camera_look_at,
     gen_scene,
@@ -28,6 +29,8 @@ from .api import (  # noqa: F401
     march_batch,
     march_rays,
     march_rays_batch,
+    rgba8_snorm,
+    rgba8_unorm,
     voxelize_batch,
 )
 from ._capi import LIB_PATH, exported_symbols  # noqa: F401

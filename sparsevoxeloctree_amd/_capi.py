@@ -147,8 +147,10 @@ _SIGS = {
     "svo_octree_load": (i32, [P, C.c_char_p, PP]),
     "svo_march": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, P, P]),
     "svo_march_beam": (i32, [P, P, C.POINTER(svo_camera), u32, u32, P, P]),
+    "svo_shade": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, C.POINTER(C.c_float), P, f32, P]),
+    "svo_octree_transfer": (i32, [P, P]),
     "svo_march_rays": (i32, [P, P, P, u64, i32, P]),
-    "svo_bounce_rays": (i32, [P, P, u64, u64, P]),
+    "svo_bounce_rays": (i32, [P, P, u32, u32, u32, u32, P]),
     "svo_march_batch": (i32, [P, PP, C.POINTER(svo_camera), u32, u32, u32, i32, P]),
     "svo_march_plan_create": (i32, [P, PP, C.POINTER(svo_camera), u32, u32, u32, i32, PP]),
     "svo_march_plan_set_cameras": (i32, [P, P, C.POINTER(svo_camera)]),
@@ -170,11 +172,11 @@ _SIGS = {
     "svo_autotune_report": (i32, [P, C.c_char_p, u64, C.POINTER(u64)]),
     "svo_camera_from_yaw_pitch": (i32, [C.POINTER(C.c_float), f32, f32, f32, f32, C.POINTER(svo_camera)]),
     "svo_gen_scene": (i32, [i32, u64, u64, P, P, C.POINTER(u64)]),
-    "svo_path_trace": (i32, [P, P, C.POINTER(svo_camera), u32, u32, u32, u32, C.POINTER(C.c_float), u64, P, P, P]),
+    "svo_path_trace": (i32, [P, P, C.POINTER(svo_camera), u32, u32, u32, u32, C.POINTER(C.c_float), P, P, P]),
     "svo_envmap_create": (i32, [P, u32, u32, P, PP]),
     "svo_envmap_download": (i32, [P, P, P, P]),
     "svo_envmap_destroy": (i32, [P]),
-    "svo_path_trace_env": (i32, [P, P, C.POINTER(svo_camera), u32, u32, u32, u32, P, f32, f32, u64, P, P, P]),
+    "svo_path_trace_env": (i32, [P, P, C.POINTER(svo_camera), u32, u32, u32, u32, P, f32, f32, P, P, P]),
     "svo_obj_load": (i32, [C.c_char_p, C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.POINTER(C.c_uint32)),
                            C.POINTER(u64), C.c_char_p, u64]),
     "svo_obj_free": (None, [C.POINTER(C.c_float), C.POINTER(C.c_uint32)]),

@@ -266,6 +266,11 @@ class Octree:
     def save(self, path):
         self.ctx.check(lib.svo_octree_save(self.ctx.h, self.h, str(path).encode()), "octree save")
 
+    def CmdTransferOwnership(self, dst_ctx):
+        """Orders work on dst_ctx's stream after this octree's build (svo_octree_transfer;
+        Octree::CmdTransferOwnership, src/Octree.cpp:37-43)."""
+        self.ctx.check(lib.svo_octree_transfer(self.h, dst_ctx.h), "octree transfer")
+
     def Update(self, builder):
         W = builder.GetOctreeRange() // 4
         self.ctx, self.h, self.level, self.W = builder.ctx, builder.h, builder.level, W
@@ -325,6 +330,12 @@ class OctreeBuilder:
         o.Update(self)
         return o
 
+    def CmdTransferOctreeOwnership(self, dst_ctx):
+        """Hands the octree being built to dst_ctx's stream: work enqueued there afterwards runs
+        after the build, without a host wait (OctreeBuilder::CmdTransferOctreeOwnership,
+        src/OctreeBuilder.cpp:215-220; svo_octree_transfer)."""
+        self.ctx.check(lib.svo_octree_transfer(self.h, dst_ctx.h), "octree transfer")
+
     def __del__(self):
         if getattr(self, "h", None) and self.ctx.h:
             lib.svo_octree_destroy(self.h)
@@ -361,15 +372,23 @@ def camera_look_at(eye, target, fov=math.pi / 3.0, aspect=1.0):
 
 
 class OctreeTracer:
-    """Primary-ray marching of a whole frame into a device hit buffer (svo_hit records)."""
+    """Primary-ray marching of a whole frame into a device hit buffer (svo_hit records), then the
+    reference tracer's shading of it into an RGBA8 image by m_view_type (OctreeTracer,
+    src/OctreeTracer.hpp:15-16, shader/octree_tracer.frag:18-47)."""
+
+    kDiffuse, kNormal, kPosition, kIteration = 0, 1, 2, 3  # OctreeTracer::ViewTypes
 
     def __init__(self, octree, camera):
         self.octree, self.camera = octree, camera
         self.ctx = octree.ctx
         self.m_beam_enable = True
+        self.m_view_type = OctreeTracer.kDiffuse
+        self.m_light = (5.0, 5.0, 5.0)  # the constant light (kDefaultConstantColor, src/Config.hpp:28)
+        self.m_env_map = None  # an EnvironmentMap: misses show its radiance instead
         self.width, self.height = 1280, 720
         self.hits = None
         self.beam = None
+        self.image = None
 
     @staticmethod
     def Create(octree, camera):
@@ -382,6 +401,7 @@ class OctreeTracer:
     def _ensure(self):
         if self.hits is None or self.hits.nbytes != self.width * self.height * 32:
             self.hits = self.ctx.alloc(self.width * self.height * 32)
+            self.image = self.ctx.alloc(self.width * self.height * 4)
             bw, bh = (self.width + 7) // 8 + 1, (self.height + 7) // 8 + 1
             self.beam = self.ctx.alloc(bw * bh * 4)
 
@@ -404,9 +424,24 @@ class OctreeTracer:
             ),
             "march",
         )
+        self.CmdShade()
+
+    def CmdShade(self):
+        """Shades the current hit buffer into the RGBA8 image by m_view_type."""
+        self._ensure()
+        cam = self.camera.uniform() if isinstance(self.camera, Camera) else self.camera
+        light = (C.c_float * 3)(*self.m_light)
+        env = self.m_env_map
+        self.ctx.check(lib.svo_shade(self.ctx.h, self.hits.ptr, C.byref(cam), self.width, self.height,
+                                     int(self.m_view_type), light, env.h if env is not None else None,
+                                     env.m_rotation if env is not None else 0.0, self.image.ptr), "shade")
 
     def GetHits(self):
         return self.hits.download(HIT_DTYPE, self.width * self.height).reshape(self.height, self.width)
+
+    def GetImage(self):
+        """(height, width) uint32 RGBA8 UNORM texels (R in the low byte)."""
+        return self.image.download(np.uint32, self.width * self.height).reshape(self.height, self.width)
 
     def GetBeam(self):
         bw, bh = (self.width + 7) // 8 + 1, (self.height + 7) // 8 + 1
@@ -456,9 +491,10 @@ class EnvironmentMap:
 
 
 class PathTracer:
-    """Progressive path tracing with a constant sky (PathTracer, src/PathTracer.cpp:176-215;
-    ConstColorPathTrace, shader/path_tracer.comp:32-60): each CmdRender adds one sample per pixel to
-    the running means of colour, first-hit albedo and first-hit normal (float4 per pixel)."""
+    """Progressive path tracing (PathTracer, src/PathTracer.cpp:176-215; ConstColorPathTrace /
+    EnvMapPathTrace, shader/path_tracer.comp): each CmdRender adds sample spp to the running means of
+    colour (RGBA32F), first-hit albedo (RGBA8 UNORM) and first-hit normal (RGBA8 SNORM), drawing it
+    with the reference's sampler (Sobol state after spp advances, 256x256 blue noise)."""
 
     def __init__(self, octree, camera):
         self.octree, self.camera = octree, camera
@@ -466,7 +502,6 @@ class PathTracer:
         self.m_width, self.m_height = 1280, 720
         self.m_bounce = 4  # kDefaultBounce, src/Config.hpp:25
         self.m_sun_radiance = (5.0, 5.0, 5.0)  # kDefaultConstantColor, src/Config.hpp:28 (uConstColor)
-        self.seed = 0x5EED
         self.spp = 0
         self.color = self.albedo = self.normal = None
         self.m_env_map = None  # an EnvironmentMap: EnvMapPathTrace instead of the constant colour
@@ -485,9 +520,10 @@ class PathTracer:
         self.spp = 0
 
     def _ensure(self):
-        n = self.m_width * self.m_height * 16
-        if self.color is None or self.color.nbytes != n:
-            self.color, self.albedo, self.normal = (self.ctx.alloc(n) for _ in range(3))
+        n = self.m_width * self.m_height
+        if self.color is None or self.color.nbytes != n * 16:
+            self.color = self.ctx.alloc(n * 16)
+            self.albedo, self.normal = self.ctx.alloc(n * 4), self.ctx.alloc(n * 4)
             self.spp = 0
 
     def CmdRender(self):
@@ -496,30 +532,50 @@ class PathTracer:
         env = self.m_env_map
         if env is not None:
             rc = lib.svo_path_trace_env(self.ctx.h, self.octree.h, C.byref(cam), self.m_width, self.m_height,
-                                        self.m_bounce, self.spp, env.h, env.m_rotation, env.m_multiplier, self.seed,
+                                        self.m_bounce, self.spp, env.h, env.m_rotation, env.m_multiplier,
                                         self.color.ptr, self.albedo.ptr, self.normal.ptr)
         else:
             sky = (C.c_float * 3)(*self.m_sun_radiance)
             rc = lib.svo_path_trace(self.ctx.h, self.octree.h, C.byref(cam), self.m_width, self.m_height,
-                                    self.m_bounce, self.spp, sky, self.seed, self.color.ptr, self.albedo.ptr,
-                                    self.normal.ptr)
+                                    self.m_bounce, self.spp, sky, self.color.ptr, self.albedo.ptr, self.normal.ptr)
         self.ctx.check(rc, "path trace")
-        self.spp += 1
+        self.spp += 1  # Sobol::CmdNext after the dispatch (src/PathTracer.cpp:214)
 
     def GetSampleCount(self):
         return self.spp
 
-    def _get(self, buf):
-        return buf.download(np.float32, self.m_width * self.m_height * 4).reshape(self.m_height, self.m_width, 4)
-
     def GetColor(self):
-        return self._get(self.color)
+        """(height, width, 4) float32."""
+        return self.color.download(np.float32, self.m_width * self.m_height * 4).reshape(self.m_height, self.m_width, 4)
 
     def GetAlbedo(self):
-        return self._get(self.albedo)
+        """(height, width) uint32 RGBA8 UNORM texels (R in the low byte)."""
+        return self.albedo.download(np.uint32, self.m_width * self.m_height).reshape(self.m_height, self.m_width)
 
     def GetNormal(self):
-        return self._get(self.normal)
+        """(height, width) uint32 RGBA8 SNORM texels."""
+        return self.normal.download(np.uint32, self.m_width * self.m_height).reshape(self.m_height, self.m_width)
+
+
+def rgba8_unorm(texels):
+    """(…, 4) float32 of RGBA8 UNORM texels: byte / 255."""
+    b = np.asarray(texels, dtype=np.uint32)[..., None] >> np.array([0, 8, 16, 24], np.uint32)
+    return (b & 0xFF).astype(np.float32) / np.float32(255.0)
+
+
+def rgba8_snorm(texels):
+    """(…, 4) float32 of RGBA8 SNORM texels: max(signed byte / 127, -1)."""
+    b = np.asarray(texels, dtype=np.uint32)[..., None] >> np.array([0, 8, 16, 24], np.uint32)
+    v = (b & 0xFF).astype(np.uint8).view(np.int8).astype(np.float32) / np.float32(127.0)
+    return np.maximum(v, np.float32(-1.0))
+
+
+def bounce_rays(ctx, hits, width, height, spp, cur=1):
+    """Bounce `cur` of path-tracer sample spp from a device buffer of width*height primary hits:
+    a device buffer of width*height x 6 floats (svo_bounce_rays)."""
+    rays = ctx.alloc(width * height * 24)
+    ctx.check(lib.svo_bounce_rays(ctx.h, hits.ptr, width, height, spp, cur, rays.ptr), "bounce rays")
+    return rays
 
 
 def march_rays(ctx, octree, rays, variant=_capi.SVO_MARCH_LEAF_ITER):

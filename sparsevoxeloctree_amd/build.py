@@ -29,6 +29,9 @@ SOURCES = [
     "svo_obj.cpp",
 ]
 
+DATA = os.path.join(HERE, "data")
+GEN = os.path.join(BUILD, "gen")
+
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("SVO_OFFLOAD_ARCH", "gfx950")
 
@@ -41,6 +44,7 @@ COMMON = [
     "-Wall",
     "-Wno-unused-function",
     "-I" + os.path.join(ROOT, "include"),
+    "-I" + GEN,
 ]
 DEVICE = [
     "--offload-arch=" + ARCH,
@@ -60,8 +64,34 @@ def _compile(src):
     return obj
 
 
+def write_tables():
+    """The sampler's data tables (data/*.bin, see data/make_tables.py) as C initializer lists in
+    _build/gen/, included by svo_march.hip: the Sobol matrices as 2048 uint32 and the blue-noise
+    texture as 65536 uint16 texels (R | G << 8)."""
+    import numpy as np
+
+    os.makedirs(GEN, exist_ok=True)
+    sob = np.fromfile(os.path.join(DATA, "sobol_matrices.bin"), dtype="<u4")
+    noise = np.fromfile(os.path.join(DATA, "blue_noise_rg8.bin"), dtype=np.uint8)
+    if sob.size != 64 * 32 or noise.size != 256 * 256 * 2:
+        raise RuntimeError("sampler tables have the wrong size")
+    texels = noise[0::2].astype(np.uint32) | (noise[1::2].astype(np.uint32) << 8)
+    for name, vals, fmt in (("svo_sobol_matrices.inc", sob, "0x%08xu"), ("svo_blue_noise.inc", texels, "0x%04xu")):
+        lines = [",".join(fmt % int(v) for v in vals[i:i + 16]) + "," for i in range(0, vals.size, 16)]
+        text = "\n".join(lines) + "\n"
+        path = os.path.join(GEN, name)
+        old = None
+        if os.path.exists(path):
+            with open(path) as f:
+                old = f.read()
+        if old != text:
+            with open(path, "w") as f:
+                f.write(text)
+
+
 def build_library(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
+    write_tables()
     os.makedirs(LIB_DIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
         objs = list(ex.map(_compile, SOURCES))

@@ -301,6 +301,7 @@ int svo_ctx_create(int device, svo_ctx **out) {
 	if (hipMalloc((void **)&ctx->d_scan_parts, sizeof(uint64_t) * (2 * kScanBlocks + 8)) != hipSuccess ||
 	    hipMalloc((void **)&ctx->d_state, sizeof(BuildState)) != hipSuccess ||
 	    hipMalloc((void **)&ctx->d_err, sizeof(uint32_t)) != hipSuccess ||
+	    hipMalloc((void **)&ctx->d_sobol, sizeof(uint32_t) * 65) != hipSuccess ||
 	    hipHostMalloc((void **)&ctx->h_state, sizeof(BuildState) + 64, hipHostMallocDefault) != hipSuccess ||
 	    ctx->stage.init() != hipSuccess) {
 		svo_ctx_destroy(ctx);
@@ -332,6 +333,8 @@ int svo_ctx_destroy(svo_ctx *ctx) {
 		(void)hipFree(ctx->d_state);
 	if (ctx->d_err)
 		(void)hipFree(ctx->d_err);
+	if (ctx->d_sobol)
+		(void)hipFree(ctx->d_sobol);
 	if (ctx->h_state)
 		(void)hipHostFree(ctx->h_state);
 	ctx->stage.destroy();
@@ -911,6 +914,28 @@ static_assert(sizeof(OctreeFileHeader) == 24, "octree file header is 24 bytes");
 const char kOctreeMagic[8] = {'S', 'V', 'O', 'C', 'T', 'R', 'E', '1'};
 } // namespace
 
+int svo_octree_transfer(const svo_octree *oct, svo_ctx *dst) {
+	if (!oct || !dst || !oct->ctx)
+		return SVO_ERR_INVALID_ARGUMENT;
+	svo_ctx *src = oct->ctx;
+	if (src->device != dst->device)
+		return set_error(dst, SVO_ERR_INVALID_ARGUMENT, "octree is on another device");
+	if (src == dst)
+		return SVO_OK; // one stream: already ordered
+	DeviceGuard g(src);
+	hipEvent_t ev = nullptr;
+	hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+	if (e == hipSuccess)
+		e = hipEventRecord(ev, src->stream); // release: after every kernel enqueued on the builder's stream
+	if (e == hipSuccess)
+		e = hipStreamWaitEvent(dst->stream, ev, 0); // acquire: later work on dst waits for it
+	if (ev)
+		(void)hipEventDestroy(ev); // the wait keeps the recorded state
+	if (e != hipSuccess)
+		return hip_check(dst, e, "octree transfer");
+	return SVO_OK;
+}
+
 int svo_octree_save(svo_ctx *ctx, const svo_octree *o, const char *path) {
 	if (!ctx || !o || !path || o->pending)
 		return SVO_ERR_INVALID_ARGUMENT;
@@ -976,6 +1001,20 @@ int svo_march_beam(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uin
 	return march_beam(ctx, o, cam, width, height, beam_dev, nfetch_dev);
 }
 
+int svo_shade(svo_ctx *ctx, const svo_hit *hits_dev, const svo_camera *cam, uint32_t width, uint32_t height,
+              int view_type, const float light[3], const svo_envmap *env, float env_rotation, uint32_t *rgba8_dev) {
+	if (!ctx || !cam || !light || ((uint64_t)width * height && (!hits_dev || !rgba8_dev)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (view_type < 0 || view_type > 3)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "view type must be 0..3 (diffuse, normal, position, iteration)");
+	if ((uint64_t)width * height > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame too large");
+	if (env && env->ctx->device != ctx->device)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "environment map is on another device");
+	DeviceGuard g(ctx);
+	return shade_frame(ctx, hits_dev, cam, width, height, (uint32_t)view_type, light, env, env_rotation, rgba8_dev);
+}
+
 int svo_march_rays(svo_ctx *ctx, const svo_octree *o, const float *rays_dev, uint64_t R, int variant,
                    svo_hit *hits_dev) {
 	if (!ctx || !o || (R && (!rays_dev || !hits_dev)) || variant < 0 || variant > 2)
@@ -986,28 +1025,32 @@ int svo_march_rays(svo_ctx *ctx, const svo_octree *o, const float *rays_dev, uin
 	return march_rays(ctx, o, rays_dev, R, variant, hits_dev);
 }
 
-int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint64_t R, uint64_t seed, float *rays_dev) {
+int svo_bounce_rays(svo_ctx *ctx, const svo_hit *hits_dev, uint32_t width, uint32_t height, uint32_t spp,
+                    uint32_t cur, float *rays_dev) {
+	const uint64_t R = (uint64_t)width * height;
 	if (!ctx || (R && (!hits_dev || !rays_dev)))
 		return SVO_ERR_INVALID_ARGUMENT;
-	if (R > 0xFFFFFFFFull)
-		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "too many rays in one call");
+	if (cur < 1 || cur > 31)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce index must be 1..31 (64 Sobol dimensions)");
+	if (R > 0xFFFFFFFFull || spp == 0xFFFFFFFFu)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame or sample index too large");
 	DeviceGuard g(ctx);
-	return bounce_rays(ctx, hits_dev, R, seed, rays_dev);
+	return bounce_rays(ctx, hits_dev, width, height, spp, cur, rays_dev);
 }
 
 int svo_path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
-                   uint32_t bounce, uint32_t spp, const float sky[3], uint64_t seed, float *color_dev,
-                   float *albedo_dev, float *normal_dev) {
+                   uint32_t bounce, uint32_t spp, const float sky[3], float *color_dev, uint32_t *albedo_dev,
+                   uint32_t *normal_dev) {
 	if (!ctx || !oct || !cam || !sky || !color_dev || width == 0 || height == 0)
 		return SVO_ERR_INVALID_ARGUMENT;
-	if (bounce < 1 || bounce > 64)
-		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce must be 1..64");
+	if (bounce < 1 || bounce > 31)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce must be 1..31 (64 Sobol dimensions)");
 	if ((uint64_t)width * height > 0xFFFFFFFFull || spp == 0xFFFFFFFFu)
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame or sample count too large");
 	if (!oct->words || oct->W < 8)
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "octree is empty");
 	DeviceGuard g(ctx);
-	return path_trace(ctx, oct, cam, width, height, bounce, spp, sky, seed, color_dev, albedo_dev, normal_dev);
+	return path_trace(ctx, oct, cam, width, height, bounce, spp, sky, color_dev, albedo_dev, normal_dev);
 }
 
 // ---- environment maps (EnvironmentMap::Reset, src/EnvironmentMap.cpp:55-190) ----------------------
@@ -1129,11 +1172,11 @@ int svo_envmap_destroy(svo_envmap *env) {
 
 int svo_path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
                        uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
-                       uint64_t seed, float *color_dev, float *albedo_dev, float *normal_dev) {
+                       float *color_dev, uint32_t *albedo_dev, uint32_t *normal_dev) {
 	if (!ctx || !oct || !cam || !env || !color_dev || width == 0 || height == 0)
 		return SVO_ERR_INVALID_ARGUMENT;
-	if (bounce < 1 || bounce > 64)
-		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce must be 1..64");
+	if (bounce < 1 || bounce > 31)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "bounce must be 1..31 (64 Sobol dimensions)");
 	if ((uint64_t)width * height > 0xFFFFFFFFull || spp == 0xFFFFFFFFu)
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame or sample count too large");
 	if (!oct->words || oct->W < 8)
@@ -1141,8 +1184,8 @@ int svo_path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *ca
 	if (env->ctx != ctx)
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "environment map belongs to another context");
 	DeviceGuard g(ctx);
-	return path_trace_env(ctx, oct, cam, width, height, bounce, spp, env, rotation, multiplier, seed, color_dev,
-	                      albedo_dev, normal_dev);
+	return path_trace_env(ctx, oct, cam, width, height, bounce, spp, env, rotation, multiplier, color_dev, albedo_dev,
+	                      normal_dev);
 }
 
 // ---- batched tracing

@@ -624,16 +624,44 @@ __global__ void __launch_bounds__(kT)
 		atomicOr(err_out, err);
 }
 
-// ---- bounce sampling ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t sobol0(uint32_t n) { return __brev(n); }
-__device__ __forceinline__ uint32_t sobol1(uint32_t n) { // primitive polynomial x + 1
-	uint32_t v = 0x80000000u, r = 0u;
-	for (int k = 0; k < 32; ++k) {
-		if ((n >> k) & 1u)
-			r ^= v;
-		v ^= v >> 1;
-	}
-	return r;
+// ---- the path tracer's sampler (shader/path_tracer.comp:42,72,113-117, shader/sobol.glsl:9-10) ----
+// The Sobol generator matrices of shader/sobol.comp:4-86 (64 dimensions x 32 columns) and the 256x256
+// RG8 blue-noise texture of src/Noise.inl, compiled in from the package's data files.
+__device__ const uint32_t g_sobol_mat[64 * 32] = {
+#include "svo_sobol_matrices.inc"
+};
+__device__ const uint16_t g_blue_noise[256 * 256] = { // texel (x, y) at y * 256 + x: R | G << 8
+#include "svo_blue_noise.inc"
+};
+
+// Sobol state of sample index n in all 64 dimensions, into out[0..63] (out[64] = n, the counter of
+// shader/sobol.comp).  n advances of sobol.comp from zero XOR column findLSB(~i) in at step i,
+// which is the bit where the Gray codes of i and i + 1 differ, so the state is the XOR of the
+// columns of the set bits of gray(n) = n ^ (n >> 1).  One block of 64 threads.
+__global__ void __launch_bounds__(64) sobol_point_kernel(uint32_t n, uint32_t *__restrict__ out) {
+	const uint32_t d = threadIdx.x, g = n ^ (n >> 1);
+	uint32_t x = 0u;
+	for (uint32_t c = 0; c < 32u; ++c)
+		if ((g >> c) & 1u)
+			x ^= g_sobol_mat[d * 32u + c];
+	out[d] = x;
+	if (d == 0u)
+		out[64] = n;
+}
+
+// Sobol_GetVec2 component: uint / 2^32.
+__device__ __forceinline__ float sobol_unit(uint32_t x) { return (float)x / 4294967296.0f; }
+// texelFetch(uNoise, pixel & 0xff): both UNORM8 channels of the texel, packed R | G << 8.
+__device__ __forceinline__ uint32_t noise_texel(uint32_t px, uint32_t py) {
+	return g_blue_noise[((py & 255u) << 8) | (px & 255u)];
+}
+// Bounce cur's sample: fract(Sobol_GetVec2(cur) + noise) (shader/path_tracer.comp:42,72).
+__device__ __forceinline__ void bounce_sample(const uint32_t *__restrict__ sob, uint32_t cur, uint32_t nz, float &u,
+                                              float &v) {
+	const float s = sobol_unit(sob[2u * cur]) + (float)(nz & 0xffu) / 255.0f;
+	const float t = sobol_unit(sob[2u * cur + 1u]) + (float)(nz >> 8) / 255.0f;
+	u = s - floorf(s);
+	v = t - floorf(t);
 }
 
 // sin/cos: quadrant reduction with a two-constant split of pi/2, Taylor polynomials on [-pi/4, pi/4].
@@ -657,54 +685,6 @@ __device__ __forceinline__ void sincos_poly(float phi, float &s, float &c) {
 		s = -cp, c = sp;
 }
 
-__global__ void __launch_bounds__(kBlock)
-    bounce_kernel(const uint4 *__restrict__ hits, uint64_t R, uint32_t s0, uint32_t s1, float *__restrict__ rays) {
-	const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-	if (k >= R)
-		return;
-	const uint4 h0 = hits[2 * k], h1 = hits[2 * k + 1];
-	float *r = rays + 6 * k;
-	if (!(h1.y & 0x80000000u)) {
-		r[0] = r[1] = r[2] = 0.0f;
-		r[3] = r[4] = r[5] = -1.0f;
-		return;
-	}
-	const float kPi = 3.1415926535897932384626433832795f; // shader/util.glsl:4
-	const uint32_t n = (uint32_t)k;
-	float u = (float)((sobol0(n) ^ s0) >> 8) * 5.9604644775390625e-08f;
-	float v = (float)((sobol1(n) ^ s1) >> 8) * 5.9604644775390625e-08f;
-	const uint32_t code = (h1.y >> 24) & 7u;
-	const float sg = (code & 1u) ? 1.0f : -1.0f;
-	f3 nrm{code >> 1 == 0u ? sg : 0.0f, code >> 1 == 1u ? sg : 0.0f, code >> 1 == 2u ? sg : 0.0f};
-	// Diffuse_Sample (shader/diffuse.glsl:7-14)
-	float rr = __builtin_sqrtf(u), phi = 2.0f * kPi * v;
-	float sn, cs;
-	sincos_poly(phi, sn, cs);
-	f3 dl{rr * cs, rr * sn, __builtin_sqrtf(1.0f - u)};
-	// AlignDirection (shader/util.glsl:5-9)
-	f3 ax = fabsf(nrm.x) > 0.01f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
-	f3 uu = normalize3(cross3(ax, nrm));
-	f3 vv = cross3(nrm, uu);
-	r[0] = __uint_as_float(h0.x);
-	r[1] = __uint_as_float(h0.y);
-	r[2] = __uint_as_float(h0.z);
-	r[3] = dl.x * uu.x + dl.y * vv.x + dl.z * nrm.x;
-	r[4] = dl.x * uu.y + dl.y * vv.y + dl.z * nrm.y;
-	r[5] = dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z;
-}
-
-// ---- progressive path tracing (ConstColorPathTrace + accumulation, shader/path_tracer.comp:32-60,
-// 111-132) --------------------------------------------------------------------------------------
-// lowbias32 integer hash (C. Wellons' hash-prospector).
-__device__ __forceinline__ uint32_t hash32(uint32_t x) {
-	x ^= x >> 16;
-	x *= 0x7feb352du;
-	x ^= x >> 15;
-	x *= 0x846ca68bu;
-	x ^= x >> 16;
-	return x;
-}
-
 // Diffuse_Sample + AlignDirection (shader/diffuse.glsl:7-14, shader/util.glsl:5-9) around the axis
 // normal of face code `code` (axis * 2 + positive), for the sample (u, v).
 __device__ __forceinline__ f3 diffuse_dir(uint32_t code, float u, float v, f3 &nrm) {
@@ -722,30 +702,61 @@ __device__ __forceinline__ f3 diffuse_dir(uint32_t code, float u, float v, f3 &n
 	          dl.x * uu.z + dl.y * vv.z + dl.z * nrm.z};
 }
 
+// Bounce cur of sample index spp for every primary hit of a width-wide frame (ray k = pixel
+// (k % width, k / width)): o = o_pos, d = Diffuse_Sample(normal, fract(Sobol_GetVec2(cur) + noise))
+// (shader/path_tracer.comp:42-44).  A miss gets o = 0, d = -1.
+__global__ void __launch_bounds__(kBlock) bounce_kernel(const uint4 *__restrict__ hits, uint64_t R, uint32_t width,
+                                                        const uint32_t *__restrict__ sob, uint32_t cur,
+                                                        float *__restrict__ rays) {
+	const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	if (k >= R)
+		return;
+	const uint4 h0 = hits[2 * k], h1 = hits[2 * k + 1];
+	float *r = rays + 6 * k;
+	if (!(h1.y & 0x80000000u)) {
+		r[0] = r[1] = r[2] = 0.0f;
+		r[3] = r[4] = r[5] = -1.0f;
+		return;
+	}
+	const uint32_t px = (uint32_t)(k % width), py = (uint32_t)(k / width);
+	float u, v;
+	bounce_sample(sob, cur, noise_texel(px, py), u, v);
+	f3 nrm;
+	const f3 d = diffuse_dir((h1.y >> 24) & 7u, u, v, nrm);
+	r[0] = __uint_as_float(h0.x);
+	r[1] = __uint_as_float(h0.y);
+	r[2] = __uint_as_float(h0.z);
+	r[3] = d.x;
+	r[4] = d.y;
+	r[5] = d.z;
+}
+
+// ---- progressive path tracing (ConstColorPathTrace + accumulation, shader/path_tracer.comp:32-60,
+// 111-132) --------------------------------------------------------------------------------------
+
 struct PathArgs {
 	CamArgs cam;
 	uint32_t width, height, bounce, spp;
 	float sky[3];
-	uint32_t s0, s1, s2, s3; // sampling keys from the seed
+	const uint32_t *sobol; // Sobol state of sample spp (sobol_point_kernel)
 	uint32_t tiles_x;
-	float4 *color, *albedo, *normal;
+	float4 *color;             // RGBA32F running mean
+	uint32_t *albedo, *normal; // RGBA8 UNORM / RGBA8 SNORM running means
 };
 
 // The steps of one path, shared by both schedules so that they give the same bits.
-// Primary ray of pixel (px, py): through the pixel plus the frame's 2-D Sobol point.
+// Primary ray of pixel (px, py): through pixel + Sobol_GetVec2(0) (shader/path_tracer.comp:117).
 __device__ __forceinline__ void path_primary(const PathArgs &a, uint32_t px, uint32_t py, f3 &o, f3 &d) {
-	const float jx = (float)(sobol0(a.spp) >> 8) * 5.9604644775390625e-08f,
-	            jy = (float)(sobol1(a.spp) >> 8) * 5.9604644775390625e-08f;
+	const float jx = sobol_unit(a.sobol[0]), jy = sobol_unit(a.sobol[1]);
 	o = f3{a.cam.px, a.cam.py, a.cam.pz};
 	d = gen_ray(a.cam, ((float)px + jx) / (float)a.width, ((float)py + jy) / (float)a.height);
 }
-// Bounce cur >= 1 of pixel k from a hit record: origin o_pos, cosine-weighted direction about the
-// hit face's normal.
-__device__ __forceinline__ void path_bounce(const PathArgs &a, uint32_t k, uint32_t cur, uint4 h0, uint32_t code, f3 &o,
-                                            f3 &d) {
-	const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3); // per-pixel rotation
-	const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
-	const float u = (float)(su >> 8) * 5.9604644775390625e-08f, v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+// Bounce cur >= 1 from a hit record: origin o_pos, cosine-weighted direction about the hit face's
+// normal for the sample fract(Sobol_GetVec2(cur) + noise); nz = the pixel's noise texel.
+__device__ __forceinline__ void path_bounce(const PathArgs &a, uint32_t nz, uint32_t cur, uint4 h0, uint32_t code,
+                                            f3 &o, f3 &d) {
+	float u, v;
+	bounce_sample(a.sobol, cur, nz, u, v);
 	f3 nrm;
 	o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
 	d = diffuse_dir(code, u, v, nrm);
@@ -758,8 +769,8 @@ __device__ __forceinline__ f3 face_normal(uint32_t code) {
 	const float sg = (code & 1u) ? 1.0f : -1.0f;
 	return f3{code >> 1 == 0u ? sg : 0.0f, code >> 1 == 1u ? sg : 0.0f, code >> 1 == 2u ? sg : 0.0f};
 }
-// Running mean of one pixel (shader/path_tracer.comp:126-133); spp 0 starts it (the reference
-// clears its images first).
+// Running mean of one pixel (shader/path_tracer.comp:123-131): (old * spp + v) / (spp + 1); spp 0
+// starts it (the reference clears its images first).
 __device__ __forceinline__ void path_mean(const PathArgs &a, float4 *img, uint32_t k, f3 v) {
 	if (!img)
 		return;
@@ -771,15 +782,42 @@ __device__ __forceinline__ void path_mean(const PathArgs &a, float4 *img, uint32
 	const float4 p = img[k];
 	img[k] = make_float4((p.x * n + v.x) / n1, (p.y * n + v.y) / n1, (p.z * n + v.z) / n1, 1.0f);
 }
+// The same mean through an RGBA8 texel (src/PathTracer.cpp:13-16): imageLoad gives byte / 255
+// (UNORM) or max(signed byte / 127, -1) (SNORM); imageStore rounds the clamped value times 255 (127)
+// to nearest, halves up; alpha stores 1.
+template <bool kSnorm> __device__ __forceinline__ float rgba8_load(uint32_t b) {
+	if (kSnorm) {
+		const float f = (float)(int32_t)(int8_t)(uint8_t)(b & 0xffu) / 127.0f;
+		return f < -1.0f ? -1.0f : f;
+	}
+	return (float)(b & 0xffu) / 255.0f;
+}
+template <bool kSnorm> __device__ __forceinline__ uint32_t rgba8_store(float f) {
+	if (kSnorm) {
+		f = f < -1.0f ? -1.0f : (f > 1.0f ? 1.0f : f);
+		return (uint32_t)((int32_t)floorf(f * 127.0f + 0.5f) & 0xff);
+	}
+	f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+	return (uint32_t)floorf(f * 255.0f + 0.5f);
+}
+template <bool kSnorm> __device__ __forceinline__ void path_mean8(const PathArgs &a, uint32_t *img, uint32_t k, f3 v) {
+	if (!img)
+		return;
+	const uint32_t old = a.spp == 0u ? 0u : img[k];
+	const float n = (float)a.spp, n1 = (float)(a.spp + 1u);
+	const float m0 = (rgba8_load<kSnorm>(old) * n + v.x) / n1, m1 = (rgba8_load<kSnorm>(old >> 8) * n + v.y) / n1,
+	            m2 = (rgba8_load<kSnorm>(old >> 16) * n + v.z) / n1;
+	img[k] = rgba8_store<kSnorm>(m0) | rgba8_store<kSnorm>(m1) << 8 | rgba8_store<kSnorm>(m2) << 16 |
+	         (kSnorm ? 127u : 255u) << 24;
+}
 
-// One sample per pixel of a progressive render: the camera ray through pixel + the frame's 2-D Sobol
-// point (the pixel corner at spp 0), then up to bounce-1 cosine-weighted diffuse bounces; a path
-// that escapes brings the constant sky colour times the product of the albedos it met.  Sampling
-// replaces the reference's running Sobol dimensions and 256^2 blue-noise table: bounce `cur` uses
-// the frame's 2-D Sobol point (index spp) XOR-shifted by a hash of cur, rotated (fract of the sum)
-// by a per-pixel hash, all in 32-bit integers.  The colour, first-hit albedo and first-hit normal
-// are running means over spp + 1 samples (float4 per pixel, w = 1).  This schedule: one path per
-// lane, 8x8-pixel tiles.
+// Sample spp of every pixel of a progressive render (ConstColorPathTrace + main,
+// shader/path_tracer.comp:32-60,111-132): the camera ray through pixel + Sobol_GetVec2(0) (the pixel
+// corner at spp 0), then up to bounce-1 cosine-weighted diffuse bounces sampled by
+// fract(Sobol_GetVec2(cur) + blue noise); a path that escapes brings the constant sky colour times
+// the product of the albedos it met.  The colour (RGBA32F), first-hit albedo (RGBA8 UNORM) and
+// first-hit normal (RGBA8 SNORM) are running means over spp + 1 samples.  This schedule: one path
+// per lane, 8x8-pixel tiles.
 __global__ void __launch_bounds__(64) path_trace_kernel(const uint32_t *__restrict__ oct, uint64_t W, PathArgs a,
                                                         uint32_t *err_out) {
 	__shared__ uint32_t s_stack[kStackSlots * 64];
@@ -788,7 +826,7 @@ __global__ void __launch_bounds__(64) path_trace_kernel(const uint32_t *__restri
 	if (px >= a.width || py >= a.height)
 		return;
 	uint32_t err = 0u;
-	const uint32_t k = py * a.width + px;
+	const uint32_t k = py * a.width + px, nz = noise_texel(px, py);
 	f3 o, d;
 	path_primary(a, px, py, o, d);
 	uint4 h0, h1;
@@ -801,7 +839,7 @@ __global__ void __launch_bounds__(64) path_trace_kernel(const uint32_t *__restri
 		nrm0 = face_normal(code);
 		rad = f3{0.0f, 0.0f, 0.0f};
 		for (uint32_t cur = 1; cur < a.bounce; ++cur) {
-			path_bounce(a, k, cur, h0, code, o, d);
+			path_bounce(a, nz, cur, h0, code, o, d);
 			march_leaf<1, 64>(oct, W, o, d, s_stack + lane, err, h0, h1);
 			if (!(h1.y & 0x80000000u)) {
 				rad = f3{acc.x * a.sky[0], acc.y * a.sky[1], acc.z * a.sky[2]};
@@ -813,8 +851,8 @@ __global__ void __launch_bounds__(64) path_trace_kernel(const uint32_t *__restri
 		}
 	}
 	path_mean(a, a.color, k, rad);
-	path_mean(a, a.albedo, k, alb0);
-	path_mean(a, a.normal, k, nrm0);
+	path_mean8<false>(a, a.albedo, k, alb0);
+	path_mean8<true>(a, a.normal, k, nrm0);
 	if (err)
 		atomicOr(err_out, err);
 }
@@ -917,7 +955,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) pa
 		return;
 	const float kPi = 0x1.921fb6p+1f, kBsdf = 1.0f / kPi;
 	uint32_t err = 0u;
-	const uint32_t k = py * a.width + px;
+	const uint32_t k = py * a.width + px, nz = noise_texel(px, py);
 	f3 o, d;
 	path_primary(a, px, py, o, d);
 	uint4 h0, h1;
@@ -931,10 +969,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) pa
 		alb0 = acc;
 		uint32_t code = (h1.y >> 24) & 7u;
 		nrm0 = face_normal(code);
-		const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3);
 		for (uint32_t cur = 1; cur < a.bounce; ++cur) {
-			const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
-			const float u = (float)(su >> 8) * 5.9604644775390625e-08f, v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+			float u, v;
+			bounce_sample(a.sobol, cur, nz, u, v);
 			o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
 			const f3 nrm = face_normal(code);
 			{ // light sample
@@ -973,8 +1010,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8))) pa
 		}
 	}
 	path_mean(a, a.color, k, rad);
-	path_mean(a, a.albedo, k, alb0);
-	path_mean(a, a.normal, k, nrm0);
+	path_mean8<false>(a, a.albedo, k, alb0);
+	path_mean8<true>(a, a.normal, k, nrm0);
 	if (err)
 		atomicOr(err_out, err);
 }
@@ -993,7 +1030,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) pa
 	const uint64_t below = (1ull << lane) - 1ull;
 	March m;
 	f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f}, acc{0.0f, 0.0f, 0.0f};
-	uint32_t k = 0u, cur = 0u, iter = 0u, nfetch = 0u, err = 0u;
+	uint32_t k = 0u, nz = 0u, cur = 0u, iter = 0u, nfetch = 0u, err = 0u;
 	bool active = false;
 	uint64_t pool_next = 0, pool_end = 0, chunk = wave;
 	bool exhausted = false;
@@ -1018,6 +1055,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) pa
 					const uint32_t px = (tile % a.tiles_x) * 8u + (r & 7u), py = (tile / a.tiles_x) * 8u + (r >> 3);
 					if (q < pool_end && px < a.width && py < a.height) {
 						k = py * a.width + px;
+						nz = noise_texel(px, py);
 						cur = 0u;
 						path_primary(a, px, py, o, d);
 						march_init<true>(o, d, m);
@@ -1053,8 +1091,8 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) pa
 				} else {
 					rad = f3{a.sky[0], a.sky[1], a.sky[2]};
 				}
-				path_mean(a, a.albedo, k, alb0);
-				path_mean(a, a.normal, k, nrm0);
+				path_mean8<false>(a, a.albedo, k, alb0);
+				path_mean8<true>(a, a.normal, k, nrm0);
 			} else if (!hit) {
 				rad = f3{acc.x * a.sky[0], acc.y * a.sky[1], acc.z * a.sky[2]};
 			} else {
@@ -1067,7 +1105,7 @@ __global__ void __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(5))) pa
 				active = false;
 			} else { // the next bounce of the same path
 				++cur;
-				path_bounce(a, k, cur, h0, (h1.y >> 24) & 7u, o, d);
+				path_bounce(a, nz, cur, h0, (h1.y >> 24) & 7u, o, d);
 				march_init<true>(o, d, m);
 				iter = nfetch = 0u;
 			}
@@ -1095,7 +1133,7 @@ __global__ void __launch_bounds__(kT) path_persist_env(const uint32_t *__restric
 	March m;
 	f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f}, acc{0.0f, 0.0f, 0.0f}, rad{0.0f, 0.0f, 0.0f}, pend{0.0f, 0.0f, 0.0f};
 	float u = 0.0f, v = 0.0f, bpdf = 0.0f;
-	uint32_t k = 0u, cur = 0u, code = 0u, iter = 0u, nfetch = 0u, err = 0u;
+	uint32_t k = 0u, nz = 0u, cur = 0u, code = 0u, iter = 0u, nfetch = 0u, err = 0u;
 	bool active = false, shadow = false;
 	uint64_t pool_next = 0, pool_end = 0, chunk = wave;
 	bool exhausted = false;
@@ -1120,6 +1158,7 @@ __global__ void __launch_bounds__(kT) path_persist_env(const uint32_t *__restric
 					const uint32_t px = (tile % a.tiles_x) * 8u + (r & 7u), py = (tile / a.tiles_x) * 8u + (r >> 3);
 					if (q < pool_end && px < a.width && py < a.height) {
 						k = py * a.width + px;
+						nz = noise_texel(px, py);
 						cur = 0u;
 						shadow = false;
 						rad = f3{0.0f, 0.0f, 0.0f};
@@ -1171,8 +1210,8 @@ __global__ void __launch_bounds__(kT) path_persist_env(const uint32_t *__restric
 				const float4 t = env_lookup(e, d);
 				rad = f3{t.x, t.y, t.z};
 			}
-			path_mean(a, a.albedo, k, alb0);
-			path_mean(a, a.normal, k, nrm0);
+			path_mean8<false>(a, a.albedo, k, alb0);
+			path_mean8<true>(a, a.normal, k, nrm0);
 		} else if (!hit) {
 			const float lpdf = env_pdf(e, d);
 			if (!__builtin_isnan(lpdf) && lpdf > 0.0f) {
@@ -1194,10 +1233,7 @@ __global__ void __launch_bounds__(kT) path_persist_env(const uint32_t *__restric
 		}
 		// bounce cur + 1 from this hit: the light sample's occlusion ray first when it faces the face
 		++cur;
-		const uint32_t na = hash32(k ^ a.s2), nb = hash32(na ^ a.s3);
-		const uint32_t su = (sobol0(a.spp) ^ hash32(a.s0 ^ cur)) + na, sv = (sobol1(a.spp) ^ hash32(a.s1 ^ cur)) + nb;
-		u = (float)(su >> 8) * 5.9604644775390625e-08f;
-		v = (float)(sv >> 8) * 5.9604644775390625e-08f;
+		bounce_sample(a.sobol, cur, nz, u, v);
 		o = f3{__uint_as_float(h0.x), __uint_as_float(h0.y), __uint_as_float(h0.z)};
 		const f3 nrm = face_normal(code);
 		f3 light;
@@ -1223,16 +1259,68 @@ __global__ void __launch_bounds__(kT) path_persist_env(const uint32_t *__restric
 		atomicOr(err_out, err);
 }
 
+// ---- view-type shading of a primary frame (shader/octree_tracer.frag:18-47) ------------------------
+// oColor of the reference's tracer from the hit records: view 0 = pow(albedo, 1/2.2), 1 = normal *
+// 0.5 + 0.5, 2 = pos - 1, 3 = Heat(iter / 128); a miss has pos = 1, normal = 0 and the light's
+// radiance in its direction as colour (the constant colour, or the environment map without its
+// multiplier).  Stored as RGBA8 UNORM (alpha 255), rounded to nearest.
+struct ShadeArgs {
+	CamArgs cam;
+	uint32_t width, height, view;
+	float light[3];
+	const float4 *env_texels; // null: the constant colour
+	uint32_t env_w, env_h;
+	float env_rotation;
+};
+__device__ __forceinline__ uint32_t unorm8_store(float f) {
+	f = f < 0.0f ? 0.0f : (f > 1.0f ? 1.0f : f);
+	return (uint32_t)floorf(f * 255.0f + 0.5f);
+}
+__global__ void __launch_bounds__(kBlock) shade_kernel(const uint4 *__restrict__ hits, ShadeArgs a,
+                                                       uint32_t *__restrict__ out) {
+	const uint64_t k = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+	if (k >= (uint64_t)a.width * a.height)
+		return;
+	const uint4 h0 = hits[2 * k], h1 = hits[2 * k + 1];
+	const bool hit = (h1.y & 0x80000000u) != 0u;
+	float c[3];
+	if (a.view == 3u) { // Heat(x) = sin(clamp(x, 0, 1) * 3 - vec3(1, 2, 3)) * 0.5 + 0.5
+		float x = (float)h0.w / 128.0f;
+		x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+		for (int i = 0; i < 3; ++i) {
+			float sn, cs;
+			sincos_poly(x * 3.0f - (float)(i + 1), sn, cs);
+			c[i] = sn * 0.5f + 0.5f;
+		}
+	} else if (a.view == 2u) { // pos - 1.0
+		c[0] = hit ? __uint_as_float(h0.x) - 1.0f : 0.0f;
+		c[1] = hit ? __uint_as_float(h0.y) - 1.0f : 0.0f;
+		c[2] = hit ? __uint_as_float(h0.z) - 1.0f : 0.0f;
+	} else if (a.view == 1u) { // normal * 0.5 + 0.5
+		const f3 n = hit ? face_normal((h1.y >> 24) & 7u) : f3{0.0f, 0.0f, 0.0f};
+		c[0] = n.x * 0.5f + 0.5f, c[1] = n.y * 0.5f + 0.5f, c[2] = n.z * 0.5f + 0.5f;
+	} else { // pow(color, 1 / 2.2)
+		f3 col;
+		if (hit) {
+			col = leaf_albedo(h1.z);
+		} else if (a.env_texels) {
+			const uint32_t px = (uint32_t)(k % a.width), py = (uint32_t)(k / a.width);
+			const f3 d = gen_ray(a.cam, (float)px / (float)a.width, (float)py / (float)a.height);
+			const EnvArgs e{a.env_texels, nullptr, a.env_w, a.env_h, a.env_rotation, 1.0f};
+			const float4 t = env_lookup(e, d);
+			col = f3{t.x, t.y, t.z};
+		} else {
+			col = f3{a.light[0], a.light[1], a.light[2]};
+		}
+		const float g = 1.0f / 2.2f;
+		c[0] = powf(col.x, g), c[1] = powf(col.y, g), c[2] = powf(col.z, g);
+	}
+	out[k] = unorm8_store(c[0]) | unorm8_store(c[1]) << 8 | unorm8_store(c[2]) << 16 | 0xFF000000u;
+}
+
 CamArgs cam_args(const svo_camera *c) {
 	return CamArgs{c->pos[0],  c->pos[1],  c->pos[2],  c->look[0], c->look[1], c->look[2],
 	               c->side[0], c->side[1], c->side[2], c->up[0],   c->up[1],   c->up[2]};
-}
-
-uint64_t splitmix64(uint64_t &s) {
-	uint64_t z = (s += 0x9E3779B97F4A7C15ull);
-	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-	return z ^ (z >> 31);
 }
 
 // Times fn(shape) for every candidate (one warm-up, best of two timed runs) and returns the fastest;
@@ -1313,18 +1401,18 @@ int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
 	return SVO_OK;
 }
 
-int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
-                svo_hit *hits, float *beam_out) {
+namespace {
+
+// The primary march of a frame (and its beam pre-pass when beam_img is not null, written there).
+// may_tune: the autotuner may time the candidate schedules of a new shape class; without it the
+// schedule is the context's fixed one, or one ray per lane in 16x16 tiles.
+int march_frame_into(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
+                     float *beam_img, svo_hit *hits, bool may_tune) {
 	hipError_t e;
 	const CamArgs ca = cam_args(cam);
-	const uint32_t bw = (w + 7u) / 8u + 1u, bh = (h + 7u) / 8u + 1u; // src/OctreeTracer.cpp:6
-	float *beam_img = nullptr;
+	const uint32_t bw = (w + 7u) / 8u + 1u; // src/OctreeTracer.cpp:6
+	const int beam = beam_img != nullptr;
 	if (beam) {
-		beam_img = beam_out;
-		if (!beam_img) {
-			if ((e = ctx->pool->alloc((void **)&beam_img, sizeof(float) * bw * bh)) != hipSuccess)
-				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "beam image allocation failed");
-		}
 		int rc = march_beam(ctx, oct, cam, w, h, beam_img, nullptr);
 		if (rc)
 			return rc;
@@ -1351,7 +1439,7 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 	int mode = ctx->march_mode;
 	if (mode < 0) {
 		mode = 0;
-		if (ctx->autotune) {
+		if (ctx->autotune && may_tune) {
 			char key[96];
 			std::snprintf(key, sizeof key, "march_frame %ux%u level=%u beam=%d", w, h, oct->level, beam ? 1 : 0);
 			const int rc = tune(ctx, key, 3, kScheduleName, run, mode);
@@ -1364,9 +1452,24 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 	ctx->t_march.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "march_frame");
+	return SVO_OK;
+}
+
+} // namespace
+
+int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
+                svo_hit *hits, float *beam_out) {
+	const uint32_t bw = (w + 7u) / 8u + 1u, bh = (h + 7u) / 8u + 1u; // src/OctreeTracer.cpp:6
+	float *beam_img = nullptr;
+	if (beam) {
+		beam_img = beam_out;
+		if (!beam_img && ctx->pool->alloc((void **)&beam_img, sizeof(float) * bw * bh) != hipSuccess)
+			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "beam image allocation failed");
+	}
+	const int rc = march_frame_into(ctx, oct, cam, w, h, beam_img, hits, true);
 	if (beam && !beam_out)
 		ctx->pool->free(beam_img); // stream-ordered reuse: the pool only hands it to later work on this stream
-	return SVO_OK;
+	return rc;
 }
 
 int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits) {
@@ -1412,42 +1515,73 @@ int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t 
 	return SVO_OK;
 }
 
-int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays) {
+namespace {
+// Writes the Sobol state of sample index n into the context's Sobol buffer (stream-ordered).
+hipError_t sobol_point(svo_ctx *ctx, uint32_t n) {
+	return launch(ctx, "sobol_point", sobol_point_kernel, dim3(1), dim3(64), 0, n, ctx->d_sobol);
+}
+} // namespace
+
+int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint32_t width, uint32_t height, uint32_t spp, uint32_t cur,
+                float *rays) {
+	const uint64_t R = (uint64_t)width * height;
 	if (R == 0)
 		return SVO_OK;
-	uint64_t st = seed;
-	uint64_t z = splitmix64(st);
+	hipError_t e = sobol_point(ctx, spp);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "sobol_point");
 	ctx->t_bounce.start(ctx->stream);
-	hipError_t e = launch(ctx, "bounce", bounce_kernel, dim3(div_up(R, kBlock)), dim3(kBlock), 0,
-	                      reinterpret_cast<const uint4 *>(hits), R, (uint32_t)z, (uint32_t)(z >> 32), rays);
+	e = launch(ctx, "bounce", bounce_kernel, dim3(div_up(R, kBlock)), dim3(kBlock), 0, reinterpret_cast<const uint4 *>(hits),
+	           R, width, (const uint32_t *)ctx->d_sobol, cur, rays);
 	ctx->t_bounce.stop(ctx->stream);
 	if (e != hipSuccess)
 		return hip_check(ctx, e, "bounce");
 	return SVO_OK;
 }
 
-// One progressive sample per pixel (PathTracer::CmdRender, src/PathTracer.cpp:196-215): one path
-// per lane in 8x8 tiles (schedule 0) or persistent waves with per-lane refill (schedule 1),
-// autotuned like the marches; the results are identical.
+namespace {
+// Autotuned launch of one progressive sample: the trials render sample 0 into a scratch buffer, so
+// the caller's running means see one sample only.  run_with(args, schedule) launches.
+template <class Run>
+int path_run(svo_ctx *ctx, const char *what, const char *key, const PathArgs &a, Run run_with) {
+	int mode = ctx->march_mode == 1 ? 1 : 0;
+	if (ctx->march_mode < 0 && ctx->autotune) {
+		if (ctx->tune_choice.find(key) == ctx->tune_choice.end()) {
+			float4 *scratch = nullptr;
+			if (ctx->pool->alloc((void **)&scratch, sizeof(float4) * (uint64_t)a.width * a.height) != hipSuccess)
+				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "path tracer tuning buffer");
+			PathArgs trial = a;
+			trial.spp = 0u;
+			trial.color = scratch;
+			trial.albedo = trial.normal = reinterpret_cast<uint32_t *>(scratch);
+			const int rc = tune(ctx, key, 2, kScheduleName, [&](int m) { return run_with(trial, m); }, mode);
+			ctx->pool->free(scratch); // stream-ordered reuse
+			if (rc)
+				return rc;
+		} else {
+			mode = ctx->tune_choice[key];
+		}
+	}
+	ctx->t_path.start(ctx->stream);
+	const hipError_t e = run_with(a, mode);
+	ctx->t_path.stop(ctx->stream);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, what);
+	return SVO_OK;
+}
+} // namespace
+
+// One progressive sample per pixel (PathTracer::CmdRender, src/PathTracer.cpp:196-215): the Sobol
+// state of sample spp, then one path per lane in 8x8 tiles (schedule 0) or persistent waves with
+// per-lane refill (schedule 1), autotuned like the marches; the results are identical.
 int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t bounce,
-               uint32_t spp, const float sky[3], uint64_t seed, float *color, float *albedo, float *normal) {
-	uint64_t st = seed;
-	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
+               uint32_t spp, const float sky[3], float *color, uint32_t *albedo, uint32_t *normal) {
 	const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
-	const PathArgs a{cam_args(cam),
-	                 w,
-	                 h,
-	                 bounce,
-	                 spp,
-	                 {sky[0], sky[1], sky[2]},
-	                 (uint32_t)z0,
-	                 (uint32_t)(z0 >> 32),
-	                 (uint32_t)z1,
-	                 (uint32_t)(z1 >> 32),
-	                 tiles_x,
-	                 reinterpret_cast<float4 *>(color),
-	                 reinterpret_cast<float4 *>(albedo),
-	                 reinterpret_cast<float4 *>(normal)};
+	const PathArgs a{cam_args(cam), w, h, bounce, spp, {sky[0], sky[1], sky[2]}, ctx->d_sobol, tiles_x,
+	                 reinterpret_cast<float4 *>(color), albedo, normal};
+	hipError_t e = sobol_point(ctx, spp);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "sobol_point");
 	auto run_with = [&](const PathArgs &pa, int mode) -> hipError_t {
 		if (mode == 1) {
 			const uint64_t n = (uint64_t)tiles_x * tiles_y * 64u;
@@ -1460,46 +1594,23 @@ int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint3
 		return launch(ctx, "path_trace", path_trace_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
 		              (const uint32_t *)oct->words, oct->W, pa, ctx->d_err);
 	};
-	int mode = ctx->march_mode == 1 ? 1 : 0;
-	if (ctx->march_mode < 0 && ctx->autotune) {
-		char key[96];
-		std::snprintf(key, sizeof key, "path_trace %ux%u level=%u bounce=%u", w, h, oct->level, bounce);
-		if (ctx->tune_choice.find(key) == ctx->tune_choice.end()) {
-			// the trials render sample 0 into scratch, so the caller's running means see one sample
-			float4 *scratch = nullptr;
-			if (ctx->pool->alloc((void **)&scratch, sizeof(float4) * (uint64_t)w * h) != hipSuccess)
-				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "path tracer tuning buffer");
-			PathArgs trial = a;
-			trial.spp = 0u;
-			trial.color = trial.albedo = trial.normal = scratch;
-			const int rc = tune(ctx, key, 2, kScheduleName, [&](int m) { return run_with(trial, m); }, mode);
-			ctx->pool->free(scratch); // stream-ordered reuse
-			if (rc)
-				return rc;
-		} else {
-			mode = ctx->tune_choice[key];
-		}
-	}
-	ctx->t_path.start(ctx->stream);
-	hipError_t e = run_with(a, mode);
-	ctx->t_path.stop(ctx->stream);
-	if (e != hipSuccess)
-		return hip_check(ctx, e, "path_trace");
-	return SVO_OK;
+	char key[96];
+	std::snprintf(key, sizeof key, "path_trace %ux%u level=%u bounce=%u", w, h, oct->level, bounce);
+	return path_run(ctx, "path_trace", key, a, run_with);
 }
 
 // One progressive sample per pixel with the environment-map light (EnvMapPathTrace).
 int path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
                    uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
-                   uint64_t seed, float *color, float *albedo, float *normal) {
-	uint64_t st = seed;
-	const uint64_t z0 = splitmix64(st), z1 = splitmix64(st);
+                   float *color, uint32_t *albedo, uint32_t *normal) {
 	const uint32_t tiles_x = (w + 7u) / 8u, tiles_y = (h + 7u) / 8u;
-	const PathArgs a{cam_args(cam), w, h, bounce, spp, {0.0f, 0.0f, 0.0f}, (uint32_t)z0, (uint32_t)(z0 >> 32),
-	                 (uint32_t)z1, (uint32_t)(z1 >> 32), tiles_x, reinterpret_cast<float4 *>(color),
-	                 reinterpret_cast<float4 *>(albedo), reinterpret_cast<float4 *>(normal)};
+	const PathArgs a{cam_args(cam), w, h, bounce, spp, {0.0f, 0.0f, 0.0f}, ctx->d_sobol, tiles_x,
+	                 reinterpret_cast<float4 *>(color), albedo, normal};
 	const EnvArgs e{reinterpret_cast<const float4 *>(env->texels), reinterpret_cast<const uint2 *>(env->alias),
 	                env->width, env->height, rotation, multiplier};
+	hipError_t err = sobol_point(ctx, spp);
+	if (err != hipSuccess)
+		return hip_check(ctx, err, "sobol_point");
 	auto run_with = [&](const PathArgs &pa, int mode) -> hipError_t {
 		if (mode == 1) {
 			const uint64_t n = (uint64_t)tiles_x * tiles_y * 64u;
@@ -1512,31 +1623,24 @@ int path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, u
 		return launch(ctx, "path_trace_env", path_trace_env_kernel, dim3(tiles_x * tiles_y), dim3(64), 0,
 		              (const uint32_t *)oct->words, oct->W, pa, e, ctx->d_err);
 	};
-	int mode = ctx->march_mode == 1 ? 1 : 0;
-	if (ctx->march_mode < 0 && ctx->autotune) {
-		char key[96];
-		std::snprintf(key, sizeof key, "path_trace_env %ux%u level=%u bounce=%u", w, h, oct->level, bounce);
-		if (ctx->tune_choice.find(key) == ctx->tune_choice.end()) {
-			// the trials render sample 0 into scratch, so the caller's running means see one sample
-			float4 *scratch = nullptr;
-			if (ctx->pool->alloc((void **)&scratch, sizeof(float4) * (uint64_t)w * h) != hipSuccess)
-				return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "path tracer tuning buffer");
-			PathArgs trial = a;
-			trial.spp = 0u;
-			trial.color = trial.albedo = trial.normal = scratch;
-			const int rc = tune(ctx, key, 2, kScheduleName, [&](int md) { return run_with(trial, md); }, mode);
-			ctx->pool->free(scratch); // stream-ordered reuse
-			if (rc)
-				return rc;
-		} else {
-			mode = ctx->tune_choice[key];
-		}
-	}
-	ctx->t_path.start(ctx->stream);
-	hipError_t err = run_with(a, mode);
-	ctx->t_path.stop(ctx->stream);
-	if (err != hipSuccess)
-		return hip_check(ctx, err, "path_trace_env");
+	char key[96];
+	std::snprintf(key, sizeof key, "path_trace_env %ux%u level=%u bounce=%u", w, h, oct->level, bounce);
+	return path_run(ctx, "path_trace_env", key, a, run_with);
+}
+
+// The tracer's shaded image (OctreeTracer::CmdDrawPipeline's colour output, m_view_type).
+int shade_frame(svo_ctx *ctx, const svo_hit *hits, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t view,
+                const float light[3], const svo_envmap *env, float rotation, uint32_t *out) {
+	const uint64_t n = (uint64_t)w * h;
+	if (n == 0)
+		return SVO_OK;
+	const ShadeArgs a{cam_args(cam), w, h, view, {light[0], light[1], light[2]},
+	                  env ? reinterpret_cast<const float4 *>(env->texels) : nullptr, env ? env->width : 0u,
+	                  env ? env->height : 0u, rotation};
+	const hipError_t e = launch(ctx, "shade", shade_kernel, dim3(div_up(n, kBlock)), dim3(kBlock), 0,
+	                            reinterpret_cast<const uint4 *>(hits), a, out);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "shade");
 	return SVO_OK;
 }
 

@@ -123,6 +123,7 @@ struct svo_ctx {
 	uint64_t *d_scan_parts = nullptr; // 2 * kScanBlocks uint64
 	svo::BuildState *d_state = nullptr;
 	uint32_t *d_err = nullptr;
+	uint32_t *d_sobol = nullptr; // Sobol state of the path tracer's current sample: 64 words + counter
 	svo::BuildState *h_state = nullptr; // pinned mirror for read-back
 	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_build2, t_beam, t_march, t_rays, t_bounce, t_path;
 	svo::Staging stage;
@@ -214,12 +215,15 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
 int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam,
                uint32_t *nfetch);
 int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits);
-int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint64_t R, uint64_t seed, float *rays);
+int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint32_t width, uint32_t height, uint32_t spp, uint32_t cur,
+                float *rays);
 int path_trace(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t bounce,
-               uint32_t spp, const float sky[3], uint64_t seed, float *color, float *albedo, float *normal);
+               uint32_t spp, const float sky[3], float *color, uint32_t *albedo, uint32_t *normal);
 int path_trace_env(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
                    uint32_t bounce, uint32_t spp, const svo_envmap *env, float rotation, float multiplier,
-                   uint64_t seed, float *color, float *albedo, float *normal);
+                   float *color, uint32_t *albedo, uint32_t *normal);
+int shade_frame(svo_ctx *ctx, const svo_hit *hits, const svo_camera *cam, uint32_t w, uint32_t h, uint32_t view,
+                const float light[3], const svo_envmap *env, float rotation, uint32_t *out);
 int copy_device(svo_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 // Device-wide exclusive scan of n (read from the device) uint32 counts; see svo_scan.hpp.

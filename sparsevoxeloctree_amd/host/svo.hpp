@@ -169,6 +169,11 @@ public:
 		return m_W * sizeof(uint32_t);
 	}
 	const svo_octree *GetOctree() const { return m_h; }
+	// The loader-to-main hand-off (src/OctreeBuilder.cpp:215-220, src/LoaderThread.cpp:77-105): work
+	// enqueued on dst afterwards runs after this build; the host does not wait.
+	bool CmdTransferOctreeOwnership(const std::shared_ptr<Context> &dst) const {
+		return dst->Check(svo_octree_transfer(m_h, dst->h));
+	}
 
 private:
 	OctreeBuilder() = default;
@@ -201,6 +206,10 @@ public:
 	uint64_t GetRange() const { return m_range; }
 	const svo_octree *Handle() const { return m_h; }
 	const std::shared_ptr<Context> &GetContextPtr() const { return m_ctx; }
+	// Octree::CmdTransferOwnership (src/Octree.cpp:37-43): orders dst's later work after the build.
+	bool CmdTransferOwnership(const std::shared_ptr<Context> &dst) const {
+		return dst->Check(svo_octree_transfer(m_h, dst->h));
+	}
 	std::vector<uint32_t> Download() const {
 		std::vector<uint32_t> out(m_range / sizeof(uint32_t));
 		m_ctx->Check(svo_octree_download(m_ctx->h, m_h, out.data(), out.size()));
@@ -230,12 +239,17 @@ public:
 	}
 };
 
-// OctreeTracer: CmdBeamRenderPass (beam pre-pass) + CmdDrawPipeline (primary march) into a device
-// hit buffer (src/OctreeTracer.cpp:183-246).  The hit records replace the shaded RGBA8 image.
+class EnvironmentMap;
+
+// OctreeTracer: CmdBeamRenderPass (beam pre-pass) + CmdDrawPipeline (primary march, then the
+// shading of shader/octree_tracer.frag:37-47 by m_view_type into an RGBA8 image), into device
+// buffers (src/OctreeTracer.cpp:183-246).  The hit records stay available beside the image.
 class OctreeTracer {
 public:
 	enum class ViewTypes { kDiffuse = 0, kNormal, kPosition, kIteration } m_view_type = ViewTypes::kDiffuse;
 	bool m_beam_enable{true};
+	float m_light[3] = {5.0f, 5.0f, 5.0f};     // constant light (kDefaultConstantColor, src/Config.hpp:28)
+	std::shared_ptr<EnvironmentMap> m_env_map; // when set, misses show its radiance (Light(), :18-20)
 
 	static std::shared_ptr<OctreeTracer> Create(const std::shared_ptr<Octree> &octree,
 	                                            const std::shared_ptr<Camera> &camera) {
@@ -253,6 +267,7 @@ public:
 		auto &ctx = m_octree->GetContextPtr();
 		const uint64_t bw = (width + 7) / 8 + 1, bh = (height + 7) / 8 + 1;
 		return ctx->Check(svo_malloc(ctx->h, (uint64_t)width * height * sizeof(svo_hit), (void **)&m_hits)) &&
+		       ctx->Check(svo_malloc(ctx->h, (uint64_t)width * height * sizeof(uint32_t), (void **)&m_image)) &&
 		       ctx->Check(svo_malloc(ctx->h, bw * bh * sizeof(float), (void **)&m_beam));
 	}
 	bool CmdBeamRenderPass() const {
@@ -262,18 +277,21 @@ public:
 		svo_camera c = m_camera->fetch_uniform_data();
 		return ctx->Check(svo_march_beam(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_beam, nullptr));
 	}
-	// Runs the beam pre-pass itself when m_beam_enable is set (so the pair can be called in order).
-	bool CmdDrawPipeline() const {
-		auto &ctx = m_octree->GetContextPtr();
-		svo_camera c = m_camera->fetch_uniform_data();
-		return ctx->Check(svo_march(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_beam_enable ? 1 : 0, m_hits,
-		                            m_beam_enable ? m_beam : nullptr));
-	}
+	// Runs the beam pre-pass itself when m_beam_enable is set (so the pair can be called in order),
+	// then shades the hits by m_view_type.
+	bool CmdDrawPipeline() const;
 	svo_hit *GetHits() const { return m_hits; }
+	uint32_t *GetImage() const { return m_image; } // width * height RGBA8 UNORM texels
 	std::vector<svo_hit> DownloadHits() const {
 		std::vector<svo_hit> out((size_t)m_width * m_height);
 		auto &ctx = m_octree->GetContextPtr();
 		ctx->Check(svo_download(ctx->h, out.data(), m_hits, out.size() * sizeof(svo_hit)));
+		return out;
+	}
+	std::vector<uint32_t> DownloadImage() const {
+		std::vector<uint32_t> out((size_t)m_width * m_height);
+		auto &ctx = m_octree->GetContextPtr();
+		ctx->Check(svo_download(ctx->h, out.data(), m_image, out.size() * sizeof(uint32_t)));
 		return out;
 	}
 	uint32_t GetWidth() const { return m_width; }
@@ -285,17 +303,18 @@ private:
 		if (!m_octree)
 			return;
 		auto &ctx = m_octree->GetContextPtr();
-		if (m_hits)
-			svo_free(ctx->h, m_hits);
-		if (m_beam)
-			svo_free(ctx->h, m_beam);
+		for (void *b : {(void *)m_hits, (void *)m_image, (void *)m_beam})
+			if (b)
+				svo_free(ctx->h, b);
 		m_hits = nullptr;
+		m_image = nullptr;
 		m_beam = nullptr;
 	}
 	std::shared_ptr<Octree> m_octree;
 	std::shared_ptr<Camera> m_camera;
 	uint32_t m_width = 0, m_height = 0;
 	svo_hit *m_hits = nullptr;
+	uint32_t *m_image = nullptr;
 	float *m_beam = nullptr;
 };
 
@@ -325,14 +344,26 @@ private:
 	svo_envmap *m_h = nullptr;
 };
 
-// PathTracer: progressive samples into device running means of colour, first-hit albedo and
-// normal, float4 per pixel (PathTracer::Reset / CmdRender, src/PathTracer.cpp:176-215).  The
-// light is m_env_map when set (EnvMapPathTrace), else the constant m_sun_radiance.
+inline bool OctreeTracer::CmdDrawPipeline() const {
+	auto &ctx = m_octree->GetContextPtr();
+	svo_camera c = m_camera->fetch_uniform_data();
+	if (!ctx->Check(svo_march(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_beam_enable ? 1 : 0, m_hits,
+	                          m_beam_enable ? m_beam : nullptr)))
+		return false;
+	return ctx->Check(svo_shade(ctx->h, m_hits, &c, m_width, m_height, (int)m_view_type, m_light,
+	                            m_env_map ? m_env_map->Handle() : nullptr, m_env_map ? m_env_map->m_rotation : 0.0f,
+	                            m_image));
+}
+
+// PathTracer: progressive samples into device running means (PathTracer::Reset / CmdRender,
+// src/PathTracer.cpp:176-215) in the reference's image formats: colour RGBA32F (float4 per pixel),
+// first-hit albedo RGBA8 UNORM and first-hit normal RGBA8 SNORM (one uint32 per pixel).  Sample n
+// uses the reference's sampler (Sobol state after n advances, blue noise).  The light is m_env_map
+// when set (EnvMapPathTrace), else the constant m_sun_radiance.
 class PathTracer {
 public:
 	uint32_t m_bounce = 4;                      // kDefaultBounce, src/Config.hpp:25
 	float m_sun_radiance[3] = {5.0f, 5.0f, 5.0f}; // kDefaultConstantColor, src/Config.hpp:28
-	uint64_t m_seed = 0x5EED;
 	std::shared_ptr<EnvironmentMap> m_env_map;
 
 	static std::shared_ptr<PathTracer> Create(const std::shared_ptr<Octree> &octree,
@@ -349,52 +380,54 @@ public:
 		m_height = height;
 		m_spp = 0;
 		auto &ctx = m_octree->GetContextPtr();
-		const uint64_t bytes = (uint64_t)width * height * 4 * sizeof(float);
-		for (float *&b : m_images)
-			if (!ctx->Check(svo_malloc(ctx->h, bytes, (void **)&b)))
-				return false;
-		return true;
+		const uint64_t px = (uint64_t)width * height;
+		return ctx->Check(svo_malloc(ctx->h, px * 4 * sizeof(float), (void **)&m_color)) &&
+		       ctx->Check(svo_malloc(ctx->h, px * sizeof(uint32_t), (void **)&m_albedo)) &&
+		       ctx->Check(svo_malloc(ctx->h, px * sizeof(uint32_t), (void **)&m_normal));
 	}
-	void Reset() { m_spp = 0; } // the next sample overwrites the means
+	void Reset() { m_spp = 0; } // the next sample overwrites the means, as the cleared images
 	bool CmdRender() {
 		auto &ctx = m_octree->GetContextPtr();
 		svo_camera c = m_camera->fetch_uniform_data();
 		const int rc =
 		    m_env_map ? svo_path_trace_env(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_bounce, m_spp,
-		                                   m_env_map->Handle(), m_env_map->m_rotation, m_env_map->m_multiplier, m_seed,
-		                                   m_images[0], m_images[1], m_images[2])
+		                                   m_env_map->Handle(), m_env_map->m_rotation, m_env_map->m_multiplier,
+		                                   m_color, m_albedo, m_normal)
 		              : svo_path_trace(ctx->h, m_octree->Handle(), &c, m_width, m_height, m_bounce, m_spp,
-		                               m_sun_radiance, m_seed, m_images[0], m_images[1], m_images[2]);
+		                               m_sun_radiance, m_color, m_albedo, m_normal);
 		if (!ctx->Check(rc))
 			return false;
-		++m_spp;
+		++m_spp; // Sobol::CmdNext after the dispatch (src/PathTracer.cpp:214)
 		return true;
 	}
 	uint32_t GetSampleCount() const { return m_spp; }
-	// image 0 = colour, 1 = albedo, 2 = normal: width * height float4
-	std::vector<float> Download(int image) const {
-		std::vector<float> out((size_t)m_width * m_height * 4);
-		auto &ctx = m_octree->GetContextPtr();
-		ctx->Check(svo_download(ctx->h, out.data(), m_images[image], out.size() * sizeof(float)));
-		return out;
-	}
+	std::vector<float> DownloadColor() const { return download<float>(m_color, 4); }  // width * height float4
+	std::vector<uint32_t> DownloadAlbedo() const { return download<uint32_t>(m_albedo, 1); } // RGBA8 UNORM
+	std::vector<uint32_t> DownloadNormal() const { return download<uint32_t>(m_normal, 1); } // RGBA8 SNORM
 
 private:
 	PathTracer() = default;
+	template <class T> std::vector<T> download(const void *src, size_t per_pixel) const {
+		std::vector<T> out((size_t)m_width * m_height * per_pixel);
+		auto &ctx = m_octree->GetContextPtr();
+		ctx->Check(svo_download(ctx->h, out.data(), src, out.size() * sizeof(T)));
+		return out;
+	}
 	void release() {
 		if (!m_octree)
 			return;
 		auto &ctx = m_octree->GetContextPtr();
-		for (float *&b : m_images) {
+		for (void *b : {(void *)m_color, (void *)m_albedo, (void *)m_normal})
 			if (b)
 				svo_free(ctx->h, b);
-			b = nullptr;
-		}
+		m_color = nullptr;
+		m_albedo = m_normal = nullptr;
 	}
 	std::shared_ptr<Octree> m_octree;
 	std::shared_ptr<Camera> m_camera;
 	uint32_t m_width = 0, m_height = 0, m_spp = 0;
-	float *m_images[3] = {nullptr, nullptr, nullptr};
+	float *m_color = nullptr;
+	uint32_t *m_albedo = nullptr, *m_normal = nullptr;
 };
 
 } // namespace svo

@@ -1,9 +1,12 @@
 // svo_cli: the reference's load path (LoaderThread::thread_func, src/LoaderThread.cpp:43-116) on the
 // C++ class surface: Scene -> Voxelizer (count pass) -> timed {CmdVoxelize, CmdBuild} ->
-// Octree::Update, then optionally one primary frame through OctreeTracer.
+// Octree::Update, then optionally one primary frame through OctreeTracer.  As in the reference, the
+// scene is loaded and built on a loader context (its own HIP stream, the loader queue) and handed to
+// the main context with OctreeBuilder::CmdTransferOctreeOwnership (src/LoaderThread.cpp:77-105);
+// the tracer and path tracer run on the main context.
 //
-//   svo_cli -scene KIND[,SEED[,PARAM]] -lvl N [-frame W H] [-no-beam] [-save FILE]
-//           [-pt SAMPLES BOUNCE [-env FILE] [-pt-out FILE]]
+//   svo_cli -scene KIND[,SEED[,PARAM]] -lvl N [-frame W H] [-no-beam] [-view V [-image-out FILE]]
+//           [-save FILE] [-pt SAMPLES BOUNCE [-env FILE] [-pt-out FILE]]
 //   svo_cli -tris FILE.bin -lvl N ...      (FILE.bin: uint64 T, T*9 float32 xyz, T uint32 rgb)
 //
 // Prints the reference's timing line, "FINISHED in {total} ms (Voxelize {v} ms, Octree building
@@ -11,7 +14,9 @@
 // octree's used range.  -save writes the octree with svo_octree_save.  -pt renders SAMPLES
 // progressive path-traced samples of the -frame view through PathTracer (constant sky, or the
 // environment map in FILE: uint32 width, height, then width*height float32 RGBA) and -pt-out
-// writes the colour means (width*height float32 RGBA).
+// writes the colour means (width*height float32 RGBA).  -view picks OctreeTracer::m_view_type (0
+// diffuse, 1 normal, 2 position, 3 iterations) and -image-out writes the shaded frame (width*height
+// uint32 RGBA8).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -23,15 +28,16 @@
 
 static int usage() {
 	std::fprintf(stderr, "usage: svo_cli (-scene KIND[,SEED[,PARAM]] | -tris FILE | -obj FILE.obj) -lvl N "
-	                     "[-frame W H] [-no-beam] [-save FILE] [-pt SAMPLES BOUNCE [-env FILE] [-pt-out FILE]]\n");
+	                     "[-frame W H] [-no-beam] [-view V [-image-out FILE]] [-save FILE] "
+	                     "[-pt SAMPLES BOUNCE [-env FILE] [-pt-out FILE]]\n");
 	return 2;
 }
 
 int main(int argc, char **argv) {
 	int kind = 0;
 	uint64_t seed = 1, param = 0;
-	std::string tris_path, obj_path, save_path, env_path, pt_out;
-	uint32_t level = 0, fw = 0, fh = 0, pt_samples = 0, pt_bounce = 4;
+	std::string tris_path, obj_path, save_path, env_path, pt_out, image_out;
+	uint32_t level = 0, fw = 0, fh = 0, pt_samples = 0, pt_bounce = 4, view = 0;
 	bool beam = true;
 	for (int i = 1; i < argc; ++i) {
 		if (!std::strcmp(argv[i], "-scene") && i + 1 < argc) {
@@ -60,6 +66,10 @@ int main(int argc, char **argv) {
 			env_path = argv[++i];
 		} else if (!std::strcmp(argv[i], "-pt-out") && i + 1 < argc) {
 			pt_out = argv[++i];
+		} else if (!std::strcmp(argv[i], "-view") && i + 1 < argc) {
+			view = (uint32_t)std::atoi(argv[++i]);
+		} else if (!std::strcmp(argv[i], "-image-out") && i + 1 < argc) {
+			image_out = argv[++i];
 		} else
 			return usage();
 	}
@@ -101,34 +111,36 @@ int main(int argc, char **argv) {
 		std::fclose(f);
 	}
 
-	auto ctx = svo::Context::Create(0);
-	if (!ctx) {
+	auto ctx = svo::Context::Create(0);    // the main queue
+	auto loader = svo::Context::Create(0); // the loader queue (src/LoaderThread.cpp:43-48)
+	if (!ctx || !loader) {
 		std::fprintf(stderr, "no HIP device\n");
 		return 1;
 	}
-	auto scene = svo::Scene::Create(ctx, xyz.data(), rgb.data(), T);
+	auto scene = svo::Scene::Create(loader, xyz.data(), rgb.data(), T);
 	if (!scene) {
-		std::fprintf(stderr, "scene: %s\n", ctx->error.c_str());
+		std::fprintf(stderr, "scene: %s\n", loader->error.c_str());
 		return 1;
 	}
 	auto voxelizer = svo::Voxelizer::Create(scene, level);
 	if (!voxelizer) {
-		std::fprintf(stderr, "voxelizer: %s\n", ctx->error.c_str());
+		std::fprintf(stderr, "voxelizer: %s\n", loader->error.c_str());
 		return 1;
 	}
 	auto builder = svo::OctreeBuilder::Create(voxelizer);
-	// src/LoaderThread.cpp:57-75: the timed span is the fill draw plus all octree passes
-	if (!voxelizer->CmdVoxelize() || !builder->CmdBuild()) {
-		std::fprintf(stderr, "build: %s\n", ctx->error.c_str());
+	// src/LoaderThread.cpp:57-75: the timed span is the fill draw plus all octree passes; then the
+	// octree is handed to the main queue (:77-82)
+	if (!voxelizer->CmdVoxelize() || !builder->CmdBuild() || !builder->CmdTransferOctreeOwnership(ctx)) {
+		std::fprintf(stderr, "build: %s %s\n", loader->error.c_str(), ctx->error.c_str());
 		return 1;
 	}
 	auto octree = svo::Octree::Create(ctx);
 	octree->Update(builder); // waits for the build (GetOctreeRange)
 	if (octree->GetRange() == 0) {
-		std::fprintf(stderr, "build: %s\n", ctx->error.c_str());
+		std::fprintf(stderr, "build: %s\n", loader->error.c_str());
 		return 1;
 	}
-	svo_timing t = ctx->Timing();
+	svo_timing t = loader->Timing();
 	std::printf("FINISHED in %.3f ms (Voxelize %.3f ms, Octree building %.3f ms)\n", t.vox_fill_ms + t.build_ms,
 	            t.vox_fill_ms, t.build_ms);
 	std::printf("triangles %llu fragments %u octree_words %llu level %u\n", (unsigned long long)T,
@@ -147,6 +159,7 @@ int main(int argc, char **argv) {
 		camera->m_position[2] = 1.04f; // near the -z face, looking along +z
 		auto tracer = svo::OctreeTracer::Create(octree, camera);
 		tracer->m_beam_enable = beam;
+		tracer->m_view_type = (svo::OctreeTracer::ViewTypes)(view & 3u);
 		if (!tracer->Resize(fw, fh) || !tracer->CmdBeamRenderPass() || !tracer->CmdDrawPipeline()) {
 			std::fprintf(stderr, "tracer: %s\n", ctx->error.c_str());
 			return 1;
@@ -160,6 +173,16 @@ int main(int argc, char **argv) {
 		svo_timing t2 = ctx->Timing();
 		std::printf("frame %ux%u hits %llu mean_iter %.2f beam %.3f ms march %.3f ms\n", fw, fh,
 		            (unsigned long long)nhit, (double)iters / hits.size(), beam ? t2.beam_ms : 0.0, t2.march_ms);
+		if (!image_out.empty()) {
+			std::vector<uint32_t> img = tracer->DownloadImage();
+			FILE *f = std::fopen(image_out.c_str(), "wb");
+			if (!f || std::fwrite(img.data(), sizeof(uint32_t), img.size(), f) != img.size()) {
+				std::fprintf(stderr, "cannot write %s\n", image_out.c_str());
+				return 1;
+			}
+			std::fclose(f);
+			std::printf("image view %u written to %s\n", view & 3u, image_out.c_str());
+		}
 		if (pt_samples) {
 			auto pt = svo::PathTracer::Create(octree, camera);
 			if (!pt || !pt->Resize(fw, fh)) {
@@ -197,7 +220,7 @@ int main(int argc, char **argv) {
 				}
 			ctx->Synchronize();
 			const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-			std::vector<float> col = pt->Download(0);
+			std::vector<float> col = pt->DownloadColor();
 			double mean[3] = {0.0, 0.0, 0.0};
 			for (size_t k = 0; k < col.size(); k += 4)
 				for (int c = 0; c < 3; ++c)

@@ -131,7 +131,8 @@ def test_path_trace_and_obj_argument_checks():
     from sparsevoxeloctree_amd._capi import lib
 
     sky = (ctypes.c_float * 3)(1.0, 1.0, 1.0)
-    assert lib.svo_path_trace(None, None, None, 8, 8, 4, 0, sky, 1, None, None, None) == 1
+    assert lib.svo_path_trace(None, None, None, 8, 8, 4, 0, sky, None, None, None) == 1
+    assert lib.svo_bounce_rays(None, None, 8, 8, 0, 1, None) == 1
     xyz = ctypes.POINTER(ctypes.c_float)()
     rgb = ctypes.POINTER(ctypes.c_uint32)()
     T = ctypes.c_uint64()
@@ -145,4 +146,4 @@ def test_path_trace_and_obj_argument_checks():
     assert lib.svo_envmap_create(None, 4, 2, rgba, ctypes.byref(env)) == 1 and not env
     assert lib.svo_envmap_download(None, None, None, None) == 1
     assert lib.svo_envmap_destroy(None) == 1
-    assert lib.svo_path_trace_env(None, None, None, 8, 8, 4, 0, None, 0.0, 1.0, 1, None, None, None) == 1
+    assert lib.svo_path_trace_env(None, None, None, 8, 8, 4, 0, None, 0.0, 1.0, None, None, None) == 1

@@ -65,9 +65,9 @@ def test_cli_path_tracer_matches_oracle(tmp_path, light):
     tex, alias = oracle.envmap_tables(rgba)
     for s in range(n):
         if light == "env":
-            oracle.path_trace_env(words, 7, cam, W, H, bounce, s, tex, alias, 0.0, 1.0, 0x5EED, color=ref)
+            oracle.path_trace_env(words, 7, cam, W, H, bounce, s, tex, alias, 0.0, 1.0, color=ref)
         else:
-            oracle.path_trace(words, 7, cam, W, H, bounce, s, np.full(3, 5.0, np.float32), 0x5EED, color=ref)
+            oracle.path_trace(words, 7, cam, W, H, bounce, s, np.full(3, 5.0, np.float32), color=ref)
     assert np.array_equal(got, ref)
 
 

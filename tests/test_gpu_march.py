@@ -111,23 +111,24 @@ def test_march_rays_match_oracle(ctx, no_device_errors, scene_octree, variant):
     assert ok, msg
 
 
-def test_bounce_rays_and_march_match_oracle(ctx, no_device_errors, scene_octree):
+@pytest.mark.parametrize("spp,cur", [(0, 1), (5, 1), (1021, 3)])
+def test_bounce_rays_and_march_match_oracle(ctx, no_device_errors, scene_octree, spp, cur):
+    # bounce `cur` of path-tracer sample spp (shader/path_tracer.comp:42-44): Sobol state after spp
+    # advances, fract(Sobol_GetVec2(cur) + blue noise of the pixel), then the 5-argument march
     import sparsevoxeloctree_amd as S
     from sparsevoxeloctree_amd._capi import lib
 
     octree, words, level, _ = scene_octree
     cam = cameras()[0]
-    W, H = 128, 96
+    W, H = 300, 96  # wider than the 256-texel noise tile: the noise wraps at pixel & 255
     tr = S.OctreeTracer.Create(octree, _camera_struct(cam))
     tr.Resize(W, H)
     tr.m_beam_enable = True
     tr.CmdDrawPipeline()
     R = W * H
-    d_rays = ctx.alloc(R * 24)
-    seed = 0x5EED0002
-    ctx.check(lib.svo_bounce_rays(ctx.h, tr.hits.ptr, R, seed, d_rays.ptr))
+    d_rays = S.bounce_rays(ctx, tr.hits, W, H, spp, cur)
     rays_gpu = d_rays.download(np.float32, R * 6).reshape(R, 6)
-    rays_orc, mask = oracle.bounce_rays(tr.GetHits(), seed)
+    rays_orc, mask = oracle.bounce_rays(tr.GetHits(), W, H, spp, cur)
     assert np.array_equal(rays_gpu.view(np.uint32), rays_orc.view(np.uint32))
     assert mask.sum() > 100
     d_hits = ctx.alloc(R * 32)
@@ -223,6 +224,16 @@ def test_path_trace_matches_oracle(ctx, no_device_errors, scene_octree, bounce, 
         ctx.check(lib.svo_march_mode(ctx.h, -1))
 
 
+def _pt_buffers(W, H):
+    return [np.zeros((W * H, 4), np.float32), np.zeros(W * H, np.uint32), np.zeros(W * H, np.uint32)]
+
+
+def _pt_compare(pt, ref, s):
+    got = [pt.GetColor().reshape(-1, 4), pt.GetAlbedo().reshape(-1), pt.GetNormal().reshape(-1)]
+    for g, r, name in zip(got, ref, ("colour", "albedo", "normal")):
+        assert np.array_equal(g.view(np.uint32), r.view(np.uint32)), "%s after sample %d" % (name, s)
+
+
 def _path_trace_case(scene_octree, bounce):
     import sparsevoxeloctree_amd as S
 
@@ -234,19 +245,20 @@ def _path_trace_case(scene_octree, bounce):
     pt.Resize(W, H)
     pt.m_bounce = bounce
     pt.m_sun_radiance = sky
-    pt.seed = 0x5EED0C0DE
-    ref = [np.zeros((W * H, 4), np.float32) for _ in range(3)]
+    ref = _pt_buffers(W, H)
     for s in range(3):
         pt.CmdRender()
-        oracle.path_trace(words, level, cam, W, H, bounce, s, np.array(sky, np.float32), pt.seed, *ref)
-        got = [pt.GetColor(), pt.GetAlbedo(), pt.GetNormal()]
-        for g, r, name in zip(got, ref, ("colour", "albedo", "normal")):
-            assert np.array_equal(g.reshape(-1, 4), r), "%s after sample %d" % (name, s)
+        oracle.path_trace(words, level, cam, W, H, bounce, s, np.array(sky, np.float32), *ref)
+        _pt_compare(pt, ref, s)
     assert pt.GetSampleCount() == 3
+    pt.spp = 1021  # a later sample index: most Sobol columns set
+    pt.CmdRender()
+    oracle.path_trace(words, level, cam, W, H, bounce, 1021, np.array(sky, np.float32), *ref)
+    _pt_compare(pt, ref, 1021)
     pt.Reset()
     pt.CmdRender()  # a restart overwrites the means
-    first = oracle.path_trace(words, level, cam, W, H, bounce, 0, np.array(sky, np.float32), pt.seed)
-    assert np.array_equal(pt.GetColor().reshape(-1, 4), first[0])
+    first = oracle.path_trace(words, level, cam, W, H, bounce, 0, np.array(sky, np.float32))
+    _pt_compare(pt, first, 0)
 
 
 def _env_rgba(h, w, seed):
@@ -289,15 +301,16 @@ def _path_trace_env_case(ctx, scene_octree, bounce):
     pt.Resize(W, H)
     pt.m_bounce = bounce
     pt.m_env_map = env
-    pt.seed = 0xE5A11
-    ref = [np.zeros((W * H, 4), np.float32) for _ in range(3)]
+    ref = _pt_buffers(W, H)
     for s in range(3):
         pt.CmdRender()
         oracle.path_trace_env(words, level, cam, W, H, bounce, s, rtex, ralias, env.m_rotation, env.m_multiplier,
-                              pt.seed, *ref)
-        got = [pt.GetColor(), pt.GetAlbedo(), pt.GetNormal()]
-        for g, r, name in zip(got, ref, ("colour", "albedo", "normal")):
-            assert np.array_equal(g.reshape(-1, 4), r), "%s after sample %d" % (name, s)
+                              *ref)
+        _pt_compare(pt, ref, s)
+    pt.spp = 77
+    pt.CmdRender()
+    oracle.path_trace_env(words, level, cam, W, H, bounce, 77, rtex, ralias, env.m_rotation, env.m_multiplier, *ref)
+    _pt_compare(pt, ref, 77)
     assert np.isfinite(ref[0]).all() and ref[0][:, :3].any()
     env.close()
 
@@ -329,14 +342,11 @@ def test_path_trace_env_edge_cases(ctx, no_device_errors, scene_octree, mode, sh
         pt.Resize(W, H)
         pt.m_bounce = bounce
         pt.m_env_map = env
-        pt.seed = 0xED6E + bounce
-        ref = [np.zeros((W * H, 4), np.float32) for _ in range(3)]
+        ref = _pt_buffers(W, H)
         for s in range(2):
             pt.CmdRender()
-            oracle.path_trace_env(words, level, cam, W, H, bounce, s, rtex, ralias, rotation, multiplier, pt.seed, *ref)
-            got = [pt.GetColor(), pt.GetAlbedo(), pt.GetNormal()]
-            for g, r, name in zip(got, ref, ("colour", "albedo", "normal")):
-                assert np.array_equal(g.reshape(-1, 4), r), "%s after sample %d" % (name, s)
+            oracle.path_trace_env(words, level, cam, W, H, bounce, s, rtex, ralias, rotation, multiplier, *ref)
+            _pt_compare(pt, ref, s)
     finally:
         ctx.check(lib.svo_march_mode(ctx.h, -1))
     assert np.isfinite(ref[0]).all()
@@ -364,12 +374,11 @@ def test_path_trace_tiny_frames(ctx, no_device_errors, scene_octree, mode, frame
             pt.m_env_map = env if use_env else None
             pt.CmdRender()
             if use_env:
-                ref = oracle.path_trace_env(words, level, cam, W, H, pt.m_bounce, 0, rtex, ralias, 0.0, 1.0, pt.seed)
+                ref = oracle.path_trace_env(words, level, cam, W, H, pt.m_bounce, 0, rtex, ralias, 0.0, 1.0)
             else:
                 ref = oracle.path_trace(words, level, cam, W, H, pt.m_bounce, 0,
-                                        np.array(pt.m_sun_radiance, np.float32), pt.seed)
-            assert np.array_equal(pt.GetColor().reshape(-1, 4), ref[0])
-            assert np.array_equal(pt.GetNormal().reshape(-1, 4), ref[2])
+                                        np.array(pt.m_sun_radiance, np.float32))
+            _pt_compare(pt, ref, 0)
     finally:
         ctx.check(lib.svo_march_mode(ctx.h, -1))
     env.close()
@@ -380,3 +389,74 @@ def test_envmap_rejects_a_dark_map(ctx):
 
     with pytest.raises(RuntimeError):
         S.EnvironmentMap.Create(ctx, np.zeros((4, 8, 4), np.float32))
+
+
+@pytest.mark.parametrize("light", ["const", "env"])
+@pytest.mark.parametrize("view", [0, 1, 2, 3])
+def test_shade_view_types_match_oracle(ctx, no_device_errors, scene_octree, view, light):
+    # OctreeTracer::m_view_type (shader/octree_tracer.frag:37-47): normal, position and the
+    # iteration heat map bit-exact; diffuse (pow(colour, 1/2.2): device powf against the host
+    # libm's) within one step of the 8-bit channel, and exact for nearly every pixel
+    import sparsevoxeloctree_amd as S
+
+    octree, words, level, _ = scene_octree
+    cam = cameras()[0]
+    W, H = 160, 128
+    tr = S.OctreeTracer.Create(octree, _camera_struct(cam))
+    tr.Resize(W, H)
+    tr.m_view_type = view
+    tr.m_light = (0.8, 0.35, 2.0)
+    env = None
+    if light == "env":
+        rgba = _env_rgba(16, 32, 9)
+        env = S.EnvironmentMap.Create(ctx, rgba)
+        env.m_rotation = 1.1
+        tr.m_env_map = env
+    tr.CmdDrawPipeline()
+    img = tr.GetImage().reshape(-1)
+    hits = tr.GetHits()
+    assert hit_mask(hits).any() and (~hit_mask(hits)).any()
+    tex = oracle.envmap_tables(rgba)[0] if env is not None else None
+    ref = oracle.shade(hits, cam, W, H, view, tr.m_light, tex, 1.1)
+    if view == 0:
+        a = (img[:, None] >> np.arange(0, 32, 8, dtype=np.uint32)) & 0xFF
+        b = (ref[:, None] >> np.arange(0, 32, 8, dtype=np.uint32)) & 0xFF
+        assert np.abs(a.astype(int) - b.astype(int)).max() <= 1
+        assert (img == ref).mean() > 0.99
+    else:
+        assert np.array_equal(img, ref)
+    assert np.all(img >> 24 == 0xFF)
+    if env is not None:
+        env.close()
+
+
+def test_octree_transfer_orders_the_main_stream(ctx, no_device_errors):
+    # OctreeBuilder::CmdTransferOctreeOwnership (src/LoaderThread.cpp:77-105): build on a loader
+    # context, hand the octree to the main context without a host wait, march it there at once;
+    # the hits equal the oracle's
+    import sparsevoxeloctree_amd as S
+
+    loader = S.Context(0)
+    try:
+        xyz, rgb = S.gen_scene(1, 0x5EED0011, 30)
+        level = 9
+        sc = S.Scene.Create(loader, xyz, rgb)
+        vox = S.Voxelizer.Create(sc, level)
+        vox.CmdVoxelize()
+        b = S.OctreeBuilder.Create(vox)
+        b.CmdBuild()
+        b.CmdTransferOctreeOwnership(ctx)
+        octree = S.Octree(ctx, b.h, level, 0)  # the builder's octree, used on the main context
+        cam = cameras()[1]
+        W, H = 192, 160
+        tr = S.OctreeTracer.Create(octree, _camera_struct(cam))
+        tr.Resize(W, H)
+        tr.m_beam_enable = False
+        tr.CmdDrawPipeline()  # enqueued on the main stream right after the hand-off
+        hits = tr.GetHits()
+        words = oracle.build(oracle.voxelize(oracle.normalize(xyz), rgb, level), level)
+        ok, msg = hits_equal(hits, oracle.march_frame(words, level, cam, W, H))
+        assert ok, msg
+        assert np.array_equal(b.GetOctree().words(), words)
+    finally:
+        loader.synchronize()
