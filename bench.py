@@ -57,11 +57,12 @@ CONFIGS = {
         "256x256 primary rays each; batched launches vs one scene per launch",
         kind=4, seed=0x5EED0004, scenes=256, width=256, height=256,
     ),
-    # serving: Poisson arrivals, 10% builds of sphere+sliver scenes, 90% ray queries (4K..2M rays)
+    # serving: Poisson arrivals, 10% builds of c3-type scenes, 90% ray queries (4K..2M rays)
     5: dict(
-        workload="c5: serving loop, Poisson arrivals at 1000 QPS for 5 s; 10% builds of sphere+sliver scenes "
-        "(32K..516K tris, level U{8..11}), 90% ray queries of log-U[4K,2M] rays (half camera, half "
-        "uniform-sphere directions from in-volume origins) against up to 512 resident octrees",
+        workload="c5: serving loop, Poisson arrivals at 1000 QPS for 5 s; 10% builds of c3-type scenes (the c3 "
+        "generator and scale: spheres + slivers, 64K..8M tris, log-uniform sphere count 8..992, level "
+        "U{9..12}), 90% ray queries of log-U[4K,2M] rays (half camera, half uniform-sphere directions from "
+        "in-volume origins) against up to 512 resident octrees",
         seed=0x5EED0005, qps=1000.0, seconds=5.0, initial=64, pool_scenes=24, budget_gib=160, max_octrees=512,
     ),
 }
@@ -259,13 +260,21 @@ def bench_batch(args, cfg):
 
     split = []
     S.autotune(ctx, True)
+    # both forms warmed the same way: allocator pools filled and the autotuner's trials for every
+    # shape class both use done before any timed pass
     for _ in range(max(1, args.warmup)):
         keep = []
         batched(keep)
         del keep
-    seq_build, seq_march, keep_seq = one_by_one()
+        one_by_one()
+    sb, sm = [], []
+    for _ in range(args.steps):  # as many timed passes as the batched form; medians of both
+        b, m, keep_seq = one_by_one()
+        sb.append(b)
+        sm.append(m)
+        del keep_seq
+    seq_build, seq_march = float(np.median(sb)), float(np.median(sm))
     seq_hits = hits.download(S.HIT_DTYPE, R)
-    del keep_seq
     tb, tm = [], []
     keep = []
     split.clear()
@@ -338,12 +347,15 @@ def bench_serve(args, cfg):
 
     rng = np.random.default_rng(cfg["seed"])
     ctx = S.Context(0)
-    # scene pool: sphere + sliver scenes for build requests, uploaded once
+    # scene pool for build requests, uploaded once: the c3 generator at the c3 scale (spheres of
+    # radius U[0.005, 0.025] plus 4032 slivers per sphere), log-uniform sphere counts 8..992
     pool = []
+    c3_scale = CONFIGS[3]["param"] & ~0xFFFFFFFF
     for i in range(cfg["pool_scenes"]):
-        n = int(rng.integers(4, 65))
-        xyz, rgb = S.gen_scene(3, cfg["seed"] + (i << 16), n)
+        n = int(np.exp(rng.uniform(math.log(8), math.log(992))))
+        xyz, rgb = S.gen_scene(3, cfg["seed"] + (i << 16), n | c3_scale)
         pool.append(S.Scene.Create(ctx, xyz, rgb))
+        del xyz, rgb
     # ray pools in HBM: camera fans and uniform-sphere rays, 2M rays each
     RMAX = 2 * 1024 * 1024
     ray_sets = []
@@ -371,7 +383,7 @@ def bench_serve(args, cfg):
     for _ in range(48):
         free_hits.put(ctx.alloc(RMAX * 32))
     srv = S.Server(0, budget_bytes=cfg["budget_gib"] << 30, max_octrees=cfg["max_octrees"])
-    levels = list(range(8, 12))
+    levels = list(range(9, 13))
     # initial resident octrees
     resident = []
     init = [srv.submit_build(pool[i % len(pool)], levels[i % len(levels)]) for i in range(cfg["initial"])]
@@ -387,7 +399,7 @@ def bench_serve(args, cfg):
     kinds = rng.uniform(size=t_arr.size) < 0.1
     Rs = np.exp(rng.uniform(np.log(4096), np.log(RMAX), t_arr.size)).astype(np.int64)
     pending = queue.Queue()
-    done = {"lat_b": [], "lat_r": [], "fail": 0, "rej": 0}
+    done = {"lat_b": [], "lat_r": [], "fail": 0, "rej": 0, "retargeted": 0}
     lock = threading.Lock()
 
     def collector():
@@ -418,11 +430,19 @@ def bench_serve(args, cfg):
             rid = srv.submit_build(pool[int(rng.integers(len(pool)))], int(levels[int(rng.integers(4))]))
             pending.put((rid, True, None))
         else:
-            with lock:
-                target = resident[int(rng.integers(len(resident)))]
             buf = free_hits.get()
             rs = ray_sets[int(rng.integers(len(ray_sets)))]
-            rid = srv.submit_rays(target, rs, int(Rs[k]), buf, 0)
+            while True:
+                with lock:
+                    target = resident[int(rng.integers(len(resident)))]
+                try:
+                    rid = srv.submit_rays(target, rs, int(Rs[k]), buf, 0)
+                    break
+                except S.SvoError:  # evicted since the client learnt of it: forget it, pick another
+                    with lock:
+                        if target in resident:
+                            resident.remove(target)
+                        done["retargeted"] += 1
             pending.put((rid, False, buf))
     pending.put(None)
     col.join()
@@ -454,6 +474,7 @@ def bench_serve(args, cfg):
         "ray_queries_per_s": round(st["ray_queries_done"] / wall, 2),
         "builds_rejected": st["builds_rejected"],
         "ray_queries_failed": st["ray_queries_failed"],
+        "ray_queries_retargeted": done["retargeted"],
         "evictions": st["evictions"],
         "resident_octrees": st["resident_octrees"],
         "resident_gib": round(st["resident_bytes"] / 2**30, 3),
@@ -521,16 +542,20 @@ def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1
     for rid in ids:
         st, _ = srv.wait(rid)
         targets.append(rid if st == 0 else None)
-    rays_served, failed = 0, 0
+    rays_served, failed, refused = 0, 0, 0
     t0 = time.perf_counter()
     reqs = []
     for k in range(n):
         if kinds[k]:
             reqs.append(srv.submit_build(pool[int(scene_i[k])], int(levels[int(level_i[k])])))
         elif targets[int(target_i[k])] is not None:
-            reqs.append(srv.submit_rays(targets[int(target_i[k])], ray_sets[int(set_i[k])], int(Rs[k]),
-                                        bufs[k % len(bufs)], 0))
-            rays_served += int(Rs[k])
+            try:
+                reqs.append(srv.submit_rays(targets[int(target_i[k])], ray_sets[int(set_i[k])], int(Rs[k]),
+                                            bufs[k % len(bufs)], 0))
+                rays_served += int(Rs[k])
+            except S.SvoError:  # the target was evicted by an earlier build of the list
+                targets[int(target_i[k])] = None
+                refused += 1
     for rid in reqs:
         st, _ = srv.wait(rid)
         failed += st != 0
@@ -543,7 +568,7 @@ def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1
         "plain_requests_per_s": round(n / t_plain, 2), "served_requests_per_s": round(n / t_served, 2),
         "plain_mrays_per_s": round(rays_plain / t_plain / 1e6, 3),
         "served_mrays_per_s": round(rays_served / t_served / 1e6, 3),
-        "speedup": round(t_plain / t_served, 3), "served_failed": int(failed),
+        "speedup": round(t_plain / t_served, 3), "served_failed": int(failed), "served_refused": refused,
     }
 
 
@@ -583,40 +608,11 @@ def copy_calibration(mib):
                           "avg_ms": ms / cnt, "gbs": 2 * n / (ms / cnt * 1e-3) / 1e9}))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument(
-        "--no-extra", action="store_true", help="with --config 2: skip the batched (c4) and served (c5) summaries"
-    )
-    ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument(
-        "--copy-calibration", type=int, default=0, metavar="MIB",
-        help="only run svo_copy_device over MIB MiB, 5 times (the known-size copy used to scale PMC byte counters)",
-    )
-    args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(
-            "bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
-            file=sys.stderr,
-        )
-        sys.exit(2)
-
+def bench_primary(args, cfg, cpu=True):
+    """One single-scene configuration (c1, c2 or c3): args.warmup untimed and args.steps timed steps
+    of voxelize + build + march (+ the bounce for c2); returns the result line."""
     import sparsevoxeloctree_amd as S
 
-    cfg = CONFIGS[args.config]
-    if args.config in (4, 5):
-        line = (bench_batch if args.config == 4 else bench_serve)(args, cfg)
-        print(json.dumps(line))
-        sys.stdout.flush()
-        return
     xyz, rgb = S.gen_scene(cfg["kind"], cfg["seed"], cfg["param"])
     T = rgb.shape[0]
     cam = make_camera(S, cfg, xyz)
@@ -700,7 +696,8 @@ def main():
     if dominant in alg:
         achieved = alg[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
         traffic = None
-        tpath = os.path.join(ROOT, "profiles", "traffic_%s_c%d.json" % (dominant, args.config))
+        cid = next(k for k, v in CONFIGS.items() if v is cfg)
+        tpath = os.path.join(ROOT, "profiles", "traffic_%s_c%d.json" % (dominant, cid))
         if os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f).get("bytes_per_launch")
@@ -765,15 +762,66 @@ def main():
     line["peak_pool_bytes"] = mem["peak_in_use"]
     line["autotune"] = S.autotune(ctx, True).strip().splitlines()
     line["reference_lavapipe"] = "unavailable: no Vulkan loader, lavapipe, Xvfb or glslc on the host"
-    if not args.no_cpu:
+    if cpu:
         words = octree.words()
         line["cpu_baseline"] = cpu_baseline(S, cfg, cam, words, args.cpu_budget)
     else:
         line["cpu_baseline"] = None
     del keep, octree, _vox, _builder
     ctx.close()
+    return line
+
+
+def bench_primary_extra(args, cfg):
+    """A single-scene configuration summarised into the c2 line: 3 timed steps after 1 warm-up, no
+    CPU leg."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup = 3, 1
+    return bench_primary(sub, cfg, cpu=False)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", type=int, default=2, choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument(
+        "--no-extra", action="store_true", help="with --config 2: skip the batched (c4) and served (c5) summaries"
+    )
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument(
+        "--copy-calibration", type=int, default=0, metavar="MIB",
+        help="only run svo_copy_device over MIB MiB, 5 times (the known-size copy used to scale PMC byte counters)",
+    )
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(
+            "bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+
+    import sparsevoxeloctree_amd as S
+
+    cfg = CONFIGS[args.config]
+    if args.config in (4, 5):
+        line = (bench_batch if args.config == 4 else bench_serve)(args, cfg)
+        print(json.dumps(line))
+        sys.stdout.flush()
+        return
+    line = bench_primary(args, cfg, cpu=not args.no_cpu)
     if args.config == 2 and not args.no_extra:
-        # the metric's batched and served figures: c4 and c5 at their own configurations
+        # the metric's other headline figures: the 2^12 build and 3840x2160 frame (c3), the small
+        # plain case (c1), batched (c4) and served (c5), each at its own configuration
+        keys = ("build_ms_ref_scope", "build_ms_end_to_end", "build_gbs_ref_scope", "primary_mrays_per_s",
+                "primary_ms", "peak_pool_bytes", "stage_ms", "roofline")
+        line["c3"] = summarise_extra(bench_primary_extra, args, CONFIGS[3], keys + ("config",))
+        line["c1"] = summarise_extra(bench_primary_extra, args, CONFIGS[1], keys + ("config",))
         line["batched"] = summarise_extra(bench_batch, args, CONFIGS[4], (
             "value", "batched_builds_per_s", "sequential_builds_per_s", "batched_march_ms", "sequential_march_ms",
             "speedup_build", "speedup_march", "speedup_total", "hits_identical_to_sequential", "fragments"))

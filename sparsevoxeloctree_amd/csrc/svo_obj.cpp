@@ -17,14 +17,41 @@
 // 0xffffff of its material, packed on the host by glm (src/Scene.cpp:164): round(clamp(c, 0, 1) *
 // 255) in float, halfway cases away from zero.  Diffuse textures are not read: a textured material
 // contributes its Kd.
+//
+// The number parser and the polygon triangulation restate steps of tinyobjloader, which carries
+// this notice:
+//
+//   The MIT License (MIT)
+//
+//   Copyright (c) 2012-2018 Syoyo Fujita and many contributors.
+//
+//   Permission is hereby granted, free of charge, to any person obtaining a copy
+//   of this software and associated documentation files (the "Software"), to deal
+//   in the Software without restriction, including without limitation the rights
+//   to use, copy, modify, merge, publish, distribute, sublicense, and/or sell
+//   copies of the Software, and to permit persons to whom the Software is
+//   furnished to do so, subject to the following conditions:
+//
+//   The above copyright notice and this permission notice shall be included in
+//   all copies or substantial portions of the Software.
+//
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS OR
+//   IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF MERCHANTABILITY,
+//   FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT. IN NO EVENT SHALL THE
+//   AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY CLAIM, DAMAGES OR OTHER
+//   LIABILITY, WHETHER IN AN ACTION OF CONTRACT, TORT OR OTHERWISE, ARISING FROM,
+//   OUT OF OR IN CONNECTION WITH THE SOFTWARE OR THE USE OR OTHER DEALINGS IN
+//   THE SOFTWARE.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <limits>
 #include <map>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -150,12 +177,23 @@ bool read_lines(const std::string &path, std::vector<std::string> &lines) {
 	std::ifstream f(path, std::ios::binary);
 	if (!f)
 		return false;
+	// a line ends at "\n", "\r\n" or a lone "\r", as the reference reader's safeGetline
+	// (dep/tiny_obj_loader.h:731-760)
+	std::string data((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
 	std::string line;
-	while (std::getline(f, line)) {
-		if (!line.empty() && line.back() == '\r')
-			line.pop_back();
-		lines.push_back(line);
+	for (size_t i = 0; i < data.size(); ++i) {
+		const char c = data[i];
+		if (c == '\n' || c == '\r') {
+			if (c == '\r' && i + 1 < data.size() && data[i + 1] == '\n')
+				++i;
+			lines.push_back(line);
+			line.clear();
+		} else {
+			line.push_back(c);
+		}
 	}
+	if (!line.empty())
+		lines.push_back(line);
 	return true;
 }
 
@@ -213,8 +251,13 @@ bool parse_obj(const std::string &path, Parsed &ps, std::string &err) {
 			const int nv = (int)(ps.pos.size() / 3);
 			while (*p) {
 				int v, t, n;
+				const size_t len = std::strcspn(p, "/ \t\r");
+				if (len == 0) { // not an index token (lines hold no '\r': they end there)
+					err = "bad face token on line " + std::to_string(ln + 1);
+					return false;
+				}
 				bool ok = resolve_index(std::atoi(p), nv, &v);
-				p += std::strcspn(p, "/ \t\r");
+				p += len;
 				if (ok && *p == '/') {
 					++p;
 					if (*p == '/') { // i//k
@@ -351,7 +394,8 @@ uint32_t unorm8(float c) {
 
 } // namespace
 
-extern "C" int svo_obj_load(const char *path, float **xyz, uint32_t **rgb, uint64_t *T, char *msg, uint64_t msgcap) {
+namespace {
+int obj_load(const char *path, float **xyz, uint32_t **rgb, uint64_t *T, char *msg, uint64_t msgcap) {
 	auto fail = [&](int code, const std::string &m) {
 		if (msg && msgcap) {
 			std::snprintf(msg, msgcap, "%s", m.c_str());
@@ -415,6 +459,17 @@ extern "C" int svo_obj_load(const char *path, float **xyz, uint32_t **rgb, uint6
 	*rgb = C;
 	*T = n;
 	return SVO_OK;
+}
+} // namespace
+
+extern "C" int svo_obj_load(const char *path, float **xyz, uint32_t **rgb, uint64_t *T, char *msg, uint64_t msgcap) {
+	try {
+		return obj_load(path, xyz, rgb, T, msg, msgcap);
+	} catch (const std::bad_alloc &) { // a file too large for host memory: no exception crosses the C ABI
+		if (msg && msgcap)
+			std::snprintf(msg, msgcap, "out of host memory");
+		return SVO_ERR_OUT_OF_MEMORY;
+	}
 }
 
 extern "C" void svo_obj_free(float *xyz, uint32_t *rgb) {

@@ -1,8 +1,14 @@
 // Serving loop: build and ray-query requests for many scenes arrive at any time and are batched by
-// two worker threads, each with its own context (HIP stream) on the same device, so ray queries
+// two worker threads with their own contexts (HIP streams) on the same device, so ray queries
 // never wait behind a build:
-//   ray worker    drains every pending ray query, pins the target octrees and marches all of them
-//                 with one launch per march variant (per-request ray and hit buffers, no copies);
+//   ray worker    continuous batching over kRaySlots streams: whenever a stream is free and queries
+//                 wait, it takes every waiting query and marches them all with one launch per march
+//                 variant (per-request ray and hit buffers, no copies), without waiting for the
+//                 batches already on the GPU; completions are polled by event, so a query that
+//                 arrives while batches run starts on the next free stream, and batches overlap on
+//                 the device.  A query pins its octree at submission (it cannot be evicted while
+//                 queued or marching); a query for an octree that is not resident is refused at
+//                 submission;
 //   build worker  drains pending builds (up to kMaxBuildBatch), voxelizes them with one count and
 //                 one fill launch, builds them with one launch per pass, then admits each octree
 //                 into the resident set by its exact size: least-recently-used unpinned octrees are
@@ -30,6 +36,7 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr uint32_t kMaxBuildBatch = 64;
+constexpr int kRaySlots = 2; // ray batches in flight at once, one stream each
 
 struct Request {
 	uint64_t id = 0;
@@ -69,8 +76,9 @@ double percentile(std::vector<double> v, double p) {
 
 struct svo_server {
 	int device = 0;
-	svo_ctx *cb = nullptr; // build worker's context
-	svo_ctx *cr = nullptr; // ray worker's context
+	svo_ctx *cb = nullptr;            // build worker's context
+	svo_ctx *cr[kRaySlots] = {};      // ray worker's contexts, one per slot
+	hipEvent_t ev[kRaySlots] = {};    // completion of each slot's batch
 	uint64_t budget = 0;
 	uint32_t max_octrees = 0;
 	std::mutex mu;
@@ -132,62 +140,88 @@ struct svo_server {
 
 void svo_server::ray_loop() {
 	(void)hipSetDevice(device);
+	struct Flight {
+		std::vector<Request *> reqs;
+		bool busy = false;
+	};
+	Flight slot[kRaySlots];
+	int inflight = 0;
+	Clock::time_point busy_since{};
 	std::unique_lock<std::mutex> lk(mu);
 	for (;;) {
-		cv_rays.wait(lk, [&] { return stop || !qr.empty(); });
-		if (qr.empty())
-			break; // stop requested and drained
-		std::vector<Request *> batch(qr.begin(), qr.end());
-		qr.clear();
-		const Clock::time_point t0 = Clock::now();
-		// resolve and pin the target octrees
-		std::vector<Request *> run;
-		std::vector<const svo_octree *> octs;
-		for (Request *r : batch) {
-			auto it = res.find(r->octree);
-			if (it == res.end()) {
-				finish(r, SVO_ERR_INVALID_ARGUMENT, t0); // not resident (never built, rejected or evicted)
+		// completed batches: finish their queries and unpin their octrees
+		for (int k = 0; k < kRaySlots; ++k) {
+			if (!slot[k].busy)
 				continue;
+			const hipError_t q = hipEventQuery(ev[k]);
+			if (q == hipErrorNotReady)
+				continue;
+			const Clock::time_point t1 = Clock::now();
+			for (Request *r : slot[k].reqs) {
+				auto it = res.find(r->octree);
+				if (it != res.end())
+					it->second.pins--;
+				finish(r, q == hipSuccess ? r->status : SVO_ERR_HIP, t1);
 			}
-			it->second.pins++;
-			it->second.last_use = ++use_clock;
-			run.push_back(r);
-			octs.push_back(it->second.oct);
+			slot[k].reqs.clear();
+			slot[k].busy = false;
+			if (--inflight == 0)
+				busy_rays_ms += ms_between(busy_since, t1);
+			cv_done.notify_all();
 		}
-		if (!run.empty())
+		int free_slot = -1;
+		for (int k = 0; k < kRaySlots && free_slot < 0; ++k)
+			if (!slot[k].busy)
+				free_slot = k;
+		if (free_slot >= 0 && !qr.empty()) {
+			// every waiting query goes into this batch; their octrees were pinned at submission
+			std::vector<Request *> run(qr.begin(), qr.end());
+			qr.clear();
+			std::vector<const svo_octree *> octs;
+			for (Request *r : run)
+				octs.push_back(res.at(r->octree).oct);
 			++batches;
-		lk.unlock();
-		int rc = SVO_OK;
-		for (int variant = 0; variant < 3 && !rc; ++variant) {
-			std::vector<const svo_octree *> o;
-			std::vector<const float *> rays;
-			std::vector<svo_hit *> hits;
-			std::vector<uint64_t> count;
-			for (size_t i = 0; i < run.size(); ++i)
-				if (run[i]->variant == variant) {
-					o.push_back(octs[i]);
-					rays.push_back(run[i]->rays);
-					hits.push_back(run[i]->hits);
-					count.push_back(run[i]->R);
+			if (inflight++ == 0)
+				busy_since = Clock::now();
+			slot[free_slot].busy = true;
+			lk.unlock();
+			svo_ctx *c = cr[free_slot];
+			int rc = SVO_OK;
+			for (int variant = 0; variant < 3 && !rc; ++variant) {
+				std::vector<const svo_octree *> o;
+				std::vector<const float *> rays;
+				std::vector<svo_hit *> hits;
+				std::vector<uint64_t> count;
+				for (size_t i = 0; i < run.size(); ++i)
+					if (run[i]->variant == variant) {
+						o.push_back(octs[i]);
+						rays.push_back(run[i]->rays);
+						hits.push_back(run[i]->hits);
+						count.push_back(run[i]->R);
+					}
+				for (size_t b = 0; b < o.size() && !rc; b += svo::kMaxBatch) {
+					const uint32_t n = (uint32_t)std::min<size_t>(svo::kMaxBatch, o.size() - b);
+					rc = svo::march_rays_segments(c, o.data() + b, rays.data() + b, hits.data() + b, count.data() + b,
+					                              n, variant);
 				}
-			for (size_t b = 0; b < o.size() && !rc; b += svo::kMaxBatch) {
-				const uint32_t n = (uint32_t)std::min<size_t>(svo::kMaxBatch, o.size() - b);
-				rc = svo::march_rays_segments(cr, o.data() + b, rays.data() + b, hits.data() + b, count.data() + b, n,
-				                              variant);
 			}
+			if (hipEventRecord(ev[free_slot], c->stream) != hipSuccess) {
+				rc = rc ? rc : SVO_ERR_HIP;
+				(void)hipStreamSynchronize(c->stream); // no event: complete this batch here
+				(void)hipEventRecord(ev[free_slot], nullptr);
+			}
+			for (Request *r : run)
+				r->status = rc;
+			lk.lock();
+			slot[free_slot].reqs = std::move(run);
+			continue;
 		}
-		if (!run.empty() && !rc && hipStreamSynchronize(cr->stream) != hipSuccess)
-			rc = SVO_ERR_HIP;
-		const Clock::time_point t1 = Clock::now();
-		lk.lock();
-		busy_rays_ms += ms_between(t0, t1);
-		for (size_t i = 0; i < run.size(); ++i) {
-			auto it = res.find(run[i]->octree);
-			if (it != res.end())
-				it->second.pins--;
-			finish(run[i], rc, t1);
-		}
-		cv_done.notify_all();
+		if (stop && qr.empty() && inflight == 0)
+			break; // stop requested and drained
+		if (inflight == 0)
+			cv_rays.wait(lk, [&] { return stop || !qr.empty(); });
+		else // poll the batches on the GPU; a new query wakes the loop at once
+			cv_rays.wait_for(lk, std::chrono::microseconds(40));
 	}
 }
 
@@ -298,9 +332,18 @@ int svo_server_create(int device, uint64_t budget_bytes, uint32_t max_octrees, s
 	s->budget = budget_bytes;
 	s->max_octrees = max_octrees;
 	int rc = svo_ctx_create(device, &s->cb);
-	if (!rc)
-		rc = svo_ctx_create(device, &s->cr);
+	for (int k = 0; k < kRaySlots && !rc; ++k) {
+		rc = svo_ctx_create(device, &s->cr[k]);
+		if (!rc && hipEventCreateWithFlags(&s->ev[k], hipEventDisableTiming) != hipSuccess)
+			rc = SVO_ERR_HIP;
+	}
 	if (rc) {
+		for (int k = 0; k < kRaySlots; ++k) {
+			if (s->ev[k])
+				(void)hipEventDestroy(s->ev[k]);
+			if (s->cr[k])
+				svo_ctx_destroy(s->cr[k]);
+		}
 		if (s->cb)
 			svo_ctx_destroy(s->cb);
 		delete s;
@@ -327,7 +370,10 @@ int svo_server_destroy(svo_server *s) {
 		svo_octree_destroy(kv.second.oct);
 	for (auto &kv : s->reqs)
 		delete kv.second;
-	svo_ctx_destroy(s->cr);
+	for (int k = 0; k < kRaySlots; ++k) {
+		(void)hipEventDestroy(s->ev[k]);
+		svo_ctx_destroy(s->cr[k]);
+	}
 	svo_ctx_destroy(s->cb);
 	delete s;
 	return SVO_OK;
@@ -377,10 +423,13 @@ int svo_server_submit_rays(svo_server *s, uint64_t octree_id, const float *rays_
 	r->t_submit = Clock::now();
 	{
 		std::lock_guard<std::mutex> lk(s->mu);
-		if (s->stop) {
+		auto it = s->res.find(octree_id);
+		if (s->stop || it == s->res.end()) { // not resident (never built, rejected or evicted): no request
 			delete r;
 			return SVO_ERR_INVALID_ARGUMENT;
 		}
+		it->second.pins++; // held until the query completes: the octree cannot be evicted meanwhile
+		it->second.last_use = ++s->use_clock;
 		r->id = s->next_id++;
 		s->reqs[r->id] = r;
 		s->qr.push_back(r);

@@ -99,14 +99,14 @@ def test_server_builds_and_queries_match_single_path(ctx, no_device_errors):
             ref = S.march_rays(ctx, octs[i], rays, variant)
             ok, msg = hits_equal(got, ref)
             assert ok, "query %d: %s" % (k, msg)
-        # unknown octree
+        # an unknown octree is refused at submission
         d = ctx.alloc(32 * 4)
-        st, _ = srv.wait(srv.submit_rays(987654, d, 1, d, 0))
-        assert st == SVO_ERR_INVALID_ARGUMENT
+        with pytest.raises(S.SvoError):
+            srv.submit_rays(987654, d, 1, d, 0)
         stats = srv.stats()
         assert stats["builds_done"] == len(ids)
         assert stats["ray_queries_done"] == len(jobs)
-        assert stats["ray_queries_failed"] == 1
+        assert stats["ray_queries_failed"] == 0
         assert stats["rays_p99_ms"] >= stats["rays_p50_ms"] > 0.0
         assert stats["resident_octrees"] == len(ids)
     finally:
@@ -128,7 +128,9 @@ def test_server_admission_and_eviction(ctx, no_device_errors):
         c = srv.submit_build(scenes[2], 6)
         assert srv.wait(c)[0] == 0
         assert srv.resident(b) is None and srv.resident(a) is not None
-        assert srv.wait(srv.submit_rays(b, d, 64, h, 0))[0] == SVO_ERR_INVALID_ARGUMENT
+        with pytest.raises(S.SvoError) as e:  # evicted: refused at submission
+            srv.submit_rays(b, d, 64, h, 0)
+        assert e.value.code == SVO_ERR_INVALID_ARGUMENT
         assert srv.stats()["evictions"] == 1
     finally:
         srv.close()

@@ -130,3 +130,32 @@ def test_errors(tmp_path, body, why):
     assert why in str(e.value)
     with pytest.raises(S.SvoError):
         S.load_obj(str(tmp_path / "missing.obj"))
+
+
+def test_lone_cr_ends_a_line(tmp_path):
+    # a file with CR-only line endings reads like its LF form (tinyobj's safeGetline ends a line at
+    # "\n", "\r\n" or a lone "\r", dep/tiny_obj_loader.h:731-760), and a face line followed by a lone
+    # CR and more text ends at the CR instead of being read without end
+    src = os.path.join(GOLD, "shapes.obj")
+    with open(src, "rb") as f:
+        lf = f.read().replace(b"\r\n", b"\n")
+    (tmp_path / "lf.obj").write_bytes(lf)
+    (tmp_path / "cr.obj").write_bytes(lf.replace(b"\n", b"\r"))
+    for mtl in os.listdir(GOLD):
+        if mtl.endswith(".mtl"):
+            with open(os.path.join(GOLD, mtl), "rb") as f:
+                (tmp_path / mtl).write_bytes(f.read())
+    a = S.load_obj(str(tmp_path / "lf.obj"))
+    b = S.load_obj(str(tmp_path / "cr.obj"))
+    assert a[1].shape[0] > 0
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32)) and np.array_equal(a[1], b[1])
+    (tmp_path / "m.mtl").write_bytes(b"newmtl a\nKd 0.5 0.25 1\n")
+    (tmp_path / "tail.obj").write_bytes(b"mtllib m.mtl\nusemtl a\nv 0 0 0\nv 1 0 0\nv 0 1 0\nv 0 0 1\n"
+                                        b"f 1 2 3\r4\nf 1 2 4\n")
+    xyz, rgb = S.load_obj(str(tmp_path / "tail.obj"))
+    assert rgb.shape[0] == 2  # "4" alone on its line is not a statement
+    if os.path.exists(OBJREF):
+        for name in ("cr.obj", "tail.obj"):
+            r = subprocess.run([OBJREF, str(tmp_path / name)], capture_output=True, text=True, timeout=60)
+            assert r.returncode == 0
+            assert as_lines(*S.load_obj(str(tmp_path / name))) == r.stdout.splitlines()
