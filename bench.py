@@ -670,12 +670,10 @@ def bench_primary(args, cfg, cpu=True):
         "vox_count": 40 * T + 4 * T,
         # occupancy marks: every fragment read once
         "bm_mark": 8 * F,
-        # cell counts: fragment read, its 4-byte place in its cell written
-        "cell_count": 12 * F,
-        # fragment and place read, fragment written into its cell's stretch
-        "cell_scatter": 20 * F,
-        # grouping by leaf block: every fragment read and written once
-        "cell_group": 16 * F,
+        # leaf-block index of every fragment: fragment read, 4-byte bin written
+        "build_tag_leaf": 12 * F,
+        # bin index and fragment read, fragment written into its bin's bucket
+        "build_scatter": 20 * F,
         # the canonical colour fold: every bucketed fragment read once (leaf blocks written: not counted)
         "build_fold": 8 * F,
     }

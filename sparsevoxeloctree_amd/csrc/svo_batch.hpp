@@ -112,9 +112,6 @@ struct BuildSeg {
 	uint32_t overflow;   // W > 2^30
 	uint64_t off[14];    // word offset of B[d] relative to bm, d = 1..L; off[L] = the scene's bitmap words
 	uint32_t fb[16];     // first block of every depth: fb[0] = 0 (root block), fb[1] = 1
-	uint64_t cbeg;       // global index of the scene's first dense cell (node phase)
-	uint32_t k3;         // 3 (L - 1 - D): Morton bits of a depth-(L-1) node below its depth-D cell
-	uint32_t pad2;
 };
 
 // Voxelize batch state, from the count pass to the fill.
@@ -133,7 +130,7 @@ struct VoxBatch {
 struct BuildBatch {
 	uint32_t S = 0;
 	uint32_t maxL = 0;
-	uint64_t Ftot = 0, TW = 0, nbins = 0, ncells = 0, big_cap = 0;
+	uint64_t Ftot = 0, TW = 0, nbins = 0;
 	std::vector<BuildSeg> h_segs;
 	// begin tables, S + 1 entries each: fbeg (fragments), bbeg (leaf blocks), dw[d] at 2 + d (words of
 	// B[d] of the scenes with L > d), dr[d] at 3 + kMaxLevel + d (same, scenes with L > d + 1); then
@@ -143,12 +140,9 @@ struct BuildBatch {
 	uint64_t *index = nullptr;
 	uint32_t *bm = nullptr, *rk = nullptr; // bitmaps (TW words), ranks of 8-word chunks (TW / 8 + 1)
 	uint32_t *cnt = nullptr, *aux = nullptr; // per leaf block (nbins)
-	uint32_t *ccnt = nullptr;    // 2 * ncells: dense cell counts, then their offsets
-	uint32_t *place = nullptr;   // Ftot: a fragment's slot among its cell's fragments
-	uint32_t *bucket0 = nullptr; // 2 * Ftot: fragments grouped by cell
-	uint32_t *bucket = nullptr;  // 2 * Ftot: fragments grouped by leaf block
-	uint32_t *list = nullptr;    // 3 + 3 * cap: large leaf blocks
-	uint4 *bigc = nullptr;       // 1 + big_cap: count (in .x), then the cells for cell_group_big
+	uint32_t *node = nullptr;   // 2 * Ftot: {leaf block, place in its bucket}
+	uint32_t *bucket = nullptr; // 2 * Ftot
+	uint32_t *list = nullptr;   // 2 + 2 * cap: large leaf blocks
 	const uint64_t *fbeg() const { return index; }
 	const uint64_t *bbeg() const { return index + (S + 1); }
 	const uint64_t *dw(uint32_t d) const { return index + (2 + d) * (uint64_t)(S + 1); }
@@ -159,10 +153,6 @@ struct BuildBatch {
 	uint64_t pw_at() const { return (3 + 2 * (uint64_t)kMaxLevel + 1) * (S + 1); }
 	uint32_t pw_segs() const { return (kMaxLevel - 2u) * S; }
 	const uint64_t *pw() const { return index + pw_at(); }
-	// then cb: the first dense cell of every scene (S + 1 entries)
-	uint64_t cb_at() const { return pw_at() + pw_segs() + 1; }
-	const uint64_t *cb() const { return index + cb_at(); }
-	uint64_t index_size() const { return cb_at() + S + 1; }
 };
 
 void vox_batch_release(svo_ctx *ctx, VoxBatch *vb);

@@ -19,13 +19,11 @@
 //   octree_modify_arg.comp    -> bm_levels: per-depth counts -> first block of every depth, the
 //                                leaf blocks and W = (counter + 1) * 8
 //   octree_init_node.comp     -> bm_write_blocks writes every block whole (pointers and zeros)
-//   octree_tag_node.comp      -> cell_count / cell_scatter / cell_group{,_big}: the fragments are
-//   (level L) + leaf CAS fold    grouped by cell (a node of a coarser depth D), then each cell by
-//                                leaf block ("bin": the rank of the fragment's depth-(L-1) node), so
-//                                every bin's fragments are consecutive; build_fold{,_lists} sort each
-//                                bin by the 64-bit fragment and fold the 8 leaves of the block in that
-//                                order (the reference's order is whatever its atomics produce),
-//                                writing the leaf block whole.
+//   octree_tag_node.comp      -> build_tag_leaf: leaf block ("bin") of each fragment = rank of its
+//   (level L) + leaf CAS fold    depth-(L-1) node; count per bin, scan, scatter into per-bin buckets;
+//                                build_fold{,_mid,_big} sort each bucket by the 64-bit fragment and
+//                                fold the 8 leaves of the block in that order (the reference's order
+//                                is whatever its atomics produce), writing the leaf block whole.
 // The node words are identical to the reference's pass structure run with a serial allocator.
 //
 // Batch layout: the scenes' fragments, bitmap words and leaf blocks each form one index space
@@ -120,13 +118,11 @@ __global__ void __launch_bounds__(kThreads) bm_zero(uint32_t *B, uint64_t n) {
 constexpr uint32_t kU = 4;
 
 // Where a fragment's scene keeps what the per-fragment passes read: the fragment itself, the first
-// word of the scene's B[L-1] (and of its ranks), the level, and the scene's cells: its first dense
-// cell and k3 = 3 (L - 1 - D), the bits of a depth-(L-1) node below its depth-D cell.
+// word of the scene's B[L-1] (and of its ranks), the level and the scene's first bin.
 struct FragSrc {
 	const uint2 *frag;
 	uint64_t w0;
-	uint32_t L, k3;
-	uint64_t cell0;
+	uint32_t L, bin_begin;
 };
 
 // Descriptor lookups for the kU fragments base + u * kThreads + threadIdx.x of one loop trip
@@ -137,12 +133,13 @@ __device__ __forceinline__ void frag_sources(const BuildSeg *__restrict__ segs, 
 #pragma unroll
 	for (uint32_t u = 0; u < kU; ++u) {
 		const uint64_t b = base + (uint64_t)u * kThreads, f = b + threadIdx.x;
-		src[u] = FragSrc{nullptr, 0u, 1u, 0u, 0u};
+		src[u] = FragSrc{nullptr, 0u, 1u, 0u};
 		if (f < Ftot)
 			with_seg<kThreads>(fbeg, S, hint, b, f, [&](uint32_t s) {
 				const BuildSeg &g = segs[s];
 				const uint32_t L = g.L;
-				src[u] = FragSrc{g.frags + (f - fbeg[s]), g.bm + (L > 1u ? g.off[L - 1u] : 0u), L, g.k3, g.cbeg};
+				src[u] = FragSrc{g.frags + (f - fbeg[s]), g.bm + (L > 1u ? g.off[L - 1u] : 0u), L,
+				                 (uint32_t)g.bin_begin};
 			});
 	}
 }
@@ -398,22 +395,19 @@ __device__ __forceinline__ uint32_t nth_bit(uint32_t v, uint32_t n) {
 // round writes the wave's k-th occupied parent, so every lane has work and the 64 blocks of a round
 // are contiguous.  A wave across a segment boundary walks each word's set bits in its own lane.
 // Block 0 also writes the root block of every scene with L > 1 (its 8 words point to the depth-1
-// blocks of the occupied children, the bits of B[1]).  The same launch clears the nc dense cell
-// counters of cell_count, the three counts of the fold's work lists and the count of the big-cell
-// list.
+// blocks of the occupied children, the bits of B[1]).  The same launch clears the nb leaf-block
+// counters of build_tag_leaf and the three counts of the fold's work lists.
 __global__ void __launch_bounds__(kThreads)
     bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ pw, uint32_t M, uint32_t S,
-                    uint64_t n, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint32_t *ccnt,
-                    uint64_t nc, uint32_t *list, uint32_t *big_count) {
+                    uint64_t n, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint32_t *cnt,
+                    uint64_t nb, uint32_t *list) {
 	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nc; i += stride)
-		ccnt[i] = 0u;
+	for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nb; i += stride)
+		cnt[i] = 0u;
 	if (blockIdx.x == 0u) {
 		if (threadIdx.x < 3u)
 			list[threadIdx.x] = 0u;
-		if (threadIdx.x == 3u)
-			*big_count = 0u;
 		for (uint32_t s = threadIdx.x; s < S; s += kThreads) {
 			const BuildSeg &g = segs[s];
 			if (g.L > 1u) // L == 1: the root block holds the leaves (build_fold)
@@ -511,37 +505,24 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
-// ---- leaf blocks: fragments grouped by cell, then by leaf block ---------------------------------
-// The fold needs every leaf block's fragments together (build_fold reads bucket[aux[j] ..
-// aux[j] + cnt[j]) for leaf block j).  Each scene is cut into cells, the nodes of one depth D
-// (chosen per scene so that a cell holds about kCellTarget fragments, with L-1-4 <= D <= L-1): a
-// cell holds at most 8^4 leaf blocks, consecutive in the canonical order, and the cells of a scene
-// follow Morton order like the leaf blocks.  So grouping the fragments by cell (a counting sort
-// over a dense array of 8^D counters per scene, indexed by the cell's Morton prefix: no rank
-// lookups) and then each cell by leaf block (in LDS, one wave or one block per cell) puts every
-// leaf block's fragments together, in leaf-block order.
-//   cell_count     per fragment: its cell's counter; place = its slot among the cell's fragments
-//   cell_offsets   exclusive scan of the counters (global fragment offsets of the cells)
-//   cell_scatter   fragment -> bucket0[cell offset + place]
-//   cell_group     per cell of at most kCellWave fragments, one wave: sort by leaf block in LDS,
-//                  write bucket, cnt and aux of the cell's leaf blocks; larger cells are listed
-//   cell_group_big per listed cell, one block: counting sort by leaf block in LDS
+// Leaf block ("bin") of every fragment: the rank of its depth-(L-1) node, offset by the scene's
+// first bin; node[f] = {bin, place}, where place is unique inside the bin, so the scatter needs no
+// atomics.  Places come from per-bin counters in global memory whose atomics the L2 cannot perform
+// (they are device-coherent, so they go to memory), and each such request stalls the L2 for the
+// reads behind it.  So a loop trip first counts its kU * kThreads fragments in LDS, per 64-byte line
+// of 16 counters (a line table with linear probing; the count's old value is the fragment's place
+// among the trip's fragments of its bin), and then adds each line's counts with 16 neighbouring
+// lanes, which the L1 merges into one request per line: a trip of 1024 fragments touches about
+// 40 lines (a room scene at 2^10) to 150 (random slivers at 2^12) instead of 200+; kLines line slots
+// (128 up to level 11, 256 at level 12).  A fragment that finds the table full takes its place with
+// its own atomic.
 constexpr uint32_t kLineBins = 16; // 4-byte counters per 64-byte line
-constexpr uint32_t kCellWave = 256; // largest cell grouped by one wave
-constexpr uint32_t kCellSpan = 4096; // leaf blocks per cell at most (8^4)
 
-// Cell counters of the trip's fragments, merged in LDS before they reach memory: counter atomics
-// on device memory are device-coherent, so the L2 cannot perform them, and each such request
-// stalls the L2 for the reads behind it.  So a loop trip first counts its kU * kThreads fragments in
-// LDS per 64-byte line of 16 counters (a line table with linear probing; the count's old value is
-// the fragment's place among the trip's fragments of its cell), and then adds each line's counts
-// with 16 neighbouring lanes, which the L1 merges into one request per line.  A fragment that finds
-// the table full takes its place with its own atomic.  Cell indices stay below 2^36, so a line
-// index fits 32 bits.
 template <uint32_t kLines>
 __global__ void __launch_bounds__(kThreads)
-    cell_count(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
-               uint32_t *ccnt, uint32_t *__restrict__ place) {
+    build_tag_leaf(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+                   const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint2 *__restrict__ node,
+                   uint32_t *cnt) {
 	__shared__ uint32_t line[kLines];
 	__shared__ uint32_t n[kLines * kLineBins];
 	for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
@@ -558,29 +539,41 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u)
 			fr[u] = src[u].frag ? *src[u].frag : make_uint2(0u, 0u);
-		uint32_t slot[kU], local[kU];
-		uint64_t cell[kU];
+		// rank of the fragment's depth-(L-1) node among its scene's occupied nodes of that depth
+		// (w0, the depth's first word, starts a chunk)
+		uint32_t below[kU], r[kU], r0[kU], bin[kU];
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
+			r[u] = r0[u] = below[u] = 0u;
+			if (src[u].frag && src[u].L > 1u) {
+				const uint64_t P = frag_parent(fr[u], src[u].L);
+				uint32_t word;
+				r[u] = bits_before(B, R, src[u].w0 + (P >> 5), word);
+				r0[u] = R[src[u].w0 >> 3];
+				below[u] = word & ((1u << (uint32_t)(P & 31u)) - 1u);
+			}
+		}
+		uint32_t slot[kU], local[kU];
+#pragma unroll
+		for (uint32_t u = 0; u < kU; ++u) {
+			bin[u] = src[u].frag ? src[u].bin_begin + (r[u] - r0[u]) + (uint32_t)__popc(below[u]) : kEmpty;
 			slot[u] = kEmpty;
 			local[u] = 0u;
-			cell[u] = ~0ull;
-			if (src[u].frag) {
-				cell[u] = src[u].cell0 + (frag_parent(fr[u], src[u].L) >> src[u].k3);
-				const uint32_t key = (uint32_t)(cell[u] / kLineBins);
+			if (bin[u] != kEmpty) {
+				const uint32_t key = bin[u] / kLineBins;
 				static_assert(kLines == 128u || kLines == 256u, "kLines: 7 or 8 hash bits");
 				uint32_t h = (key * 2654435761u) >> (kLines == 256u ? 24 : 25), probes = 0;
 				for (;;) {
 					const uint32_t old = atomicCAS(&line[h], kEmpty, key);
 					if (old == kEmpty || old == key) {
-						slot[u] = h * kLineBins + (uint32_t)(cell[u] % kLineBins);
+						slot[u] = h * kLineBins + bin[u] % kLineBins;
 						break;
 					}
 					if (++probes == kLines)
 						break; // table full
 					h = (h + 1u) & (kLines - 1u);
 				}
-				local[u] = slot[u] != kEmpty ? atomicAdd(&n[slot[u]], 1u) : atomicAdd(&ccnt[cell[u]], 1u);
+				local[u] = slot[u] != kEmpty ? atomicAdd(&n[slot[u]], 1u) : atomicAdd(&cnt[bin[u]], 1u);
 			}
 		}
 		__syncthreads();
@@ -600,15 +593,15 @@ __global__ void __launch_bounds__(kThreads)
 			for (uint32_t k = 0; k < kGroup; ++k) {
 				const uint32_t p = threadIdx.x + (k0 + k) * kThreads;
 				if (key[k] != kEmpty && c[k] != 0u)
-					n[p] = atomicAdd(&ccnt[(uint64_t)key[k] * kLineBins + p % kLineBins], c[k]);
+					n[p] = atomicAdd(&cnt[key[k] * kLineBins + p % kLineBins], c[k]);
 			}
 		}
 		__syncthreads();
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
 			const uint64_t f = base + (uint64_t)u * kThreads + threadIdx.x;
-			if (src[u].frag)
-				place[f] = slot[u] != kEmpty ? n[slot[u]] + local[u] : local[u];
+			if (bin[u] != kEmpty)
+				node[f] = make_uint2(bin[u], slot[u] != kEmpty ? n[slot[u]] + local[u] : local[u]);
 		}
 		__syncthreads();
 		for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
@@ -619,229 +612,33 @@ __global__ void __launch_bounds__(kThreads)
 	}
 }
 
-// Every fragment to its place in its cell's stretch of bucket0 (coff[c] = first slot of cell c).
+// Scatter every fragment to its place in its bin's bucket (aux[j] = first slot of bin j).
 __global__ void __launch_bounds__(kThreads)
-    cell_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
-                 const uint32_t *__restrict__ place, const uint32_t *__restrict__ coff, uint2 *__restrict__ bucket0) {
+    build_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+                  const uint2 *__restrict__ node, const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
 		FragSrc src[kU];
 		frag_sources(segs, fbeg, S, Ftot, base, hint, src);
-		uint2 fr[kU];
-		uint32_t p[kU];
+		uint2 fr[kU], np[kU];
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u) {
 			const uint64_t f = base + (uint64_t)u * kThreads + threadIdx.x;
-			fr[u] = make_uint2(0u, 0u);
-			p[u] = 0u;
+			fr[u] = np[u] = make_uint2(0u, 0u);
 			if (src[u].frag) {
 				fr[u] = *src[u].frag;
-				p[u] = place[f];
+				np[u] = node[f];
 			}
 		}
 		uint32_t first[kU];
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u)
-			first[u] = src[u].frag ? coff[src[u].cell0 + (frag_parent(fr[u], src[u].L) >> src[u].k3)] : 0u;
+			first[u] = src[u].frag ? aux[np[u].x] : 0u;
 #pragma unroll
 		for (uint32_t u = 0; u < kU; ++u)
 			if (src[u].frag)
-				bucket0[first[u] + p[u]] = fr[u];
-	}
-}
-
-// One cell of n <= kCellWave fragments (bucket0[o, o + n)), by one wave: the keys (leaf block
-// within the cell << 8 | position) are bitonic-sorted in the wave's LDS, so the fragments of each
-// leaf block become consecutive, in leaf-block order; the fragments go to bucket[o, o + n) in that
-// order, and each leaf block's first position and count to aux / cnt of bin fbin + its ordinal
-// among the cell's leaf blocks.  All arguments are wave-uniform.
-__device__ __forceinline__ void group_cell_wave(uint32_t n, uint32_t o, uint32_t fbin, uint32_t L, uint32_t k3,
-                                                const uint2 *__restrict__ bucket0, uint2 *__restrict__ bucket,
-                                                uint32_t *cnt, uint32_t *aux, uint32_t *s_key, uint2 *s_frag,
-                                                uint32_t *s_head) {
-	const uint32_t lane = threadIdx.x & 63u;
-	const uint32_t m = n <= 128u ? 128u : kCellWave;
-	const uint32_t lmask = (1u << k3) - 1u;
-	for (uint32_t i = lane; i < m; i += 64u) {
-		uint32_t key = kEmpty;
-		if (i < n) {
-			const uint2 fr = bucket0[o + i];
-			s_frag[i] = fr;
-			key = (((uint32_t)frag_parent(fr, L) & lmask) << 8u) | i;
-		}
-		s_key[i] = key;
-	}
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-	for (uint32_t k = 2; k <= m; k <<= 1) {
-		for (uint32_t jj = k >> 1; jj > 0; jj >>= 1) {
-			for (uint32_t q = lane; q < m / 2u; q += 64u) {
-				const uint32_t blk = q / jj, off = q % jj;
-				const uint32_t a = jj == (k >> 1) ? blk * k + off : blk * 2u * jj + off;
-				const uint32_t c = jj == (k >> 1) ? blk * k + k - 1u - off : a + jj;
-				const uint32_t ka = s_key[a], kc = s_key[c];
-				if (kc < ka) {
-					s_key[a] = kc;
-					s_key[c] = ka;
-				}
-			}
-			__builtin_amdgcn_wave_barrier();
-			__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-		}
-	}
-	// positions 4 lane .. 4 lane + 3: leaf-block heads and their ordinals (a prefix over the wave)
-	uint32_t kk[4], nh = 0u;
-	bool hd[4];
-#pragma unroll
-	for (uint32_t t = 0; t < 4u; ++t)
-		kk[t] = 4u * lane + t < n ? s_key[4u * lane + t] : kEmpty;
-	const uint32_t prev_last = __shfl_up(kk[3] >> 8u, 1, 64);
-#pragma unroll
-	for (uint32_t t = 0; t < 4u; ++t) {
-		const uint32_t i = 4u * lane + t, prev = t ? kk[t - 1u] >> 8u : prev_last;
-		hd[t] = i < n && (i == 0u || (kk[t] >> 8u) != prev);
-		nh += hd[t] ? 1u : 0u;
-	}
-	uint32_t incl = nh;
-#pragma unroll
-	for (uint32_t dd = 1; dd < 64u; dd <<= 1) {
-		const uint32_t v = __shfl_up(incl, dd, 64);
-		incl += lane >= dd ? v : 0u;
-	}
-	const uint32_t H = __shfl(incl, 63, 64), excl = incl - nh;
-	uint32_t ord = excl;
-#pragma unroll
-	for (uint32_t t = 0; t < 4u; ++t) {
-		const uint32_t i = 4u * lane + t;
-		if (i < n) {
-			bucket[o + i] = s_frag[kk[t] & 0xffu];
-			if (hd[t]) {
-				aux[fbin + ord] = o + i;
-				s_head[ord] = i;
-				++ord;
-			}
-		}
-	}
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-	ord = excl;
-#pragma unroll
-	for (uint32_t t = 0; t < 4u; ++t)
-		if (hd[t]) {
-			const uint32_t next = ord + 1u < H ? s_head[ord + 1u] : n;
-			cnt[fbin + ord] = next - (4u * lane + t);
-			++ord;
-		}
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-}
-
-// Items are the dense cells of all scenes (begin table cb).  Each lane reads its cell's count and
-// offset and, for an occupied cell, the bin of the cell's first leaf block (the rank of its first
-// depth-(L-1) node); then the wave groups its cells of at most kCellWave fragments one after the
-// other, and lists the larger ones for cell_group_big as {offset, count, first bin, L | k3 << 8}.
-// The bin of a scene with L = 1 and no fragments (its root block) gets cnt 0 here.
-__global__ void __launch_bounds__(kThreads)
-    cell_group(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ cb, uint32_t S, uint64_t ncells,
-               const uint32_t *__restrict__ ccnt, const uint32_t *__restrict__ coff, const uint32_t *__restrict__ B,
-               const uint32_t *__restrict__ R, const uint2 *__restrict__ bucket0, uint2 *__restrict__ bucket,
-               uint32_t *cnt, uint32_t *aux, uint32_t *big_count, uint4 *big, uint64_t big_cap) {
-	constexpr uint32_t kWaves = kThreads / 64;
-	__shared__ uint32_t s_key[kWaves][kCellWave];
-	__shared__ uint2 s_frag[kWaves][kCellWave];
-	__shared__ uint32_t s_head[kWaves][kCellWave];
-	const uint32_t wave = threadIdx.x >> 6;
-	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
-	uint32_t hint = 0;
-	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < ncells; base += stride) {
-		const uint64_t c = base + threadIdx.x;
-		uint32_t n = 0u, o = 0u, fbin = 0u, L = 1u, k3 = 0u;
-		if (c < ncells)
-			with_seg<kThreads>(cb, S, hint, base, c, [&](uint32_t s) {
-				const BuildSeg &g = segs[s];
-				n = ccnt[c];
-				o = coff[c];
-				L = g.L;
-				k3 = g.k3;
-				fbin = (uint32_t)g.bin_begin;
-				if (L > 1u && n > 0u) {
-					const uint64_t w0 = g.bm + g.off[L - 1u];
-					const uint64_t bit = (c - cb[s]) << k3; // the cell's first depth-(L-1) node
-					uint32_t word;
-					const uint32_t r = bits_before(B, R, w0 + (bit >> 5), word) - R[w0 >> 3];
-					fbin += r + (uint32_t)__popc(word & ((1u << (uint32_t)(bit & 31u)) - 1u));
-				}
-			});
-		if (c < ncells && n == 0u && L == 1u) {
-			cnt[fbin] = 0u;
-			aux[fbin] = o;
-		}
-		const bool is_big = n > kCellWave;
-		const uint32_t q = wave_append(big_count, is_big);
-		if (is_big && q < big_cap)
-			big[q] = make_uint4(o, n, fbin, L | (k3 << 8u));
-		uint64_t todo = __ballot(n > 0u && n <= kCellWave);
-		while (todo) {
-			const int src = __ffsll((unsigned long long)todo) - 1;
-			todo &= todo - 1ull;
-			group_cell_wave(__builtin_amdgcn_readlane(n, src), __builtin_amdgcn_readlane(o, src),
-			                __builtin_amdgcn_readlane(fbin, src), __builtin_amdgcn_readlane(L, src),
-			                __builtin_amdgcn_readlane(k3, src), bucket0, bucket, cnt, aux, s_key[wave], s_frag[wave],
-			                s_head[wave]);
-		}
-	}
-}
-
-// Cells of more than kCellWave fragments, one block each: counting sort by leaf block within the
-// cell (at most kCellSpan counters in LDS): count, exclusive scans of the counts and of the occupied
-// flags (the leaf blocks' positions and ordinals), then each fragment takes its slot with an LDS
-// atomic.  Within a leaf block the order is arbitrary (the fold sorts by fragment).
-__global__ void __launch_bounds__(kThreads)
-    cell_group_big(const uint32_t *__restrict__ big_count, const uint4 *__restrict__ big, uint64_t big_cap,
-                   const uint2 *__restrict__ bucket0, uint2 *__restrict__ bucket, uint32_t *cnt, uint32_t *aux) {
-	constexpr uint32_t kPer = kCellSpan / kThreads; // counters per thread in the scans
-	__shared__ uint32_t s_h[kCellSpan];
-	uint64_t count = *big_count;
-	if (count > big_cap)
-		count = big_cap;
-	for (uint64_t item = blockIdx.x; item < count; item += gridDim.x) {
-		const uint4 e = big[item];
-		const uint32_t o = e.x, n = e.y, fbin = e.z, L = e.w & 0xffu, k3 = e.w >> 8u;
-		const uint32_t lmask = (1u << k3) - 1u;
-		for (uint32_t j = threadIdx.x; j < kCellSpan; j += kThreads)
-			s_h[j] = 0u;
-		__syncthreads();
-		for (uint32_t i = threadIdx.x; i < n; i += kThreads)
-			atomicAdd(&s_h[(uint32_t)frag_parent(bucket0[o + i], L) & lmask], 1u);
-		__syncthreads();
-		uint32_t h[kPer], csum = 0u, fsum = 0u;
-#pragma unroll
-		for (uint32_t t = 0; t < kPer; ++t) {
-			h[t] = s_h[threadIdx.x * kPer + t];
-			csum += h[t];
-			fsum += h[t] ? 1u : 0u;
-		}
-		uint64_t excl;
-		block_excl_scan((uint64_t)csum | ((uint64_t)fsum << 32), excl);
-		uint32_t pos = (uint32_t)excl, ord = (uint32_t)(excl >> 32);
-#pragma unroll
-		for (uint32_t t = 0; t < kPer; ++t) {
-			if (h[t]) {
-				cnt[fbin + ord] = h[t];
-				aux[fbin + ord] = o + pos;
-				++ord;
-			}
-			s_h[threadIdx.x * kPer + t] = pos; // cursor
-			pos += h[t];
-		}
-		__syncthreads();
-		for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
-			const uint2 fr = bucket0[o + i];
-			const uint32_t p = atomicAdd(&s_h[(uint32_t)frag_parent(fr, L) & lmask], 1u);
-			bucket[o + p] = fr;
-		}
-		__syncthreads();
+				bucket[first[u] + np[u].y] = fr[u];
 	}
 }
 
@@ -1328,18 +1125,18 @@ __global__ void __launch_bounds__(kThreads)
 		                cap, s_keys, s_slots, s_beg[0], s_end[0], s_magic);
 }
 
-// ---- scan plumbing for the cell offsets --------------------------------------------------------
-struct CellCounts {
+// ---- scan plumbing for the bin offsets ---------------------------------------------------------
+struct BinCounts {
 	static constexpr int kItems = 4;
 	const uint32_t *cnt;
 	uint64_t n;
 	__device__ uint64_t size() const { return n; }
 	__device__ uint32_t operator()(uint64_t i) const { return cnt[i]; }
 };
-struct CellOffsets {
-	uint32_t *off;
+struct BinOffsets {
+	uint32_t *aux;
 	__device__ bool active() const { return true; }
-	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { off[i] = (uint32_t)k; }
+	__device__ void operator()(uint64_t i, uint64_t k, uint32_t) const { aux[i] = (uint32_t)k; }
 };
 
 // Blocks for a grid-stride loop over items (kThreads per block and trip), at most per_cu per CU:
@@ -1412,44 +1209,34 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	const uint32_t S = bb.S;
 	const BuildSeg *segs = bb.segs;
 	hipError_t e;
-	uint32_t *coff = bb.ccnt + bb.ncells;
-	uint32_t *big_count = reinterpret_cast<uint32_t *>(bb.bigc);
 	{
 		// with maxL == 1 there are no parent words and no root blocks to write (the bitmaps are empty)
 		const uint32_t M = bb.pw_segs();
 		const uint64_t n = bb.maxL > 1u ? bb.h_index[bb.pw_at() + M] : 0u; // parent words of all depths >= 2
-		if ((e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(std::max(n, bb.ncells), ctx)),
+		if ((e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(std::max(n, bb.nbins), ctx)),
 		                dim3(kThreads), 0, segs, bb.pw(), M, bb.maxL > 1u ? S : 0u, n, (const uint32_t *)bb.bm,
-		                (const uint32_t *)bb.rk, bb.ccnt, bb.ncells, bb.list, big_count)) != hipSuccess)
+		                (const uint32_t *)bb.rk, bb.cnt, bb.nbins, bb.list)) != hipSuccess)
 			return hip_check(ctx, e, "bm_write_blocks");
 	}
 	const uint32_t gF = grid_tiles(bb.Ftot, ctx);
+	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
 	if (bb.Ftot > 0 &&
-	    (e = launch(ctx, "cell_count", bb.maxL >= 12u ? cell_count<256> : cell_count<128>, dim3(gF), dim3(kThreads), 0,
-	                segs, bb.fbeg(), S, bb.Ftot, bb.ccnt, bb.place)) != hipSuccess)
-		return hip_check(ctx, e, "cell_count");
-	CellCounts cc{bb.ccnt, bb.ncells};
-	if ((e = launch(ctx, "cell_reduce", scan_reduce<CellCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, cc,
+	    (e = launch(ctx, "build_tag_leaf", bb.maxL >= 12u ? build_tag_leaf<256> : build_tag_leaf<128>, dim3(gF),
+	                dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk,
+	                node, bb.cnt)) != hipSuccess)
+		return hip_check(ctx, e, "build_tag_leaf");
+	BinCounts bc{bb.cnt, bb.nbins};
+	if ((e = launch(ctx, "bin_reduce", scan_reduce<BinCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
 	                ctx->d_scan_parts)) != hipSuccess)
-		return hip_check(ctx, e, "cell_reduce");
-	if ((e = launch(ctx, "cell_offsets", scan_down<CellCounts, CellOffsets>, dim3(kScanBlocks), dim3(kScanThreads), 0,
-	                cc, CellOffsets{coff}, (const uint64_t *)ctx->d_scan_parts, NoCommit{})) != hipSuccess)
-		return hip_check(ctx, e, "cell_offsets");
-	uint2 *bucket0 = reinterpret_cast<uint2 *>(bb.bucket0), *bucket = reinterpret_cast<uint2 *>(bb.bucket);
-	if (bb.Ftot > 0 && (e = launch(ctx, "cell_scatter", cell_scatter, dim3(grid_tiles(bb.Ftot, ctx, 256u)),
-	                               dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.place,
-	                               (const uint32_t *)coff, bucket0)) != hipSuccess)
-		return hip_check(ctx, e, "cell_scatter");
-	if ((e = launch(ctx, "cell_group", cell_group, dim3(grid_for(bb.ncells, ctx, 16u)), dim3(kThreads), 0, segs,
-	                bb.cb(), S, bb.ncells, (const uint32_t *)bb.ccnt, (const uint32_t *)coff, (const uint32_t *)bb.bm,
-	                (const uint32_t *)bb.rk, (const uint2 *)bucket0, bucket, bb.cnt, bb.aux, big_count, bb.bigc + 1,
-	                bb.big_cap)) != hipSuccess)
-		return hip_check(ctx, e, "cell_group");
-	if (bb.big_cap > 0 &&
-	    (e = launch(ctx, "cell_group_big", cell_group_big, dim3(std::min<uint64_t>(bb.big_cap, (uint64_t)ctx->cu_count * 4u)),
-	                dim3(kThreads), 0, (const uint32_t *)big_count, (const uint4 *)(bb.bigc + 1), bb.big_cap,
-	                (const uint2 *)bucket0, bucket, bb.cnt, bb.aux)) != hipSuccess)
-		return hip_check(ctx, e, "cell_group_big");
+		return hip_check(ctx, e, "bin_reduce");
+	if ((e = launch(ctx, "bin_offsets", scan_down<BinCounts, BinOffsets>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
+	                BinOffsets{bb.aux}, (const uint64_t *)ctx->d_scan_parts, NoCommit{})) != hipSuccess)
+		return hip_check(ctx, e, "bin_offsets");
+	uint2 *bucket = reinterpret_cast<uint2 *>(bb.bucket);
+	if (bb.Ftot > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(grid_tiles(bb.Ftot, ctx, 256u)),
+	                               dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint2 *)node,
+	                               (const uint32_t *)bb.aux, bucket)) != hipSuccess)
+		return hip_check(ctx, e, "build_scatter");
 	const uint64_t cap = big_capacity(bb.Ftot);
 	if ((e = launch(ctx, "build_fold", build_fold, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, segs, bb.bbeg(),
 	                S, bb.nbins, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket, bb.list,
@@ -1464,21 +1251,6 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 	return SVO_OK;
 }
 
-// Cell depth D of a scene (with L - 1 - 4 <= D <= min(L - 1, 8), so a cell spans at most
-// kCellSpan leaf blocks and a scene at most 8^8 dense counters): the shallowest depth whose nodes
-// hold at most kCellTarget fragments on average, from the per-depth node counts of the structure
-// phase (N_d = fb[d + 1] - fb[d]; N_0 = 1, the root).
-constexpr uint64_t kCellTarget = 128;
-uint32_t cell_depth(const BuildSeg &g) {
-	if (g.L <= 1u)
-		return 0u;
-	const uint32_t lo = g.L - 1u > 4u ? g.L - 5u : 0u, hi = std::min(g.L - 1u, 8u);
-	for (uint32_t d = lo; d <= hi; ++d)
-		if (g.F <= kCellTarget * (uint64_t)(g.fb[d + 1u] - g.fb[d]))
-			return d;
-	return hi;
-}
-
 } // namespace
 
 uint64_t build_bitmap_words(uint32_t L) {
@@ -1489,13 +1261,11 @@ uint64_t build_bitmap_words(uint32_t L) {
 
 void build_batch_release(svo_ctx *ctx, BuildBatch *bb) {
 	for (void *p : {(void *)bb->segs, (void *)bb->bm, (void *)bb->rk, (void *)bb->cnt, (void *)bb->aux,
-	                (void *)bb->ccnt, (void *)bb->place, (void *)bb->bucket0, (void *)bb->bucket, (void *)bb->list,
-	                (void *)bb->bigc}) // index lives in segs' allocation
+	                (void *)bb->node, (void *)bb->bucket, (void *)bb->list}) // index lives in segs' allocation
 		ctx->pool->free(p);
 	bb->segs = nullptr;
 	bb->index = nullptr;
-	bb->bm = bb->rk = bb->cnt = bb->aux = bb->ccnt = bb->place = bb->bucket0 = bb->bucket = bb->list = nullptr;
-	bb->bigc = nullptr;
+	bb->bm = bb->rk = bb->cnt = bb->aux = bb->node = bb->bucket = bb->list = nullptr;
 }
 
 // The whole build of S octrees: structure phase, one read-back of every scene's W and leaf-block
@@ -1510,7 +1280,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	bb.S = S;
 	bb.h_segs.assign(S, BuildSeg{});
 	const uint64_t stride = S + 1;
-	bb.h_index.assign(bb.index_size(), 0);
+	bb.h_index.assign(bb.pw_at() + bb.pw_segs() + 1, 0);
 	uint64_t *fbeg = bb.h_index.data();
 	for (uint32_t s = 0; s < S; ++s) {
 		const svo_fraglist *fl = octs[s]->src;
@@ -1567,15 +1337,12 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 		std::memcpy(tables.data() + seg_bytes, bb.h_index.data(), idx_bytes);
 		return upload_table(ctx, bb.segs, tables.data(), tables.size());
 	};
-	bb.big_cap = bb.Ftot / (kCellWave + 1u) + 1u; // cells of more than kCellWave fragments
 	if (pool->alloc((void **)&bb.segs, seg_bytes + idx_bytes) != hipSuccess ||
 	    pool->alloc((void **)&bb.bm, sizeof(uint32_t) * (bb.TW ? bb.TW : 1)) != hipSuccess ||
 	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW / 8u + 1)) != hipSuccess ||
-	    pool->alloc((void **)&bb.place, sizeof(uint32_t) * Fa) != hipSuccess ||
-	    pool->alloc((void **)&bb.bucket0, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
+	    pool->alloc((void **)&bb.node, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
 	    pool->alloc((void **)&bb.bucket, sizeof(uint32_t) * 2 * Fa) != hipSuccess ||
-	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (3 + 3 * cap)) != hipSuccess ||
-	    pool->alloc((void **)&bb.bigc, sizeof(uint4) * (1 + bb.big_cap)) != hipSuccess)
+	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (3 + 3 * cap)) != hipSuccess)
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	if (!rc) {
 		bb.index = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(bb.segs) + seg_bytes);
@@ -1596,7 +1363,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 				               "octree " + std::to_string(s) + " needs more than 2^30 words (30-bit child pointers)");
 				break;
 			}
-	uint64_t *bbeg = bb.h_index.data() + stride, *cbeg = bb.h_index.data() + bb.cb_at();
+	uint64_t *bbeg = bb.h_index.data() + stride;
 	if (!rc) {
 		for (uint32_t s = 0; s < S; ++s) {
 			BuildSeg &g = bb.h_segs[s];
@@ -1610,18 +1377,11 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 			g.bin_begin = bb.nbins;
 			bbeg[s] = bb.nbins;
 			bb.nbins += g.nbins;
-			const uint32_t D = cell_depth(g);
-			g.k3 = 3u * (g.L - 1u - D);
-			g.cbeg = bb.ncells;
-			cbeg[s] = bb.ncells;
-			bb.ncells += 1ull << (3u * D);
 		}
 		bbeg[S] = bb.nbins;
-		cbeg[S] = bb.ncells;
 	}
 	if (!rc && (pool->alloc((void **)&bb.cnt, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess ||
-	            pool->alloc((void **)&bb.aux, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess ||
-	            pool->alloc((void **)&bb.ccnt, sizeof(uint32_t) * 2 * bb.ncells) != hipSuccess))
+	            pool->alloc((void **)&bb.aux, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess))
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	if (!rc)
 		rc = upload_tables();
