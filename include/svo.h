@@ -201,6 +201,18 @@ int svo_march(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32
  * words each beam cell loaded. */
 int svo_march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width, uint32_t height,
                    float *beam_dev, uint32_t *nfetch_dev);
+/* Allocation-free primary march.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same records as svo_march(ctx, oct, cam, width, height, 1, hits_dev, NULL), from the
+ * same kernels, with the beam image kept in a static device array of 2^20 cells (4 MiB per device,
+ * held until the process ends) instead of pool memory: frames need (ceil(W/8)+1)*(ceil(H/8)+1) <=
+ * 1048576 beam cells (up to 7680x4320, for example); larger frames return SVO_ERR_INVALID_ARGUMENT.
+ * The launch schedule is fixed (svo_march_mode's, else one ray per lane in 16x16 tiles; the autotuner
+ * is not consulted) and no stage timer runs.  With the profiler off (svo_prof_enable) it calls no
+ * allocating routine.  Every call on one device shares the array, which keeps the last call's beam
+ * image: calls that overlap on one GPU, from any context, race; serialise them. */
+int svo_march_noalloc(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t width,
+                      uint32_t height, svo_hit *hits_dev);
 /* The tracer's shaded image (shader/octree_tracer.frag:18-47, OctreeTracer::m_view_type,
  * src/OctreeTracer.hpp:15) from a frame of hit records (svo_march's hits_dev) and the camera they
  * were marched with.  view_type: 0 = diffuse, pow(albedo, 1/2.2); 1 = normal * 0.5 + 0.5;

@@ -147,6 +147,7 @@ _SIGS = {
     "svo_octree_load": (i32, [P, C.c_char_p, PP]),
     "svo_march": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, P, P]),
     "svo_march_beam": (i32, [P, P, C.POINTER(svo_camera), u32, u32, P, P]),
+    "svo_march_noalloc": (i32, [P, P, C.POINTER(svo_camera), u32, u32, P]),
     "svo_shade": (i32, [P, P, C.POINTER(svo_camera), u32, u32, i32, C.POINTER(C.c_float), P, f32, P]),
     "svo_octree_transfer": (i32, [P, P]),
     "svo_march_rays": (i32, [P, P, P, u64, i32, P]),

@@ -307,6 +307,10 @@ int svo_ctx_create(int device, svo_ctx **out) {
 		svo_ctx_destroy(ctx);
 		return SVO_ERR_OUT_OF_MEMORY;
 	}
+	if (noalloc_init(ctx) != SVO_OK) {
+		svo_ctx_destroy(ctx);
+		return SVO_ERR_HIP;
+	}
 	(void)hipMemsetAsync(ctx->d_err, 0, sizeof(uint32_t), ctx->stream);
 	(void)hipMemsetAsync(ctx->d_state, 0, sizeof(BuildState), ctx->stream);
 	if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
@@ -987,6 +991,16 @@ int svo_march(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t
 		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "frame too large");
 	DeviceGuard g(ctx);
 	return march_frame(ctx, o, cam, width, height, beam, hits_dev, beam_dev);
+}
+
+int svo_march_noalloc(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t width, uint32_t height,
+                      svo_hit *hits_dev) {
+	if (!ctx || !o || !cam || !hits_dev)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (width == 0 || height == 0)
+		return SVO_OK;
+	DeviceGuard g(ctx);
+	return march_frame_noalloc(ctx, o, cam, width, height, hits_dev);
 }
 
 int svo_march_beam(svo_ctx *ctx, const svo_octree *o, const svo_camera *cam, uint32_t width, uint32_t height,

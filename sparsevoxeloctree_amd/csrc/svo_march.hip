@@ -41,6 +41,7 @@
 //  THIS SOFTWARE, EVEN IF ADVISED OF THE POSSIBILITY OF SUCH DAMAGE.
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "svo_batch.hpp"
@@ -1508,7 +1509,46 @@ int march_frame_into(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam,
 	return SVO_OK;
 }
 
+// ---- allocation-free primary march ----------------------------------------------------------------
+// The beam image of svo_march_noalloc lives in this file-scope device array, one copy per GPU for the
+// life of the process.  In host code `scratch` names a host-side stand-in, not device memory: its
+// device address comes from hipGetSymbolAddress, taken once per device with that device current
+// (noalloc_init, called when a context is created) and kept in g_scratch_dev.
+constexpr uint64_t kNoAllocBeamCells = 1ull << 20; // 4 MiB of floats: (ceil(W/8)+1)(ceil(H/8)+1) cells
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) float scratch[kNoAllocBeamCells];
+std::mutex g_scratch_mu;
+float *g_scratch_dev[kMaxDevices] = {};
+
 } // namespace
+
+int noalloc_init(svo_ctx *ctx) {
+	if (ctx->device < 0 || ctx->device >= kMaxDevices)
+		return set_error(ctx, SVO_ERR_NO_DEVICE, "device index beyond the allocation-free march's table");
+	std::lock_guard<std::mutex> lk(g_scratch_mu);
+	float *&p = g_scratch_dev[ctx->device];
+	if (!p) {
+		void *addr = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !addr)
+			return hip_check(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+		p = static_cast<float *>(addr);
+	}
+	ctx->noalloc_beam = p;
+	return SVO_OK;
+}
+
+int march_frame_noalloc(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
+                        svo_hit *hits) {
+	const uint64_t cells = ((uint64_t)(w + 7u) / 8u + 1u) * ((uint64_t)(h + 7u) / 8u + 1u);
+	if (cells > kNoAllocBeamCells)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT,
+		                 "frame too large for the allocation-free march: (ceil(W/8)+1)*(ceil(H/8)+1) = " +
+		                     std::to_string(cells) + " beam cells, at most " + std::to_string(kNoAllocBeamCells));
+	if (!ctx->noalloc_beam)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "allocation-free march not set up on this context");
+	return march_frame_into(ctx, oct, cam, w, h, ctx->noalloc_beam, hits, false, false);
+}
 
 int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, int beam,
                 svo_hit *hits, float *beam_out) {

@@ -124,6 +124,7 @@ struct svo_ctx {
 	svo::BuildState *d_state = nullptr;
 	uint32_t *d_err = nullptr;
 	uint32_t *d_sobol = nullptr; // Sobol state of the path tracer's current sample: 64 words + counter
+	float *noalloc_beam = nullptr; // device address of this device's static beam array (svo_march_noalloc)
 	svo::BuildState *h_state = nullptr; // pinned mirror for read-back
 	svo::StageTimer t_normalize, t_count, t_fill, t_build, t_build2, t_beam, t_march, t_rays, t_bounce, t_path;
 	svo::Staging stage;
@@ -214,6 +215,12 @@ int march_frame(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint
                 svo_hit *hits, float *beam_out);
 int march_beam(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h, float *beam,
                uint32_t *nfetch);
+// Allocation-free primary march: noalloc_init looks up the device address of the static beam array
+// for the context's device (once per device; the device must be current), march_frame_noalloc
+// marches with the beam image there.
+int noalloc_init(svo_ctx *ctx);
+int march_frame_noalloc(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam, uint32_t w, uint32_t h,
+                        svo_hit *hits);
 int march_rays(svo_ctx *ctx, const svo_octree *oct, const float *rays, uint64_t R, int variant, svo_hit *hits);
 int bounce_rays(svo_ctx *ctx, const svo_hit *hits, uint32_t width, uint32_t height, uint32_t spp, uint32_t cur,
                 float *rays);
