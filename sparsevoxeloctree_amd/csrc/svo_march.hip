@@ -491,67 +491,28 @@ __global__ void __launch_bounds__(kBlock)
 // Rays of one march diverge: a wave lasts as long as its longest ray while finished lanes idle.
 // Here a fixed grid of waves marches all rays; a lane whose ray has ended writes its record and,
 // once kRefill lanes of the wave are idle, they take the next rays of the wave's pool.  The rays
-// are dealt to the waves in chunks of 64 (an 8x8-pixel tile of a frame) by ChunkDealer, without
-// atomics.  Every ray runs the same per-ray arithmetic as the one-ray-per-lane kernels, so the
+// are dealt to the waves in chunks of 64 (an 8x8-pixel tile of a frame), chunk c to wave
+// c mod (number of waves), so costly image regions spread over all waves and no atomics are
+// needed.  Every ray runs the same per-ray arithmetic as the one-ray-per-lane kernels, so the
 // records are identical; only which lane marches which ray changes.
 // A refill costs a ray set-up for the whole wave, so checks are spaced: 16 idle lanes and 16 trips
 // between checks measured best (4 or 8 trips: 4-8% slower; 32 trips or 32 lanes: slower again).
 constexpr uint32_t kRefill = 16; // idle lanes that trigger a refill
 constexpr uint32_t kTrips = 16;  // loop trips between refill checks
 constexpr uint32_t kChunk = 64;  // rays per chunk
-constexpr uint32_t kBandChunks = 2048; // chunks per band (ChunkDealer)
-constexpr uint32_t kPanelTiles = 32;   // frame tiles per panel row (FrameRays)
-
-// Chunk dealing of a persistent grid.  Chunks form bands of kBandChunks consecutive chunks; band b
-// goes to the group of blocks b mod G, G = min(8, blocks).  Blocks are dispatched to the 8 XCDs
-// round-robin (block i to XCD i mod 8), so a group is one XCD's blocks, and its waves, taking the
-// group's chunks round-robin, work on one band at a time: the band's octree working set stays in
-// that XCD's L2, while the bands of costly image regions still spread over all XCDs.  Chunk
-// c = ((j / B) * G + g) * B + j % B is the j-th chunk of group g (B = kBandChunks), a bijection
-// of (g, j) onto the chunks, increasing in j, so a wave stops at its first chunk past the end.
-struct ChunkDealer {
-	uint64_t j, step;
-	uint32_t group, G;
-	__device__ explicit ChunkDealer(uint32_t waves_per_block) {
-		G = gridDim.x < 8u ? gridDim.x : 8u;
-		group = blockIdx.x % G;
-		const uint32_t nb = (gridDim.x - group + G - 1u) / G; // blocks of the group
-		j = (uint64_t)(blockIdx.x / G) * waves_per_block + (threadIdx.x >> 6);
-		step = (uint64_t)nb * waves_per_block;
-	}
-	__device__ uint64_t next() {
-		const uint64_t c = ((j / kBandChunks) * G + group) * kBandChunks + j % kBandChunks;
-		j += step;
-		return c;
-	}
-};
-
-// Tile of frame chunk c: tiles are numbered in panels kPanelTiles tiles wide (the last one
-// narrower), row after row inside a panel, so a band of chunks covers a compact region of the frame
-// (256 x 512 pixels for 2048 chunks) rather than a strip across it.
-__device__ __forceinline__ void panel_tile(uint32_t c, uint32_t tiles_x, uint32_t tiles_y, uint32_t &tx,
-                                           uint32_t &ty) {
-	const uint32_t per_panel = kPanelTiles * tiles_y;
-	const uint32_t p = c / per_panel, r = c - p * per_panel;
-	const uint32_t rest = tiles_x - p * kPanelTiles, pw = rest < kPanelTiles ? rest : kPanelTiles;
-	ty = r / pw;
-	tx = p * kPanelTiles + (r - ty * pw);
-}
 
 // Ray sources: ray(k, ...) sets up ray k (false: no ray, e.g. a pixel outside the frame) and the
 // address of its 32-byte record.
-struct FrameRays { // primary rays of one frame, numbered in 8x8-pixel tiles (panel_tile order)
+struct FrameRays { // primary rays of one frame, numbered in 8x8-pixel tiles
 	const uint32_t *oct;
 	uint64_t W;
 	CamArgs cam;
-	uint32_t width, height, bw, tiles_x, tiles_y;
+	uint32_t width, height, bw, tiles_x;
 	const float *beam;
 	uint4 *hits;
 	__device__ bool ray(uint64_t k, f3 &o, f3 &d, const uint32_t *&oc, uint64_t &w, uint4 *&out) const {
-		const uint32_t q = (uint32_t)k & 63u;
-		uint32_t tx, ty;
-		panel_tile((uint32_t)(k >> 6), tiles_x, tiles_y, tx, ty);
-		const uint32_t px = tx * 8u + (q & 7u), py = ty * 8u + (q >> 3);
+		const uint32_t tile = (uint32_t)(k >> 6), q = (uint32_t)k & 63u;
+		const uint32_t px = (tile % tiles_x) * 8u + (q & 7u), py = (tile / tiles_x) * 8u + (q >> 3);
 		if (px >= width || py >= height)
 			return false;
 		frame_ray(cam, width, height, beam, bw, px, py, o, d);
@@ -600,6 +561,8 @@ __global__ void __launch_bounds__(kT)
 	__shared__ uint32_t s_stack[kStackSlots * kT];
 	uint32_t *stk = s_stack + threadIdx.x;
 	const uint32_t lane = threadIdx.x & 63u;
+	const uint64_t wave = (uint64_t)blockIdx.x * (kT / 64u) + (threadIdx.x >> 6);
+	const uint64_t nwaves = (uint64_t)gridDim.x * (kT / 64u);
 	const uint64_t below = (1ull << lane) - 1ull;
 	March m;
 	f3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 0.0f};
@@ -608,19 +571,19 @@ __global__ void __launch_bounds__(kT)
 	uint4 *out = nullptr;
 	uint32_t iter = 0u, nfetch = 0u, err = 0u;
 	bool active = false;
-	// wave-uniform: the pool [pool_next, pool_end) and the dealer of the wave's chunks
-	uint64_t pool_next = 0, pool_end = 0;
-	ChunkDealer deal(kT / 64u);
+	// wave-uniform: the pool [pool_next, pool_end) and the wave's next chunk
+	uint64_t pool_next = 0, pool_end = 0, chunk = wave;
 	bool exhausted = false;
 	for (;;) {
 		const uint64_t idle = __ballot(!active);
 		const uint32_t nidle = (uint32_t)__popcll(idle);
 		if (nidle >= kRefill && (!exhausted || pool_next < pool_end)) {
 			if (pool_next == pool_end) {
-				const uint64_t lo = deal.next() * kChunk;
+				const uint64_t lo = chunk * kChunk;
 				if (lo < n) {
 					pool_next = lo;
 					pool_end = lo + kChunk < n ? lo + kChunk : n;
+					chunk += nwaves;
 				} else {
 					exhausted = true;
 				}
@@ -1474,7 +1437,7 @@ int march_frame_into(svo_ctx *ctx, const svo_octree *oct, const svo_camera *cam,
 	auto run = [&](int mode) -> hipError_t {
 		if (mode == 1) {
 			const uint32_t tx8 = (w + 7u) / 8u, ty8 = (h + 7u) / 8u;
-			FrameRays src{(const uint32_t *)oct->words, oct->W, ca, w, h, bw, tx8, ty8, beam_img,
+			FrameRays src{(const uint32_t *)oct->words, oct->W, ca, w, h, bw, tx8, beam_img,
 			              reinterpret_cast<uint4 *>(hits)};
 			return launch_persist<0>(ctx, "march_frame", src, (uint64_t)tx8 * ty8 * 64u);
 		}
