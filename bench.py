@@ -449,7 +449,10 @@ def bench_serve(args, cfg):
     wall = time.perf_counter() - t0
     st = srv.stats()
     nb, nr = int(kinds.sum()), int((~kinds).sum())
-    capacity = serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits)
+    # the capacity comparison starts from an empty device: the serving phase's resident octrees
+    # (up to the budget) would leave no room for the plain form's octrees
+    srv.close()
+    capacity = serve_capacity(S, ctx, cfg, rng, pool, levels, ray_sets, free_hits)
     line = {
         "metric": METRIC,
         "value": round(st["rays_traced"] / wall / 1e6, 3),
@@ -485,16 +488,16 @@ def bench_serve(args, cfg):
         "cpu_baseline": None,
         "roofline": None,
     }
-    srv.close()
     ctx.close()
     return line
 
 
-def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1000, n_targets=64):
+def serve_capacity(S, ctx, cfg, rng, pool, levels, ray_sets, free_hits, n=1000, n_targets=64):
     """Aggregate throughput of the same request list (10% builds, 90% ray queries of log-U[4K,2M] rays
     against n_targets octrees) in the plain form, one request per launch with a wait after each (the
-    single-scene API), and in the served form, all requests submitted at once to the serving loop.
-    Targets are built beforehand in both forms from the same scenes and levels."""
+    single-scene API), and in the served form, all requests submitted at once to a fresh serving loop
+    (same budget), after the plain form's octrees are freed.  Targets are built beforehand in both
+    forms from the same scenes and levels."""
     from sparsevoxeloctree_amd._capi import lib
 
     RMAX = 2 * 1024 * 1024
@@ -536,7 +539,9 @@ def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1
             rays_plain += int(Rs[k])
     t_plain = time.perf_counter() - t0
     del keep, plain_octs
+    ctx.trim()
     # served form: the same requests, submitted at once
+    srv = S.Server(0, budget_bytes=cfg["budget_gib"] << 30, max_octrees=cfg["max_octrees"])
     ids = [srv.submit_build(pool[sc], lv) for sc, lv in tgt]
     targets = []
     for rid in ids:
@@ -560,6 +565,7 @@ def serve_capacity(S, ctx, srv, cfg, rng, pool, levels, ray_sets, free_hits, n=1
         st, _ = srv.wait(rid)
         failed += st != 0
     t_served = time.perf_counter() - t0
+    srv.close()
     for b in bufs:
         free_hits.put(b)
     return {
@@ -695,12 +701,18 @@ def bench_primary(args, cfg, cpu=True):
     roofline = None
     if dominant in alg:
         achieved = alg[dominant] / (kernel_ms[dominant] * 1e-3) / 1e9
-        traffic = None
+        traffic = valu_busy = None
         cid = next(k for k, v in CONFIGS.items() if v is cfg)
         tpath = os.path.join(ROOT, "profiles", "traffic_%s_c%d.json" % (dominant, cid))
         if os.path.exists(tpath):
             with open(tpath) as f:
                 traffic = json.load(f).get("bytes_per_launch")
+        # VALU busy of the same kernel from the SQ counter pass (a march is bound by VALU issue,
+        # so its HBM fraction alone says little)
+        vpath = os.path.join(ROOT, "profiles", "valu_%s_c%d.json" % (dominant, cid))
+        if os.path.exists(vpath):
+            with open(vpath) as f:
+                valu_busy = json.load(f).get("valu_busy")
         roofline = {
             "kernel": dominant,
             "bound": "hbm",
@@ -709,6 +721,7 @@ def bench_primary(args, cfg, cpu=True):
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic,
+            "valu_busy": valu_busy,
             "alg_bytes_per_launch": alg[dominant],
             "avg_launch_ms": round(kernel_ms[dominant], 5),
             "copy_gbs": round(copy, 1),

@@ -99,6 +99,10 @@ class Context:
         self.check(lib.svo_ctx_take_device_errors(self.h, C.byref(b)), "device errors")
         return b.value
 
+    def trim(self):
+        """Returns the allocator's cached (free) device memory to the device (svo_ctx_trim)."""
+        self.check(lib.svo_ctx_trim(self.h), "trim")
+
     def memory(self):
         a, b, c = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self.check(lib.svo_ctx_memory(self.h, C.byref(a), C.byref(b), C.byref(c)))
