@@ -150,6 +150,18 @@ def test_c3_full_size(ctx, no_device_errors):
         ok, msg = hits_equal(hits[y0:y0 + 16], ref_rows)
         assert ok, "rows %d..%d: %s" % (y0, y0 + 16, msg)
     print("c3: beam and 5 bands of 16 rows identical", flush=True)
+    # the persistent schedule (the one the benchmark's autotuner picks at this size) on the same
+    # frame: every record identical to the one-ray-per-lane march checked above
+    from sparsevoxeloctree_amd._capi import lib
+
+    try:
+        ctx.check(lib.svo_march_mode(ctx.h, 1))
+        _, other, _ = _march(ctx, S, octree, cam, W, H, True)
+        ok, msg = hits_equal(other.reshape(H, W), hits)
+        assert ok, "persistent schedule: " + msg
+    finally:
+        ctx.check(lib.svo_march_mode(ctx.h, -1))
+    print("c3: persistent schedule identical", flush=True)
 
 
 def test_c4_all_scenes_batched_single_and_oracle(ctx, no_device_errors):
