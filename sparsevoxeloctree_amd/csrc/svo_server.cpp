@@ -10,7 +10,8 @@
 //                 queued or marching); a query for an octree that is not resident is refused at
 //                 submission;
 //   build worker  drains pending builds (up to kMaxBuildBatch), voxelizes them with one count and
-//                 one fill launch, builds them with one launch per pass, then admits each octree
+//                 one fill launch, builds them with one launch per pass (in groups whose transient
+//                 scratch fits the budget), then admits each octree
 //                 into the resident set by its exact size: least-recently-used unpinned octrees are
 //                 evicted until it fits the byte budget and the octree-count limit, else the build
 //                 is rejected (SVO_ERR_OUT_OF_MEMORY).
@@ -225,8 +226,11 @@ void svo_server::ray_loop() {
 	}
 }
 
-// Voxelize + build of a batch of build requests; on a batch failure each request is retried alone
-// so that one bad scene cannot fail the others.
+// Voxelize + build of a batch of build requests.  The batch is voxelized once; it is then built in
+// consecutive groups whose transient build scratch (20 B per fragment plus bitmaps and ranks) fits
+// the byte budget after evicting idle octrees: a group that does not fit is halved, and a single
+// scene that does not fit even alone is rejected (SVO_ERR_OUT_OF_MEMORY).  A voxelize failure of
+// the whole batch retries each request alone, so that one bad scene cannot fail the others.
 void svo_server::build_batch_requests(std::vector<Request *> &batch) {
 	const uint32_t n = (uint32_t)batch.size();
 	std::vector<const svo_scene *> scenes(n);
@@ -236,63 +240,88 @@ void svo_server::build_batch_requests(std::vector<Request *> &batch) {
 		levels[i] = batch[i]->level;
 	}
 	std::vector<svo_fraglist *> fls(n, nullptr);
-	std::vector<svo_octree *> octs(n, nullptr);
 	int rc = svo_voxelize_batch(cb, scenes.data(), levels.data(), n, fls.data());
-	if (!rc) {
-		// transient build scratch: node + bucket (16 B per fragment) plus bitmaps and ranks
-		uint64_t scratch = 0;
-		for (uint32_t i = 0; i < n; ++i) {
-			uint64_t F = 0;
-			svo_fraglist_info(fls[i], nullptr, &F, nullptr);
-			scratch += 20 * F + 8 * svo::build_bitmap_words(levels[i]);
+	if (rc) {
+		for (uint32_t i = 0; i < n; ++i)
+			if (fls[i])
+				svo_fraglist_destroy(fls[i]);
+		if (n > 1) {
+			for (uint32_t i = 0; i < n; ++i) {
+				std::vector<Request *> one{batch[i]};
+				build_batch_requests(one);
+			}
+			return;
 		}
 		std::unique_lock<std::mutex> lk(mu);
-		if (!make_room(scratch, false))
-			rc = SVO_ERR_OUT_OF_MEMORY;
-	}
-	if (!rc)
-		rc = svo_build_batch(cb, (const svo_fraglist *const *)fls.data(), n, octs.data());
-	if (!rc && hipStreamSynchronize(cb->stream) != hipSuccess)
-		rc = SVO_ERR_HIP;
-	for (uint32_t i = 0; i < n; ++i)
-		if (fls[i])
-			svo_fraglist_destroy(fls[i]);
-	if (rc && n > 1) {
-		for (uint32_t i = 0; i < n; ++i)
-			if (octs[i])
-				svo_octree_destroy(octs[i]);
-		for (uint32_t i = 0; i < n; ++i) {
-			std::vector<Request *> one{batch[i]};
-			build_batch_requests(one);
-		}
+		finish(batch[0], rc, Clock::now());
 		return;
 	}
-	const Clock::time_point t1 = Clock::now();
-	std::unique_lock<std::mutex> lk(mu);
+	std::vector<uint64_t> scratch(n);
 	for (uint32_t i = 0; i < n; ++i) {
-		Request *r = batch[i];
-		if (rc) {
-			if (octs[i])
-				svo_octree_destroy(octs[i]);
-			finish(r, rc, t1);
+		uint64_t F = 0;
+		svo_fraglist_info(fls[i], nullptr, &F, nullptr);
+		scratch[i] = 20 * F + 8 * svo::build_bitmap_words(levels[i]);
+	}
+	uint32_t single_until = 0; // after a failed group, its scenes are built one at a time
+	for (uint32_t i = 0; i < n;) {
+		// the largest group [i, j) whose scratch fits
+		uint32_t j = i < single_until ? i + 1 : n;
+		bool fits = false;
+		for (;;) {
+			uint64_t need = 0;
+			for (uint32_t k = i; k < j; ++k)
+				need += scratch[k];
+			{
+				std::unique_lock<std::mutex> lk(mu);
+				fits = make_room(need, false);
+			}
+			if (fits || j - i == 1)
+				break;
+			j = i + (j - i) / 2;
+		}
+		std::vector<svo_octree *> octs(j - i, nullptr);
+		int grc = fits ? svo_build_batch(cb, (const svo_fraglist *const *)fls.data() + i, j - i, octs.data())
+		               : SVO_ERR_OUT_OF_MEMORY;
+		if (!grc && hipStreamSynchronize(cb->stream) != hipSuccess)
+			grc = SVO_ERR_HIP;
+		if (grc && j - i > 1) { // retry the group's scenes alone: one bad scene must not fail the others
+			for (svo_octree *o : octs)
+				if (o)
+					svo_octree_destroy(o);
+			single_until = j;
 			continue;
 		}
-		svo_build_finish(cb, octs[i], nullptr);
-		uint64_t W = 0;
-		svo_octree_info(octs[i], nullptr, &W, nullptr);
-		const uint64_t bytes = 4 * W;
-		if (!make_room(bytes, true)) {
-			svo_octree_destroy(octs[i]);
-			finish(r, SVO_ERR_OUT_OF_MEMORY, t1); // rejected by admission
-			continue;
+		const Clock::time_point t1 = Clock::now();
+		std::unique_lock<std::mutex> lk(mu);
+		for (uint32_t k = i; k < j; ++k) {
+			Request *r = batch[k];
+			svo_octree *o = octs[k - i];
+			svo_fraglist_destroy(fls[k]);
+			fls[k] = nullptr;
+			if (grc) {
+				if (o)
+					svo_octree_destroy(o);
+				finish(r, grc, t1);
+				continue;
+			}
+			svo_build_finish(cb, o, nullptr);
+			uint64_t W = 0;
+			svo_octree_info(o, nullptr, &W, nullptr);
+			const uint64_t bytes = 4 * W;
+			if (!make_room(bytes, true)) {
+				svo_octree_destroy(o);
+				finish(r, SVO_ERR_OUT_OF_MEMORY, t1); // rejected by admission
+				continue;
+			}
+			Resident e;
+			e.oct = o;
+			e.bytes = bytes;
+			e.last_use = ++use_clock;
+			res[r->id] = e;
+			resident_bytes += bytes;
+			finish(r, SVO_OK, t1);
 		}
-		Resident e;
-		e.oct = octs[i];
-		e.bytes = bytes;
-		e.last_use = ++use_clock;
-		res[r->id] = e;
-		resident_bytes += bytes;
-		finish(r, SVO_OK, t1);
+		i = j;
 	}
 }
 
