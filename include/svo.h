@@ -130,6 +130,11 @@ int svo_scene_create(svo_ctx *ctx, const float *tri_xyz, const uint32_t *tri_rgb
                      svo_scene **out);
 int svo_scene_create_device(svo_ctx *ctx, const float *tri_xyz_dev, const uint32_t *tri_rgb_dev, uint64_t T,
                             int normalize, svo_scene **out);
+/* New triangles for an existing scene, the same count T (new; the reference loads a scene once):
+ * the steady-state loop of svo_build_plan_run rebuilds from them.  Same arguments as create. */
+int svo_scene_update(svo_ctx *ctx, svo_scene *scene, const float *tri_xyz, const uint32_t *tri_rgb, int normalize);
+int svo_scene_update_device(svo_ctx *ctx, svo_scene *scene, const float *tri_xyz_dev, const uint32_t *tri_rgb_dev,
+                            int normalize);
 int svo_scene_info(const svo_scene *scene, uint64_t *T, const float **tri_xyz_dev, const uint32_t **tri_rgb_dev);
 int svo_scene_download(svo_ctx *ctx, const svo_scene *scene, float *tri_xyz);
 int svo_scene_destroy(svo_scene *scene);
@@ -169,6 +174,30 @@ int svo_build(svo_ctx *ctx, const svo_fraglist *frags, svo_octree **out);
  * (src/OctreeBuilder.cpp:142-210); this batched form is new.  n <= 1024, at most 2^32-2
  * fragments in total. */
 int svo_build_batch(svo_ctx *ctx, const svo_fraglist *const *frags, uint32_t n, svo_octree **out);
+/* Captured voxelize + build (new; the reference voxelizes and builds each scene once, reading the
+ * fragment count and the octree size back to the host in between, src/Voxelizer.cpp:134-179,
+ * src/OctreeBuilder.cpp:142-214).  A plan over n scenes at fixed levels allocates each scene's
+ * fragment list and octree at a capacity: frag_caps[i] fragments and word_caps[i] words (either
+ * array may be NULL: then one plain voxelize + build of the scenes as they are now sizes it, plus
+ * a quarter).  Every count stays on the device, so no pass waits for the host, and the passes are
+ * captured once as a HIP graph.  svo_build_plan_run replays them on the context's stream; the
+ * scenes' triangles may change between runs (svo_scene_update), their number may not.  A run whose
+ * counts pass a capacity writes nothing past it.  svo_build_plan_finish waits for the last run,
+ * fills F[i] and W[i] (either may be NULL; the counts the scenes needed) and run_ms (may be NULL;
+ * device time of the run), and returns SVO_ERR_OVERFLOW when a count passed its capacity: the
+ * lists and octrees then hold no valid data until a run fits.  After a run that fits they are
+ * identical to svo_voxelize_batch + svo_build_batch of the same scenes.  They belong to the plan
+ * (svo_build_plan_get); march them after svo_build_plan_finish.  A march plan reads an octree's
+ * size when it is created.  use_graph(plan, 0) enqueues the same passes as plain launches.
+ * n <= 1024; the scenes must outlive the plan. */
+typedef struct svo_build_plan svo_build_plan;
+int svo_build_plan_create(svo_ctx *ctx, const svo_scene *const *scenes, const uint32_t *levels, uint32_t n,
+                          const uint64_t *frag_caps, const uint64_t *word_caps, svo_build_plan **out);
+int svo_build_plan_use_graph(svo_build_plan *plan, int on);
+int svo_build_plan_run(svo_build_plan *plan);
+int svo_build_plan_finish(svo_build_plan *plan, uint64_t *F, uint64_t *W, double *run_ms);
+int svo_build_plan_get(const svo_build_plan *plan, uint32_t i, const svo_fraglist **frags, const svo_octree **oct);
+int svo_build_plan_destroy(svo_build_plan *plan);
 /* An octree from host words (any node order), e.g. one read back from the reference. */
 int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t level, svo_octree **out);
 /* Octree::GetLevel / GetRange / GetBuffer (src/Octree.hpp:18-33). */

@@ -7,6 +7,7 @@ package fails if the library has not been built: there is no CPU fallback.
 """
 from .api import (  # noqa: F401
     HIT_DTYPE,
+    BuildPlan,
     Camera,
     Context,
     DeviceArray,

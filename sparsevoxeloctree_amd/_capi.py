@@ -124,6 +124,8 @@ _SIGS = {
     "svo_last_timing": (i32, [P, C.POINTER(svo_timing)]),
     "svo_scene_create": (i32, [P, P, P, u64, i32, PP]),
     "svo_scene_create_device": (i32, [P, P, P, u64, i32, PP]),
+    "svo_scene_update": (i32, [P, P, P, P, i32]),
+    "svo_scene_update_device": (i32, [P, P, P, P, i32]),
     "svo_scene_info": (i32, [P, C.POINTER(u64), PP, PP]),
     "svo_scene_download": (i32, [P, P, P]),
     "svo_scene_destroy": (i32, [P]),
@@ -139,6 +141,12 @@ _SIGS = {
     "svo_build_finish": (i32, [P, P, C.POINTER(u64)]),
     "svo_build": (i32, [P, P, PP]),
     "svo_build_batch": (i32, [P, PP, u32, PP]),
+    "svo_build_plan_create": (i32, [P, PP, C.POINTER(u32), u32, C.POINTER(u64), C.POINTER(u64), PP]),
+    "svo_build_plan_use_graph": (i32, [P, i32]),
+    "svo_build_plan_run": (i32, [P]),
+    "svo_build_plan_finish": (i32, [P, C.POINTER(u64), C.POINTER(u64), C.POINTER(C.c_double)]),
+    "svo_build_plan_get": (i32, [P, u32, PP, PP]),
+    "svo_build_plan_destroy": (i32, [P]),
     "svo_octree_create": (i32, [P, P, u64, u32, PP]),
     "svo_octree_info": (i32, [P, C.POINTER(u32), C.POINTER(u64), PP]),
     "svo_octree_download": (i32, [P, P, P, u64]),

@@ -42,6 +42,7 @@ __all__ = [
     "camera_look_at",
     "voxelize_batch",
     "build_batch",
+    "BuildPlan",
     "MarchPlan",
     "march_batch",
     "march_rays_batch",
@@ -172,6 +173,14 @@ class Scene:
         ctx.check(lib.svo_scene_create(ctx.h, ptr(xyz), ptr(rgb), T, 1 if normalize else 0, C.byref(h)), "scene")
         return Scene(ctx, h, T)
 
+    def update(self, xyz, rgb, normalize=True):
+        """svo_scene_update: new triangles, the same count (the input of a BuildPlan's next run)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1)
+        rgb = np.ascontiguousarray(rgb, dtype=np.uint32).reshape(-1)
+        assert rgb.shape[0] == self.T and xyz.shape[0] == 9 * self.T
+        self.ctx.check(lib.svo_scene_update(self.ctx.h, self.h, ptr(xyz), ptr(rgb), 1 if normalize else 0),
+                       "scene update")
+
     def GetTriangleCount(self):
         return self.T
 
@@ -235,7 +244,7 @@ class Voxelizer:
         return p
 
     def __del__(self):
-        if getattr(self, "h", None) and self.ctx.h:
+        if getattr(self, "h", None) and self.ctx.h and not getattr(self, "_borrowed", False):
             lib.svo_fraglist_destroy(self.h)
             self.h = None
 
@@ -631,6 +640,69 @@ def build_batch(voxelizers, wait=True):
         o._own = True
         octs.append(o)
     return octs
+
+
+class BuildPlan:
+    """svo_build_plan_*: voxelize + build of a fixed set of scenes at fixed levels, with every count
+    kept on the device, captured once as a HIP graph and replayed by run() (a steady-state loop:
+    Scene.update, run, finish).  Capacities: frag_caps / word_caps per scene, or None to size them
+    from one plain build of the scenes as they are now, plus a quarter."""
+
+    def __init__(self, scenes, levels, frag_caps=None, word_caps=None):
+        self.ctx = scenes[0].ctx
+        self.n = len(scenes)
+        self.levels = [int(x) for x in levels]
+        lv = (C.c_uint32 * self.n)(*self.levels)
+        fc = (C.c_uint64 * self.n)(*[int(x) for x in frag_caps]) if frag_caps is not None else None
+        wc = (C.c_uint64 * self.n)(*[int(x) for x in word_caps]) if word_caps is not None else None
+        h = C.c_void_p()
+        self.ctx.check(lib.svo_build_plan_create(self.ctx.h, _handles(scenes), lv, self.n, fc, wc, C.byref(h)),
+                       "build plan")
+        self.h = h
+        self._scenes = list(scenes)
+        self.F = [0] * self.n
+        self.W = [0] * self.n
+
+    def use_graph(self, on=True):
+        self.ctx.check(lib.svo_build_plan_use_graph(self.h, 1 if on else 0), "build plan graph")
+
+    def run(self):
+        self.ctx.check(lib.svo_build_plan_run(self.h), "build plan run")
+
+    def finish(self):
+        """Waits for the last run; returns its device ms.  F / W receive the counts the scenes
+        needed; raises SvoError (SVO_ERR_OVERFLOW) when one passed its capacity."""
+        F = (C.c_uint64 * self.n)()
+        W = (C.c_uint64 * self.n)()
+        ms = C.c_double()
+        rc = lib.svo_build_plan_finish(self.h, F, W, C.byref(ms))
+        self.F, self.W = list(F), list(W)
+        self.ctx.check(rc, "build plan finish")
+        return ms.value
+
+    def fragments(self, i):
+        """Scene i's fragment list (owned by the plan)."""
+        fl = C.c_void_p()
+        self.ctx.check(lib.svo_build_plan_get(self.h, i, C.byref(fl), None), "build plan get")
+        v = Voxelizer(self.ctx, fl, self.levels[i], self._scenes[i])
+        v._borrowed = True
+        v._plan = self
+        return v
+
+    def octree(self, i):
+        """Scene i's octree (owned by the plan), at the size of the last finished run."""
+        o = C.c_void_p()
+        self.ctx.check(lib.svo_build_plan_get(self.h, i, None, C.byref(o)), "build plan get")
+        lvl, W = C.c_uint32(), C.c_uint64()
+        lib.svo_octree_info(o, C.byref(lvl), C.byref(W), None)
+        t = Octree(self.ctx, o, lvl.value, W.value)
+        t._plan = self
+        return t
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            lib.svo_build_plan_destroy(self.h)
+            self.h = None
 
 
 class MarchPlan:

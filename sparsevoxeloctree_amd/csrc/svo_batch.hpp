@@ -122,6 +122,7 @@ struct VoxBatch {
 	std::vector<uint64_t> h_tbeg;
 	VoxSeg *segs = nullptr;     // device table
 	uint64_t *tbeg = nullptr;   // S + 1 triangle begins
+	uint64_t *fend = nullptr;   // the fill's {fragment total, limit}, after tbeg
 	uint64_t *offsets = nullptr; // Ttot exclusive fragment offsets (global)
 	uint32_t *big = nullptr;    // big-triangle list: count, then global triangle indices
 };
@@ -165,6 +166,36 @@ int voxelize_fill_batch(svo_ctx *ctx, VoxBatch *vb);
 // Build: structure phase (one read-back of every scene's W), octree allocation, node phase.
 int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S);
 
+// Voxelize passes of a captured plan: tables over lists the plan allocated, then the count pass
+// (fend[0] = total) and the fill (limited by fend[1]) with no read-back between them.
+int voxelize_plan_init(svo_ctx *ctx, VoxBatch *vb, svo_fraglist *const *fls, uint32_t S);
+hipError_t voxelize_plan_count(svo_ctx *ctx, VoxBatch *vb);
+hipError_t voxelize_plan_fill(svo_ctx *ctx, VoxBatch *vb);
+
+// A voxelize + build of a fixed set of scenes whose fragment lists and octrees are allocated at
+// capacities, so no pass waits for the host: the counts stay on the device, the grids and scratch
+// are sized by the capacities, and the whole sequence is captured once as a HIP graph and
+// replayed.  A count above its capacity empties the rest of the run and is reported by finish.
+struct BuildPlan {
+	uint32_t S = 0;
+	VoxBatch vb;
+	BuildBatch bb; // h_segs[s].F holds the fragment capacities
+	std::vector<svo_fraglist *> fls;
+	std::vector<svo_octree *> octs; // cap = word capacity
+	uint64_t *caps = nullptr;       // device: fragment capacities [S], word capacities [S]
+	uint64_t *out = nullptr;        // device: status (bit 1 fragments, bit 2 words), F[S], W[S]
+	std::vector<uint64_t> h_out;
+	uint64_t pw_value = 0; // the block writer's parent-word total (fixed by the levels)
+	bool use_graph = true, pending = false;
+	hipGraphExec_t exec = nullptr;
+	StageTimer t_run;
+};
+int build_plan_init(svo_ctx *ctx, BuildPlan *p, const svo_scene *const *scenes, const uint32_t *levels, uint32_t S,
+                    const uint64_t *fcap, const uint64_t *wcap);
+int build_plan_run(svo_ctx *ctx, BuildPlan *p);
+int build_plan_finish(svo_ctx *ctx, BuildPlan *p, uint64_t *F, uint64_t *W, double *ms);
+void build_plan_release(svo_ctx *ctx, BuildPlan *p);
+
 // A reusable batch march of S octrees, one camera each, width x height primary rays each: device
 // table of octrees and cameras, beam scratch, the launch shape (default or autotuned), and
 // optionally a captured HIP graph of the launches, replayed while the hit buffer stays the same.
@@ -196,4 +227,9 @@ int march_rays_segments(svo_ctx *ctx, const svo_octree *const *octs, const float
 struct svo_march_plan {
 	svo_ctx *ctx;
 	svo::MarchPlan p;
+};
+
+struct svo_build_plan {
+	svo_ctx *ctx;
+	svo::BuildPlan p;
 };

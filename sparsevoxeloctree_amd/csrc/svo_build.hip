@@ -166,8 +166,8 @@ __device__ __forceinline__ uint32_t hash_slot(uint64_t key) {
 // each distinct word costs one load and, only when that load does not already show all of its
 // bits, one atomic.
 __global__ void __launch_bounds__(kThreads)
-    bm_mark(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
-            uint32_t *B) {
+    bm_mark(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint32_t *B) {
+	const uint64_t Ftot = fbeg[S];
 	static_assert(kHash == 2048u && kHash >= 2u * kU * kThreads, "hash_slot returns 11 bits");
 	__shared__ unsigned long long s_word[kHash];
 	__shared__ uint32_t s_bits[kHash];
@@ -399,8 +399,9 @@ __device__ __forceinline__ uint32_t nth_bit(uint32_t v, uint32_t n) {
 // counters of build_tag_leaf and the three counts of the fold's work lists.
 __global__ void __launch_bounds__(kThreads)
     bm_write_blocks(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ pw, uint32_t M, uint32_t S,
-                    uint64_t n, const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint32_t *cnt,
-                    uint64_t nb, uint32_t *list) {
+                    const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint32_t *cnt,
+                    const uint64_t *__restrict__ nbp, uint32_t *list) {
+	const uint64_t n = pw[M], nb = *nbp;
 	const uint32_t lane = threadIdx.x & 63u;
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	for (uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x; i < nb; i += stride)
@@ -520,9 +521,10 @@ constexpr uint32_t kLineBins = 16; // 4-byte counters per 64-byte line
 
 template <uint32_t kLines>
 __global__ void __launch_bounds__(kThreads)
-    build_tag_leaf(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+    build_tag_leaf(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S,
                    const uint32_t *__restrict__ B, const uint32_t *__restrict__ R, uint2 *__restrict__ node,
                    uint32_t *cnt) {
+	const uint64_t Ftot = fbeg[S];
 	__shared__ uint32_t line[kLines];
 	__shared__ uint32_t n[kLines * kLineBins];
 	for (uint32_t i = threadIdx.x; i < kLines; i += kThreads)
@@ -614,8 +616,9 @@ __global__ void __launch_bounds__(kThreads)
 
 // Scatter every fragment to its place in its bin's bucket (aux[j] = first slot of bin j).
 __global__ void __launch_bounds__(kThreads)
-    build_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S, uint64_t Ftot,
+    build_scatter(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ fbeg, uint32_t S,
                   const uint2 *__restrict__ node, const uint32_t *__restrict__ aux, uint2 *__restrict__ bucket) {
+	const uint64_t Ftot = fbeg[S];
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads * kU;
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads * kU; base < Ftot; base += stride) {
@@ -773,9 +776,10 @@ __device__ __forceinline__ void fold_small(const uint2 *__restrict__ b, uint32_t
 }
 
 __global__ void __launch_bounds__(kThreads)
-    build_fold(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S, uint64_t nbins,
+    build_fold(const BuildSeg *__restrict__ segs, const uint64_t *__restrict__ bbeg, uint32_t S,
                const uint32_t *__restrict__ cnt, const uint32_t *__restrict__ aux, const uint2 *__restrict__ bucket,
                uint32_t *list, uint64_t cap) {
+	const uint64_t nbins = bbeg[S];
 	constexpr uint32_t kWaves = kThreads / 64;
 	__shared__ uint32_t s_magic[65];
 	__shared__ uint32_t s_bin[kThreads], s_n[kThreads];
@@ -1129,8 +1133,8 @@ __global__ void __launch_bounds__(kThreads)
 struct BinCounts {
 	static constexpr int kItems = 4;
 	const uint32_t *cnt;
-	uint64_t n;
-	__device__ uint64_t size() const { return n; }
+	const uint64_t *n; // the batch's leaf blocks, in device memory (bbeg[S])
+	__device__ uint64_t size() const { return *n; }
 	__device__ uint32_t operator()(uint64_t i) const { return cnt[i]; }
 };
 struct BinOffsets {
@@ -1177,7 +1181,7 @@ int structure_phase(svo_ctx *ctx, BuildBatch &bb) {
 			return hip_check(ctx, e, "bm_zero");
 		if (bb.Ftot > 0 &&
 		    (e = launch(ctx, "bm_mark", bm_mark, dim3(grid_tiles(bb.Ftot, ctx)), dim3(kThreads), 0,
-		                (const BuildSeg *)bb.segs, bb.fbeg(), S, bb.Ftot, bb.bm)) != hipSuccess)
+		                (const BuildSeg *)bb.segs, bb.fbeg(), S, bb.bm)) != hipSuccess)
 			return hip_check(ctx, e, "bm_mark");
 		for (uint32_t d = bb.maxL >= 3 ? bb.maxL - 2 : 0; d > kTopDepth; --d) {
 			const uint64_t n = bb.h_index[(3 + kMaxLevel + d) * (uint64_t)(S + 1) + S];
@@ -1214,18 +1218,18 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 		const uint32_t M = bb.pw_segs();
 		const uint64_t n = bb.maxL > 1u ? bb.h_index[bb.pw_at() + M] : 0u; // parent words of all depths >= 2
 		if ((e = launch(ctx, "bm_write_blocks", bm_write_blocks, dim3(grid_for(std::max(n, bb.nbins), ctx)),
-		                dim3(kThreads), 0, segs, bb.pw(), M, bb.maxL > 1u ? S : 0u, n, (const uint32_t *)bb.bm,
-		                (const uint32_t *)bb.rk, bb.cnt, bb.nbins, bb.list)) != hipSuccess)
+		                dim3(kThreads), 0, segs, bb.pw(), M, bb.maxL > 1u ? S : 0u, (const uint32_t *)bb.bm,
+		                (const uint32_t *)bb.rk, bb.cnt, bb.bbeg() + S, bb.list)) != hipSuccess)
 			return hip_check(ctx, e, "bm_write_blocks");
 	}
 	const uint32_t gF = grid_tiles(bb.Ftot, ctx);
 	uint2 *node = reinterpret_cast<uint2 *>(bb.node);
 	if (bb.Ftot > 0 &&
 	    (e = launch(ctx, "build_tag_leaf", bb.maxL >= 12u ? build_tag_leaf<256> : build_tag_leaf<128>, dim3(gF),
-	                dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk,
+	                dim3(kThreads), 0, segs, bb.fbeg(), S, (const uint32_t *)bb.bm, (const uint32_t *)bb.rk,
 	                node, bb.cnt)) != hipSuccess)
 		return hip_check(ctx, e, "build_tag_leaf");
-	BinCounts bc{bb.cnt, bb.nbins};
+	BinCounts bc{bb.cnt, bb.bbeg() + S};
 	if ((e = launch(ctx, "bin_reduce", scan_reduce<BinCounts>, dim3(kScanBlocks), dim3(kScanThreads), 0, bc,
 	                ctx->d_scan_parts)) != hipSuccess)
 		return hip_check(ctx, e, "bin_reduce");
@@ -1234,13 +1238,13 @@ int node_phase(svo_ctx *ctx, BuildBatch &bb) {
 		return hip_check(ctx, e, "bin_offsets");
 	uint2 *bucket = reinterpret_cast<uint2 *>(bb.bucket);
 	if (bb.Ftot > 0 && (e = launch(ctx, "build_scatter", build_scatter, dim3(grid_tiles(bb.Ftot, ctx, 256u)),
-	                               dim3(kThreads), 0, segs, bb.fbeg(), S, bb.Ftot, (const uint2 *)node,
+	                               dim3(kThreads), 0, segs, bb.fbeg(), S, (const uint2 *)node,
 	                               (const uint32_t *)bb.aux, bucket)) != hipSuccess)
 		return hip_check(ctx, e, "build_scatter");
 	const uint64_t cap = big_capacity(bb.Ftot);
 	if ((e = launch(ctx, "build_fold", build_fold, dim3(grid_for(bb.nbins, ctx)), dim3(kThreads), 0, segs, bb.bbeg(),
-	                S, bb.nbins, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket, bb.list,
-	                cap)) != hipSuccess)
+	                S, (const uint32_t *)bb.cnt, (const uint32_t *)bb.aux, (const uint2 *)bucket, bb.list, cap)) !=
+	    hipSuccess)
 		return hip_check(ctx, e, "build_fold");
 	const uint32_t g_mid = (uint32_t)ctx->cu_count * 8u, g_wide = (uint32_t)ctx->cu_count * 8u,
 	               g_big = (uint32_t)ctx->cu_count * 2u;
@@ -1268,26 +1272,21 @@ void build_batch_release(svo_ctx *ctx, BuildBatch *bb) {
 	bb->bm = bb->rk = bb->cnt = bb->aux = bb->node = bb->bucket = bb->list = nullptr;
 }
 
-// The whole build of S octrees: structure phase, one read-back of every scene's W and leaf-block
-// count, allocation of the octrees, node phase.  Scratch goes back to the pool at the end; the pool
-// hands it only to later work on this stream, so no wait is needed for the node phase to finish.
-int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
-	if (S == 0)
-		return SVO_OK;
-	if (S > kMaxBatch)
-		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 scenes");
-	BuildBatch bb;
+namespace {
+
+// Host layout of a build batch of S scenes with F[s] fragments at level L[s]: descriptors (bitmap
+// offsets), fragment begins, per-depth bitmap word tables and the block writer's parent words.
+void batch_layout(BuildBatch &bb, uint32_t S, const uint2 *const *frags, const uint64_t *F, const uint32_t *L) {
 	bb.S = S;
 	bb.h_segs.assign(S, BuildSeg{});
 	const uint64_t stride = S + 1;
 	bb.h_index.assign(bb.pw_at() + bb.pw_segs() + 1, 0);
 	uint64_t *fbeg = bb.h_index.data();
 	for (uint32_t s = 0; s < S; ++s) {
-		const svo_fraglist *fl = octs[s]->src;
 		BuildSeg &g = bb.h_segs[s];
-		g.frags = reinterpret_cast<const uint2 *>(fl->frags);
-		g.F = fl->F;
-		g.L = fl->level;
+		g.frags = frags[s];
+		g.F = F[s];
+		g.L = L[s];
 		g.bm = bb.TW;
 		levels_for(g.L, g.off);
 		bb.TW += g.off[g.L];
@@ -1296,8 +1295,6 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 		bb.maxL = std::max(bb.maxL, g.L);
 	}
 	fbeg[S] = bb.Ftot;
-	if (bb.Ftot > 0xFFFFFFFEull)
-		return set_error(ctx, SVO_ERR_OVERFLOW, "a build batch holds at most 2^32-2 fragments; split the batch");
 	// per-depth begin tables: dw[d] over scenes with B[d] (L > d), dr[d] over scenes with B[d+1] (L > d+1)
 	for (uint32_t d = 1; d <= kMaxLevel; ++d) {
 		uint64_t *dw = bb.h_index.data() + (2 + d) * stride, *dr = bb.h_index.data() + (3 + kMaxLevel + d) * stride;
@@ -1314,29 +1311,58 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 		dr[S] = ar;
 	}
 	// parent words of the block writer: B[d-1] of the scenes with L > d, for d = 2..kMaxLevel-1
+	uint64_t *pw = bb.h_index.data() + bb.pw_at(), a = 0;
+	for (uint32_t d = 2; d < kMaxLevel; ++d)
+		for (uint32_t s = 0; s < S; ++s) {
+			const BuildSeg &g = bb.h_segs[s];
+			*pw++ = a;
+			a += g.L > d ? g.off[d] - g.off[d - 1] : 0;
+		}
+	*pw = a;
+}
+
+// Descriptors and begin tables in one device allocation, so each upload is one copy.
+size_t seg_table_bytes(uint32_t S) { return (sizeof(BuildSeg) * S + 255) & ~size_t(255); }
+
+int upload_batch_tables(svo_ctx *ctx, BuildBatch &bb) {
+	const size_t seg_bytes = seg_table_bytes(bb.S), idx_bytes = sizeof(uint64_t) * bb.h_index.size();
+	std::vector<uint8_t> tables(seg_bytes + idx_bytes);
+	std::memcpy(tables.data(), bb.h_segs.data(), sizeof(BuildSeg) * bb.S);
+	std::memcpy(tables.data() + seg_bytes, bb.h_index.data(), idx_bytes);
+	return upload_table(ctx, bb.segs, tables.data(), tables.size());
+}
+
+} // namespace
+
+// The whole build of S octrees: structure phase, one read-back of every scene's W and leaf-block
+// count, allocation of the octrees, node phase.  Scratch goes back to the pool at the end; the pool
+// hands it only to later work on this stream, so no wait is needed for the node phase to finish.
+int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
+	if (S == 0)
+		return SVO_OK;
+	if (S > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "batch larger than 1024 scenes");
+	BuildBatch bb;
+	const uint64_t stride = S + 1;
 	{
-		uint64_t *pw = bb.h_index.data() + bb.pw_at(), a = 0;
-		for (uint32_t d = 2; d < kMaxLevel; ++d)
-			for (uint32_t s = 0; s < S; ++s) {
-				const BuildSeg &g = bb.h_segs[s];
-				*pw++ = a;
-				a += g.L > d ? g.off[d] - g.off[d - 1] : 0;
-			}
-		*pw = a;
+		std::vector<const uint2 *> frags(S);
+		std::vector<uint64_t> F(S);
+		std::vector<uint32_t> L(S);
+		for (uint32_t s = 0; s < S; ++s) {
+			const svo_fraglist *fl = octs[s]->src;
+			frags[s] = reinterpret_cast<const uint2 *>(fl->frags);
+			F[s] = fl->F;
+			L[s] = fl->level;
+		}
+		batch_layout(bb, S, frags.data(), F.data(), L.data());
 	}
+	if (bb.Ftot > 0xFFFFFFFEull)
+		return set_error(ctx, SVO_ERR_OVERFLOW, "a build batch holds at most 2^32-2 fragments; split the batch");
 	const uint64_t Fa = bb.Ftot ? bb.Ftot : 1;
 	const uint64_t cap = big_capacity(bb.Ftot);
 	Pool *pool = ctx->pool;
 	int rc = SVO_OK;
-	// the descriptors and the begin tables share one allocation, so each upload is one copy
-	const size_t seg_bytes = (sizeof(BuildSeg) * S + 255) & ~size_t(255);
-	const size_t idx_bytes = sizeof(uint64_t) * bb.h_index.size();
-	std::vector<uint8_t> tables(seg_bytes + idx_bytes);
-	auto upload_tables = [&]() {
-		std::memcpy(tables.data(), bb.h_segs.data(), sizeof(BuildSeg) * S);
-		std::memcpy(tables.data() + seg_bytes, bb.h_index.data(), idx_bytes);
-		return upload_table(ctx, bb.segs, tables.data(), tables.size());
-	};
+	const size_t seg_bytes = seg_table_bytes(S), idx_bytes = sizeof(uint64_t) * bb.h_index.size();
 	if (pool->alloc((void **)&bb.segs, seg_bytes + idx_bytes) != hipSuccess ||
 	    pool->alloc((void **)&bb.bm, sizeof(uint32_t) * (bb.TW ? bb.TW : 1)) != hipSuccess ||
 	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW / 8u + 1)) != hipSuccess ||
@@ -1346,7 +1372,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	if (!rc) {
 		bb.index = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(bb.segs) + seg_bytes);
-		rc = upload_tables();
+		rc = upload_batch_tables(ctx, bb);
 	}
 	// structure phase: bitmaps + ranks decide every W exactly; one read-back sizes the octrees
 	if (!rc) {
@@ -1384,7 +1410,7 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	            pool->alloc((void **)&bb.aux, sizeof(uint32_t) * (bb.nbins ? bb.nbins : 1)) != hipSuccess))
 		rc = set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "octree build allocation failed");
 	if (!rc)
-		rc = upload_tables();
+		rc = upload_batch_tables(ctx, bb);
 	if (!rc) {
 		ctx->t_build2.start(ctx->stream);
 		rc = node_phase(ctx, bb);
@@ -1392,6 +1418,283 @@ int build_batch(svo_ctx *ctx, svo_octree *const *octs, uint32_t S) {
 	}
 	build_batch_release(ctx, &bb);
 	return rc;
+}
+
+// ---- captured plans: voxelize + build of a fixed set of scenes with the counts kept on the device
+namespace {
+
+constexpr uint32_t kPlanThreads = kMaxBatch; // the check kernels take one scene per thread
+
+// Between the voxelize count and fill passes: the scenes' fragment counts become the build's
+// fragment begins.  A count above its list's capacity sets status bit 1 and empties the batch:
+// fill limit, every fragment begin and the fragment total 0, so no later pass reads or writes a
+// list past its end.  Also restores the block writer's parent-word total, which the node check
+// of the previous run may have cleared.  out = {status, F[S], W[S]}.
+__global__ void __launch_bounds__(kPlanThreads)
+    plan_check_frags(const VoxSeg *__restrict__ vsegs, BuildSeg *bsegs, uint64_t *fbeg, uint32_t S,
+                     const uint64_t *__restrict__ fcap, uint64_t *fend, uint64_t *pw_total, uint64_t pw_value,
+                     uint64_t *out) {
+	const uint32_t s = threadIdx.x;
+	uint64_t F = 0, b = 0;
+	int over = 0;
+	if (s < S) {
+		F = vsegs[s].F;
+		b = vsegs[s].frag_begin;
+		over = F > fcap[s];
+		out[1u + s] = F;
+		out[1u + S + s] = 0u;
+		bsegs[s].F = F;
+	}
+	over = __syncthreads_or(over);
+	if (s < S)
+		fbeg[s] = over ? 0u : b;
+	if (s == 0u) {
+		const uint64_t total = over ? 0u : fend[0];
+		fend[1] = total;
+		fbeg[S] = total;
+		*pw_total = pw_value;
+		out[0] = over ? 1u : 0u;
+	}
+}
+
+// Between the structure and node phases: leaf-block begins (an exclusive scan of the scenes' leaf
+// blocks) and W against each octree's capacity.  A W above its capacity (or above 2^30 words), or
+// a fragment overflow found before, sets status bit 2 and empties the node phase: leaf blocks,
+// fragments and parent words 0, so it writes nothing but the root blocks.
+__global__ void __launch_bounds__(kPlanThreads)
+    plan_check_nodes(BuildSeg *segs, uint64_t *bbeg, uint32_t S, const uint64_t *__restrict__ wcap, uint64_t *fbeg,
+                     uint64_t *pw_total, uint64_t *out) {
+	__shared__ uint64_t s_wave[kPlanThreads / 64];
+	const uint32_t s = threadIdx.x, lane = s & 63u, wave = s >> 6;
+	uint64_t nb = 0;
+	int over = 0;
+	if (s < S) {
+		const BuildSeg &g = segs[s];
+		nb = g.nbins;
+		over = g.overflow || g.W > wcap[s];
+		out[1u + S + s] = g.W;
+	}
+	over = __syncthreads_or(over) || out[0] != 0u;
+	uint64_t incl = nb;
+#pragma unroll
+	for (uint32_t d = 1; d < 64u; d <<= 1) {
+		const uint64_t t = __shfl_up(incl, d, 64);
+		incl += lane >= d ? t : 0u;
+	}
+	if (lane == 63u)
+		s_wave[wave] = incl;
+	__syncthreads();
+	uint64_t below = 0, total = 0;
+	for (uint32_t w = 0; w < kPlanThreads / 64; ++w) {
+		below += w < wave ? s_wave[w] : 0u;
+		total += s_wave[w];
+	}
+	const uint64_t excl = below + incl - nb;
+	if (s < S) {
+		segs[s].bin_begin = over ? 0u : excl;
+		bbeg[s] = over ? 0u : excl;
+	}
+	if (s == 0u) {
+		bbeg[S] = over ? 0u : total;
+		if (over) {
+			fbeg[S] = 0u;
+			*pw_total = 0u;
+			out[0] |= 2u;
+		}
+	}
+}
+
+// The plan's passes in stream order; nothing waits on the host, so the sequence can be captured.
+int plan_enqueue(svo_ctx *ctx, BuildPlan &p) {
+	BuildBatch &bb = p.bb;
+	const uint32_t S = p.S;
+	uint64_t *pw_total = bb.index + bb.pw_at() + bb.pw_segs();
+	hipError_t e;
+	if ((e = voxelize_plan_count(ctx, &p.vb)) != hipSuccess)
+		return hip_check(ctx, e, "plan voxelize count");
+	if ((e = launch(ctx, "plan_check_frags", plan_check_frags, dim3(1), dim3(kPlanThreads), 0,
+	                (const VoxSeg *)p.vb.segs, bb.segs, bb.index, S, (const uint64_t *)p.caps, p.vb.fend, pw_total,
+	                p.pw_value, p.out)) != hipSuccess)
+		return hip_check(ctx, e, "plan_check_frags");
+	if ((e = voxelize_plan_fill(ctx, &p.vb)) != hipSuccess)
+		return hip_check(ctx, e, "plan voxelize fill");
+	if (int rc = structure_phase(ctx, bb))
+		return rc;
+	if ((e = launch(ctx, "plan_check_nodes", plan_check_nodes, dim3(1), dim3(kPlanThreads), 0, bb.segs,
+	                bb.index + (S + 1), S, (const uint64_t *)p.caps + S, bb.index, pw_total, p.out)) != hipSuccess)
+		return hip_check(ctx, e, "plan_check_nodes");
+	return node_phase(ctx, bb);
+}
+
+} // namespace
+
+int build_plan_init(svo_ctx *ctx, BuildPlan *p, const svo_scene *const *scenes, const uint32_t *levels, uint32_t S,
+                    const uint64_t *fcap, const uint64_t *wcap) {
+	p->S = S;
+	Pool *pool = ctx->pool;
+	uint64_t Fcap = 0, Bcap = 0;
+	std::vector<uint64_t> caps(2 * (size_t)S);
+	for (uint32_t s = 0; s < S; ++s) {
+		const uint32_t L = levels[s];
+		caps[s] = fcap[s] ? fcap[s] : 1u;
+		caps[S + s] = std::max<uint64_t>((wcap[s] + 7u) & ~7ull, 8u);
+		if (caps[S + s] > (1ull << 30))
+			return set_error(ctx, SVO_ERR_OVERFLOW, "an octree capacity above 2^30 words (30-bit child pointers)");
+		Fcap += caps[s];
+		// leaf blocks: the root at level 1, else at most one per fragment and 8^(L-1)
+		Bcap += L == 1u ? 1u : std::min<uint64_t>(caps[s], 1ull << (3u * (L - 1u)));
+	}
+	if (Fcap > 0xFFFFFFFEull)
+		return set_error(ctx, SVO_ERR_OVERFLOW, "a plan holds at most 2^32-2 fragments of capacity; split it");
+	p->fls.assign(S, nullptr);
+	p->octs.assign(S, nullptr);
+	for (uint32_t s = 0; s < S; ++s) {
+		svo_fraglist *fl = p->fls[s] = new (std::nothrow) svo_fraglist();
+		svo_octree *o = p->octs[s] = new (std::nothrow) svo_octree();
+		if (!fl || !o)
+			return SVO_ERR_OUT_OF_MEMORY;
+		fl->ctx = o->ctx = ctx;
+		fl->scene = scenes[s];
+		fl->level = o->level = levels[s];
+		o->cap = caps[S + s];
+		if (pool->alloc((void **)&fl->frags, sizeof(uint32_t) * 2 * caps[s]) != hipSuccess ||
+		    pool->alloc((void **)&o->words, sizeof(uint32_t) * o->cap) != hipSuccess)
+			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "plan list or octree allocation failed");
+	}
+	if (int rc = voxelize_plan_init(ctx, &p->vb, p->fls.data(), S))
+		return rc;
+	BuildBatch &bb = p->bb;
+	{
+		std::vector<const uint2 *> frags(S);
+		for (uint32_t s = 0; s < S; ++s)
+			frags[s] = reinterpret_cast<const uint2 *>(p->fls[s]->frags);
+		batch_layout(bb, S, frags.data(), caps.data(), levels);
+	}
+	// host totals are capacities: they size the grids and scratch; the kernels read the counts
+	bb.nbins = Bcap;
+	for (uint32_t s = 0; s < S; ++s)
+		bb.h_segs[s].words = p->octs[s]->words;
+	p->pw_value = bb.h_index[bb.pw_at() + bb.pw_segs()];
+	const size_t seg_bytes = seg_table_bytes(S), idx_bytes = sizeof(uint64_t) * bb.h_index.size();
+	if (pool->alloc((void **)&bb.segs, seg_bytes + idx_bytes) != hipSuccess ||
+	    pool->alloc((void **)&bb.bm, sizeof(uint32_t) * (bb.TW ? bb.TW : 1)) != hipSuccess ||
+	    pool->alloc((void **)&bb.rk, sizeof(uint32_t) * (bb.TW / 8u + 1)) != hipSuccess ||
+	    pool->alloc((void **)&bb.node, sizeof(uint32_t) * 2 * Fcap) != hipSuccess ||
+	    pool->alloc((void **)&bb.bucket, sizeof(uint32_t) * 2 * Fcap) != hipSuccess ||
+	    pool->alloc((void **)&bb.list, sizeof(uint32_t) * (3 + 3 * big_capacity(Fcap))) != hipSuccess ||
+	    pool->alloc((void **)&bb.cnt, sizeof(uint32_t) * Bcap) != hipSuccess ||
+	    pool->alloc((void **)&bb.aux, sizeof(uint32_t) * Bcap) != hipSuccess ||
+	    pool->alloc((void **)&p->caps, sizeof(uint64_t) * 2 * S) != hipSuccess ||
+	    pool->alloc((void **)&p->out, sizeof(uint64_t) * (1 + 2 * (size_t)S)) != hipSuccess)
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "plan build allocation failed");
+	bb.index = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(bb.segs) + seg_bytes);
+	if (int rc = upload_batch_tables(ctx, bb))
+		return rc;
+	if (int rc = upload_table(ctx, p->caps, caps.data(), sizeof(uint64_t) * 2 * S))
+		return rc;
+	p->h_out.assign(1 + 2 * (size_t)S, 0);
+	return SVO_OK;
+}
+
+int build_plan_run(svo_ctx *ctx, BuildPlan *p) {
+	hipError_t e = hipSuccess;
+	int rc = SVO_OK;
+	p->t_run.start(ctx->stream);
+	if (p->use_graph) {
+		if (!p->exec) {
+			// profiler events stay out of the graph
+			const bool prof = ctx->prof.enabled;
+			ctx->prof.enabled = false;
+			hipGraph_t g = nullptr;
+			e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+			if (e == hipSuccess) {
+				rc = plan_enqueue(ctx, *p);
+				e = hipStreamEndCapture(ctx->stream, &g);
+			}
+			ctx->prof.enabled = prof;
+			if (!rc && e == hipSuccess)
+				e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+			if (g)
+				(void)hipGraphDestroy(g);
+			if (rc || e != hipSuccess) {
+				p->exec = nullptr;
+				return rc ? rc : hip_check(ctx, e, "plan capture");
+			}
+		}
+		e = hipGraphLaunch(p->exec, ctx->stream);
+	} else {
+		rc = plan_enqueue(ctx, *p);
+	}
+	p->t_run.stop(ctx->stream);
+	if (rc)
+		return rc;
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "plan launch");
+	p->pending = true;
+	return SVO_OK;
+}
+
+int build_plan_finish(svo_ctx *ctx, BuildPlan *p, uint64_t *F, uint64_t *W, double *ms) {
+	if (int rc = download_table(ctx, p->h_out.data(), p->out, sizeof(uint64_t) * p->h_out.size()))
+		return rc;
+	if (ms)
+		*ms = p->pending ? p->t_run.read() : 0.0;
+	p->pending = false;
+	const uint32_t S = p->S;
+	const uint64_t status = p->h_out[0];
+	for (uint32_t s = 0; s < S; ++s) {
+		const uint64_t f = p->h_out[1 + s], w = p->h_out[1 + S + s];
+		if (F)
+			F[s] = f;
+		if (W)
+			W[s] = w;
+		p->fls[s]->F = status & 1u ? 0u : f;
+		p->fls[s]->filled = !(status & 1u);
+		p->octs[s]->W = status ? 0u : w;
+	}
+	if (status & 1u)
+		for (uint32_t s = 0; s < S; ++s)
+			if (p->h_out[1 + s] > p->bb.h_segs[s].F)
+				return set_error(ctx, SVO_ERR_OVERFLOW,
+				                 "scene " + std::to_string(s) + " has " + std::to_string(p->h_out[1 + s]) +
+				                     " fragments, above the plan's capacity of " +
+				                     std::to_string(p->bb.h_segs[s].F));
+	if (status)
+		for (uint32_t s = 0; s < S; ++s)
+			if (p->h_out[1 + S + s] > p->octs[s]->cap)
+				return set_error(ctx, SVO_ERR_OVERFLOW,
+				                 "octree " + std::to_string(s) + " needs " + std::to_string(p->h_out[1 + S + s]) +
+				                     " words, above the plan's capacity of " + std::to_string(p->octs[s]->cap));
+	if (status)
+		return set_error(ctx, SVO_ERR_OVERFLOW, "a plan count passed its capacity");
+	return SVO_OK;
+}
+
+void build_plan_release(svo_ctx *ctx, BuildPlan *p) {
+	if (p->exec)
+		(void)hipGraphExecDestroy(p->exec);
+	p->exec = nullptr;
+	if (p->pending)
+		(void)hipStreamSynchronize(ctx->stream);
+	p->pending = false;
+	p->t_run.destroy();
+	vox_batch_release(ctx, &p->vb);
+	build_batch_release(ctx, &p->bb);
+	ctx->pool->free(p->caps);
+	ctx->pool->free(p->out);
+	p->caps = p->out = nullptr;
+	for (svo_fraglist *fl : p->fls)
+		if (fl) {
+			ctx->pool->free(fl->frags);
+			delete fl;
+		}
+	for (svo_octree *o : p->octs)
+		if (o) {
+			ctx->pool->free(o->words);
+			delete o;
+		}
+	p->fls.clear();
+	p->octs.clear();
 }
 
 } // namespace svo

@@ -49,9 +49,18 @@ uint64_t Pool::round_size(uint64_t bytes) {
 }
 
 hipError_t Pool::alloc(void **p, uint64_t bytes) {
-	const uint64_t sz = round_size(bytes);
+	uint64_t sz = round_size(bytes);
 	auto it = m_free.find(sz);
+	// a large request takes the smallest cached block of up to 1.5x its size: batches of varying
+	// composition ask for slightly different sizes, and a fresh hipMalloc / hipFree of gigabytes
+	// costs milliseconds (hipFree also waits for the device)
+	if (it == m_free.end() && sz > (64ull << 20)) {
+		it = m_free.lower_bound(sz);
+		if (it != m_free.end() && it->first > sz + sz / 2)
+			it = m_free.end();
+	}
 	if (it != m_free.end()) {
+		sz = it->first;
 		*p = it->second;
 		m_free.erase(it);
 		m_cached -= sz;
@@ -568,6 +577,35 @@ int svo_scene_create_device(svo_ctx *ctx, const float *tri_xyz_dev, const uint32
 	return SVO_OK;
 }
 
+static int scene_update(svo_ctx *ctx, svo_scene *s, const float *xyz, const uint32_t *rgb, int normalize,
+                        hipMemcpyKind kind) {
+	if (!ctx || !s || s->ctx != ctx || (s->T && (!xyz || !rgb)))
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(ctx);
+	const uint64_t T = s->T;
+	hipError_t e = hipSuccess;
+	if (T && (e = hipMemcpyAsync(s->xyz, xyz, sizeof(float) * 9 * T, kind, ctx->stream)) == hipSuccess)
+		e = hipMemcpyAsync(s->rgb, rgb, sizeof(uint32_t) * T, kind, ctx->stream);
+	if (e != hipSuccess)
+		return hip_check(ctx, e, "scene update");
+	if (normalize && T)
+		if (int rc = normalize_scene(ctx, s))
+			return rc;
+	// host arrays may be reused once the call returns
+	if (kind == hipMemcpyHostToDevice && (e = hipStreamSynchronize(ctx->stream)) != hipSuccess)
+		return hip_check(ctx, e, "scene update");
+	return SVO_OK;
+}
+
+int svo_scene_update(svo_ctx *ctx, svo_scene *scene, const float *tri_xyz, const uint32_t *tri_rgb, int normalize) {
+	return scene_update(ctx, scene, tri_xyz, tri_rgb, normalize, hipMemcpyHostToDevice);
+}
+
+int svo_scene_update_device(svo_ctx *ctx, svo_scene *scene, const float *tri_xyz_dev, const uint32_t *tri_rgb_dev,
+                            int normalize) {
+	return scene_update(ctx, scene, tri_xyz_dev, tri_rgb_dev, normalize, hipMemcpyDeviceToDevice);
+}
+
 int svo_scene_info(const svo_scene *s, uint64_t *T, const float **xyz, const uint32_t **rgb) {
 	if (!s)
 		return SVO_ERR_INVALID_ARGUMENT;
@@ -848,6 +886,101 @@ int svo_build_batch(svo_ctx *ctx, const svo_fraglist *const *fls, uint32_t n, sv
 		}
 	}
 	return rc;
+}
+
+// ---- captured voxelize + build
+int svo_build_plan_create(svo_ctx *ctx, const svo_scene *const *scenes, const uint32_t *levels, uint32_t n,
+                          const uint64_t *frag_caps, const uint64_t *word_caps, svo_build_plan **out) {
+	if (!ctx || !out || !n || !scenes || !levels)
+		return SVO_ERR_INVALID_ARGUMENT;
+	*out = nullptr;
+	if (n > kMaxBatch)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "plan larger than 1024 scenes");
+	for (uint32_t i = 0; i < n; ++i) {
+		if (!scenes[i] || scenes[i]->ctx != ctx)
+			return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "every scene of a plan must live on the plan's context");
+		if (levels[i] < 1 || levels[i] > 12)
+			return set_error(ctx, SVO_ERR_BAD_LEVEL, "level must be in [1, 12]");
+	}
+	DeviceGuard g(ctx);
+	std::vector<uint64_t> fc(n), wc(n);
+	if (!frag_caps || !word_caps) {
+		// capacities from one plain voxelize + build of the scenes as they are now, plus a quarter
+		std::vector<svo_fraglist *> fl(n, nullptr);
+		std::vector<svo_octree *> oc(n, nullptr);
+		int rc = svo_voxelize_batch(ctx, scenes, levels, n, fl.data());
+		if (!rc)
+			rc = svo_build_batch(ctx, fl.data(), n, oc.data());
+		for (uint32_t i = 0; i < n; ++i) {
+			if (!rc) {
+				fc[i] = fl[i]->F + fl[i]->F / 4u + 64u;
+				wc[i] = oc[i]->W + oc[i]->W / 4u + 64u;
+			}
+			if (oc[i])
+				svo_octree_destroy(oc[i]);
+			if (fl[i])
+				svo_fraglist_destroy(fl[i]);
+		}
+		if (rc)
+			return rc;
+	}
+	for (uint32_t i = 0; i < n; ++i) {
+		if (frag_caps)
+			fc[i] = frag_caps[i];
+		if (word_caps)
+			wc[i] = word_caps[i];
+	}
+	svo_build_plan *p = new (std::nothrow) svo_build_plan();
+	if (!p)
+		return SVO_ERR_OUT_OF_MEMORY;
+	p->ctx = ctx;
+	int rc = build_plan_init(ctx, &p->p, scenes, levels, n, fc.data(), wc.data());
+	if (rc) {
+		svo_build_plan_destroy(p);
+		return rc;
+	}
+	*out = p;
+	return SVO_OK;
+}
+
+int svo_build_plan_use_graph(svo_build_plan *plan, int on) {
+	if (!plan)
+		return SVO_ERR_INVALID_ARGUMENT;
+	plan->p.use_graph = on != 0;
+	return SVO_OK;
+}
+
+int svo_build_plan_run(svo_build_plan *plan) {
+	if (!plan)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(plan->ctx);
+	return build_plan_run(plan->ctx, &plan->p);
+}
+
+int svo_build_plan_finish(svo_build_plan *plan, uint64_t *F, uint64_t *W, double *run_ms) {
+	if (!plan)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(plan->ctx);
+	return build_plan_finish(plan->ctx, &plan->p, F, W, run_ms);
+}
+
+int svo_build_plan_get(const svo_build_plan *plan, uint32_t i, const svo_fraglist **frags, const svo_octree **oct) {
+	if (!plan || i >= plan->p.S)
+		return SVO_ERR_INVALID_ARGUMENT;
+	if (frags)
+		*frags = plan->p.fls[i];
+	if (oct)
+		*oct = plan->p.octs[i];
+	return SVO_OK;
+}
+
+int svo_build_plan_destroy(svo_build_plan *plan) {
+	if (!plan)
+		return SVO_ERR_INVALID_ARGUMENT;
+	DeviceGuard g(plan->ctx);
+	build_plan_release(plan->ctx, &plan->p);
+	delete plan;
+	return SVO_OK;
 }
 
 int svo_octree_create(svo_ctx *ctx, const uint32_t *words, uint64_t W, uint32_t level, svo_octree **out) {

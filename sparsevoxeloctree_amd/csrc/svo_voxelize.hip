@@ -336,10 +336,13 @@ __device__ __forceinline__ uint2 make_frag(const Setup &s, int32_t i, int32_t j,
 
 // Fill of the small triangles: each lane walks its triangle's rows and writes its fragments at the
 // triangle's offset (triangle order, row-major inside the triangle).  Offsets are global over the
-// batch; Ftot is the batch's fragment total.
+// batch; fend = {the batch's fragment total, the fill limit}: a triangle whose fragments end past
+// the limit is not written (the limit is the total, or 0 when a captured plan found a fragment
+// count above its list's capacity).
 __global__ void __launch_bounds__(kThreads)
     vox_fill(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
-             const uint64_t *__restrict__ offsets, uint64_t Ftot) {
+             const uint64_t *__restrict__ offsets, const uint64_t *__restrict__ fend) {
+	const uint64_t Ftot = fend[0], lim = fend[1];
 	const uint64_t stride = (uint64_t)gridDim.x * kThreads;
 	uint32_t hint = 0;
 	for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < T; base += stride) {
@@ -348,7 +351,7 @@ __global__ void __launch_bounds__(kThreads)
 			continue;
 		uint64_t k = offsets[t];
 		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : Ftot;
-		if (k == kend || kend - k > kBigTri)
+		if (k == kend || kend - k > kBigTri || kend > lim)
 			continue;
 		const float *xyz;
 		const uint32_t *rgb;
@@ -391,14 +394,20 @@ constexpr int kRowChunk = 1024;
 
 __global__ void __launch_bounds__(kThreads)
     vox_fill_big(const VoxSeg *__restrict__ segs, const uint64_t *__restrict__ tbeg, uint32_t S, uint64_t T,
-                 const uint64_t *__restrict__ offsets, const uint32_t *__restrict__ big, uint64_t Ftot) {
+                 const uint64_t *__restrict__ offsets, const uint32_t *__restrict__ big,
+                 const uint64_t *__restrict__ fend) {
 	__shared__ int32_t s_i0[kRowChunk];
 	__shared__ uint32_t s_off[kRowChunk + 1];
 	__shared__ uint32_t s_wave[kThreads / 64];
+	const uint64_t Ftot = fend[0], lim = fend[1];
 	const uint32_t nbig = big[0];
 	for (uint32_t b = blockIdx.x; b < nbig; b += gridDim.x) {
 		const uint64_t t = big[1 + b];
 		if (t >= T)
+			continue;
+		uint64_t k = offsets[t];
+		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : Ftot;
+		if (kend > lim) // block-uniform: t is the block's triangle
 			continue;
 		const uint32_t sg = S == 1u ? 0u : seg_find(tbeg, S, t);
 		const VoxSeg &g = segs[sg];
@@ -409,8 +418,6 @@ __global__ void __launch_bounds__(kThreads)
 		Setup s;
 		tri_setup(g.xyz, tl, res, s);
 		const uint32_t colour = g.rgb[tl] & 0x00ffffffu;
-		uint64_t k = offsets[t];
-		const uint64_t kend = (t + 1 < T) ? offsets[t + 1] : Ftot;
 		for (int32_t r0 = s.jlo; r0 <= s.jhi && k < kend; r0 += kRowChunk) {
 			const int32_t nrows = (s.jhi - r0 + 1) < kRowChunk ? (s.jhi - r0 + 1) : kRowChunk;
 			// spans and counts, kRowChunk / kThreads rows per thread
@@ -599,6 +606,7 @@ void vox_batch_release(svo_ctx *ctx, VoxBatch *vb) {
 		ctx->pool->free(p);
 	vb->segs = nullptr;
 	vb->tbeg = nullptr;
+	vb->fend = nullptr;
 	vb->offsets = nullptr;
 	vb->big = nullptr;
 }
@@ -630,14 +638,15 @@ int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, Vox
 		return set_error(ctx, SVO_ERR_OVERFLOW, "more than 2^32-1 triangles in one voxelize call");
 	Pool *pool = ctx->pool;
 	hipError_t e;
-	// descriptors, triangle begins and the big-triangle list share one allocation, so one copy
-	// uploads the tables and clears the list's count
+	// descriptors, triangle begins (+ the fill's {total, limit}) and the big-triangle list share one
+	// allocation, so one copy uploads the tables and clears the list's count
 	const size_t seg_bytes = (sizeof(VoxSeg) * S + 255) & ~size_t(255);
-	const size_t tbeg_bytes = (sizeof(uint64_t) * (S + 1) + 255) & ~size_t(255);
+	const size_t tbeg_bytes = (sizeof(uint64_t) * (S + 3) + 255) & ~size_t(255);
 	if (pool->alloc((void **)&vb->segs, seg_bytes + tbeg_bytes + sizeof(uint32_t) * (T + 1)) != hipSuccess ||
 	    pool->alloc((void **)&vb->offsets, sizeof(uint64_t) * (T ? T : 1)) != hipSuccess)
 		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "count-pass allocation failed");
 	vb->tbeg = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(vb->segs) + seg_bytes);
+	vb->fend = vb->tbeg + S + 1;
 	vb->big = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(vb->segs) + seg_bytes + tbeg_bytes);
 	{
 		std::vector<uint8_t> tables(seg_bytes + tbeg_bytes + sizeof(uint32_t), 0);
@@ -675,6 +684,7 @@ int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, Vox
 	ctx->t_count.stop(ctx->stream);
 	if ((rc = download_table(ctx, vb->h_segs.data(), vb->segs, sizeof(VoxSeg) * S)) != SVO_OK)
 		return rc;
+	uint64_t Ftot = 0;
 	for (uint32_t s = 0; s < S; ++s) {
 		VoxSeg &g = vb->h_segs[s];
 		if (g.F > 0xFFFFFFFFull)
@@ -684,8 +694,16 @@ int voxelize_count_batch(svo_ctx *ctx, svo_fraglist *const *fls, uint32_t S, Vox
 		if (pool->alloc((void **)&fl->frags, sizeof(uint32_t) * 2 * (g.F ? g.F : 1)) != hipSuccess)
 			return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "fragment list allocation failed");
 		g.frags = reinterpret_cast<uint2 *>(fl->frags);
+		Ftot += g.F;
 	}
-	return upload_table(ctx, vb->segs, vb->h_segs.data(), sizeof(VoxSeg) * S);
+	// the descriptors with their lists, and the fill's {total, limit} = {Ftot, Ftot}
+	std::vector<uint8_t> tables(seg_bytes + tbeg_bytes, 0);
+	std::memcpy(tables.data(), vb->h_segs.data(), sizeof(VoxSeg) * S);
+	vb->h_tbeg.push_back(Ftot);
+	vb->h_tbeg.push_back(Ftot);
+	std::memcpy(tables.data() + seg_bytes, vb->h_tbeg.data(), sizeof(uint64_t) * (S + 3));
+	vb->h_tbeg.resize(S + 1);
+	return upload_table(ctx, vb->segs, tables.data(), tables.size());
 }
 
 // Fill pass of a counted batch (CmdVoxelize, src/Voxelizer.cpp:166-179).
@@ -699,15 +717,87 @@ int voxelize_fill_batch(svo_ctx *ctx, VoxBatch *vb) {
 	ctx->t_fill.start(ctx->stream);
 	if (T > 0 && Ftot > 0) {
 		if ((e = launch(ctx, "vox_fill", vox_fill, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const VoxSeg *)vb->segs,
-		                (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets, Ftot)) != hipSuccess)
+		                (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets, (const uint64_t *)vb->fend)) !=
+		    hipSuccess)
 			return hip_check(ctx, e, "vox_fill");
 		if ((e = launch(ctx, "vox_fill_big", vox_fill_big, dim3((uint32_t)ctx->cu_count * 4u), dim3(kThreads), 0,
 		                (const VoxSeg *)vb->segs, (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets,
-		                (const uint32_t *)vb->big, Ftot)) != hipSuccess)
+		                (const uint32_t *)vb->big, (const uint64_t *)vb->fend)) != hipSuccess)
 			return hip_check(ctx, e, "vox_fill_big");
 	}
 	ctx->t_fill.stop(ctx->stream);
 	return SVO_OK;
+}
+
+// Tables of a captured plan's voxelize pass: the lists are allocated by the plan at their
+// capacities, so the descriptors (scene buffers, list pointers, levels) never change; F and
+// frag_begin are rewritten by every count pass.
+int voxelize_plan_init(svo_ctx *ctx, VoxBatch *vb, svo_fraglist *const *fls, uint32_t S) {
+	vb->S = S;
+	vb->h_segs.assign(S, VoxSeg{});
+	vb->h_tbeg.assign(S + 1, 0);
+	uint64_t T = 0;
+	for (uint32_t s = 0; s < S; ++s) {
+		VoxSeg &g = vb->h_segs[s];
+		g.xyz = fls[s]->scene->xyz;
+		g.rgb = fls[s]->scene->rgb;
+		g.T = fls[s]->scene->T;
+		g.level = fls[s]->level;
+		g.frags = reinterpret_cast<uint2 *>(fls[s]->frags);
+		vb->h_tbeg[s] = T;
+		T += g.T;
+	}
+	vb->h_tbeg[S] = T;
+	vb->Ttot = T;
+	if (T == 0 || T > 0xFFFFFFFFull)
+		return set_error(ctx, SVO_ERR_INVALID_ARGUMENT, "a captured plan needs 1 to 2^32-1 triangles in all");
+	const size_t seg_bytes = (sizeof(VoxSeg) * S + 255) & ~size_t(255);
+	const size_t tbeg_bytes = (sizeof(uint64_t) * (S + 3) + 255) & ~size_t(255);
+	if (ctx->pool->alloc((void **)&vb->segs, seg_bytes + tbeg_bytes + sizeof(uint32_t) * (T + 1)) != hipSuccess ||
+	    ctx->pool->alloc((void **)&vb->offsets, sizeof(uint64_t) * T) != hipSuccess)
+		return set_error(ctx, SVO_ERR_OUT_OF_MEMORY, "plan voxelize allocation failed");
+	vb->tbeg = reinterpret_cast<uint64_t *>(reinterpret_cast<uint8_t *>(vb->segs) + seg_bytes);
+	vb->fend = vb->tbeg + S + 1;
+	vb->big = reinterpret_cast<uint32_t *>(reinterpret_cast<uint8_t *>(vb->segs) + seg_bytes + tbeg_bytes);
+	std::vector<uint8_t> tables(seg_bytes + tbeg_bytes, 0);
+	std::memcpy(tables.data(), vb->h_segs.data(), sizeof(VoxSeg) * S);
+	std::memcpy(tables.data() + seg_bytes, vb->h_tbeg.data(), sizeof(uint64_t) * (S + 1));
+	return upload_table(ctx, vb->segs, tables.data(), tables.size());
+}
+
+// Count pass of a plan: big-triangle list cleared by a kernel (the pass may be captured, so no
+// memset), counts, scan into offsets and fend[0] = total, per-scene F and frag_begin.
+hipError_t voxelize_plan_count(svo_ctx *ctx, VoxBatch *vb) {
+	const uint64_t T = vb->Ttot;
+	const uint32_t S = vb->S;
+	hipError_t e;
+	if ((e = launch(ctx, "vox_zero", vox_zero_counter, dim3(1), dim3(1), 0, vb->big)) != hipSuccess ||
+	    (e = launch(ctx, "vox_count", vox_count, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const VoxSeg *)vb->segs,
+	                (const uint64_t *)vb->tbeg, S, T, vb->offsets, vb->big)) != hipSuccess)
+		return e;
+	CountSrc src{vb->offsets, T};
+	if ((e = launch(ctx, "vox_scan_reduce", scan_reduce<CountSrc>, dim3(kScanBlocks), dim3(kScanThreads), 0, src,
+	                ctx->d_scan_parts)) != hipSuccess ||
+	    (e = launch(ctx, "vox_scan_down", scan_down<CountSrc, OffsetDst, TotalCommit>, dim3(kScanBlocks),
+	                dim3(kScanThreads), 0, src, OffsetDst{vb->offsets}, (const uint64_t *)ctx->d_scan_parts,
+	                TotalCommit{vb->fend})) != hipSuccess)
+		return e;
+	return launch(ctx, "vox_seg_totals", vox_seg_totals, dim3(div_up(S, 64)), dim3(64), 0, vb->segs,
+	              (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets, (const uint64_t *)vb->fend);
+}
+
+// Fill pass of a plan: writes nothing when fend[1] (set between the passes) is 0.
+hipError_t voxelize_plan_fill(svo_ctx *ctx, VoxBatch *vb) {
+	const uint64_t T = vb->Ttot;
+	const uint32_t S = vb->S;
+	hipError_t e;
+	if ((e = launch(ctx, "vox_fill", vox_fill, dim3(grid_for(T, ctx)), dim3(kThreads), 0, (const VoxSeg *)vb->segs,
+	                (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets, (const uint64_t *)vb->fend)) !=
+	    hipSuccess)
+		return e;
+	return launch(ctx, "vox_fill_big", vox_fill_big, dim3((uint32_t)ctx->cu_count * 4u), dim3(kThreads), 0,
+	              (const VoxSeg *)vb->segs, (const uint64_t *)vb->tbeg, S, T, (const uint64_t *)vb->offsets,
+	              (const uint32_t *)vb->big, (const uint64_t *)vb->fend);
 }
 
 } // namespace svo
