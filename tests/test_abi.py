@@ -147,3 +147,21 @@ def test_path_trace_and_obj_argument_checks():
     assert lib.svo_envmap_download(None, None, None, None) == 1
     assert lib.svo_envmap_destroy(None) == 1
     assert lib.svo_path_trace_env(None, None, None, 8, 8, 4, 0, None, 0.0, 1.0, None, None, None) == 1
+
+
+def test_build_plan_and_scene_update_argument_checks():
+    # argument errors before any device work (no GPU needed)
+    import ctypes
+
+    from sparsevoxeloctree_amd._capi import lib
+
+    plan = ctypes.c_void_p()
+    lv = (ctypes.c_uint32 * 1)(5)
+    assert lib.svo_build_plan_create(None, None, lv, 1, None, None, ctypes.byref(plan)) == 1 and not plan
+    assert lib.svo_build_plan_use_graph(None, 1) == 1
+    assert lib.svo_build_plan_run(None) == 1
+    assert lib.svo_build_plan_finish(None, None, None, None) == 1
+    assert lib.svo_build_plan_get(None, 0, None, None) == 1
+    assert lib.svo_build_plan_destroy(None) == 1
+    assert lib.svo_scene_update(None, None, None, None, 1) == 1
+    assert lib.svo_scene_update_device(None, None, None, None, 1) == 1
