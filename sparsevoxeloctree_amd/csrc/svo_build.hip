@@ -1592,8 +1592,9 @@ int build_plan_init(svo_ctx *ctx, BuildPlan *p, const svo_scene *const *scenes, 
 		return rc;
 	if (int rc = upload_table(ctx, p->caps, caps.data(), sizeof(uint64_t) * 2 * S))
 		return rc;
+	// results before the first run: status 0, no fragments, no words
 	p->h_out.assign(1 + 2 * (size_t)S, 0);
-	return SVO_OK;
+	return upload_table(ctx, p->out, p->h_out.data(), sizeof(uint64_t) * p->h_out.size());
 }
 
 int build_plan_run(svo_ctx *ctx, BuildPlan *p) {

@@ -79,6 +79,8 @@ def test_plan_equals_batch_and_oracle(ctx, no_device_errors, geometry):
     scenes = [S.Scene.Create(ctx, x, c) for x, c in geometry]
     print("plan over %d scenes, measured capacities" % len(scenes), flush=True)
     plan = S.BuildPlan(scenes, levels)
+    plan.finish()  # before any run: no counts, no error
+    assert plan.F == [0] * plan.n and plan.W == [0] * plan.n
     frags, words = _reference(ctx, geometry, levels)
     for rep in range(3):  # replays of one graph, with eager work and waits between them
         plan.run()
