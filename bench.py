@@ -705,6 +705,20 @@ def bench_primary(args, cfg, cpu=True):
 
     st = {k: float(np.mean([r["stage"][k] for r in results])) for k in results[0]["stage"]}
     F, W = results[-1]["F"], results[-1]["W"]
+    peak_plain = ctx.memory()["peak_in_use"]  # the allocator's peak over the plain steps
+    # the same voxelize + build in its steady-state form: captured once as a HIP graph with every
+    # count kept on the device (svo_build_plan) and replayed; device ms of a replay and host ms of
+    # run + finish, medians over args.steps replays after one warm-up.  Checked below by its
+    # counts and by the hit records of its octree.
+    bplan = S.BuildPlan([scene], [cfg["level"]])
+    bplan.run()
+    bplan.finish()
+    cap_dev, cap_host = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        bplan.run()
+        cap_dev.append(bplan.finish())
+        cap_host.append(time.perf_counter() - t1)
     march_ms = st["beam_ms"] + st["march_ms"]
     value = R / (march_ms * 1e-3) / 1e6
     build_ref = st["vox_fill_ms"] + st["build_ms"]
@@ -716,6 +730,12 @@ def bench_primary(args, cfg, cpu=True):
     nfetch_primary = int(hits["nfetch"].astype(np.uint64).sum())
     from sparsevoxeloctree_amd._capi import lib
 
+    cap_hits = ctx.alloc(R * 32)
+    ctx.check(lib.svo_march(ctx.h, bplan.octree(0).h, C_cam(cam), cfg["width"], cfg["height"], 1, cap_hits.ptr, None),
+              "march of the plan's octree")
+    captured_same = bool(bplan.F == [F] and bplan.W == [W] and np.array_equal(
+        cap_hits.download(S.HIT_DTYPE, R).view(np.uint32), hits.view(np.uint32)))
+    del cap_hits, bplan
     nf_beam_buf = ctx.alloc(bw * bh * 4)
     beam_tmp = ctx.alloc(bw * bh * 4)
     ctx.check(lib.svo_march_beam(ctx.h, octree.h, C_cam(cam), cfg["width"], cfg["height"], beam_tmp.ptr, nf_beam_buf.ptr))
@@ -810,6 +830,9 @@ def bench_primary(args, cfg, cpu=True):
         "primary_ms": round(march_ms, 4),
         "build_ms_ref_scope": round(build_ref, 4),
         "build_ms_end_to_end": round(build_e2e, 4),
+        "build_ms_captured": round(float(np.median(cap_dev)), 4),
+        "build_ms_captured_host": round(float(np.median(cap_host)) * 1e3, 4),
+        "captured_identical": captured_same,
         "build_gbs_ref_scope": round((build_bytes + vox_bytes) / (build_ref * 1e-3) / 1e9, 2),
         "stage_ms": {k: round(v, 4) for k, v in st.items()},
         "kernel_ms": {k: round(v, 5) for k, v in sorted(kernel_ms.items(), key=lambda kv: -kernel_total[kv[0]])},
@@ -823,8 +846,7 @@ def bench_primary(args, cfg, cpu=True):
         env = synthetic_envmap(S, ctx)
         line["path_trace_env"] = dict(path_trace_rate(S, ctx, octree, cam, cfg, env=env), envmap="2048x1024")
         env.close()
-    mem = ctx.memory()
-    line["peak_pool_bytes"] = mem["peak_in_use"]
+    line["peak_pool_bytes"] = peak_plain
     line["autotune"] = S.autotune(ctx, True).strip().splitlines()
     line["reference_lavapipe"] = "unavailable: no Vulkan loader, lavapipe, Xvfb or glslc on the host"
     if cpu:
@@ -883,8 +905,8 @@ def main():
     if args.config == 2 and not args.no_extra:
         # the metric's other headline figures: the 2^12 build and 3840x2160 frame (c3), the small
         # plain case (c1), batched (c4) and served (c5), each at its own configuration
-        keys = ("build_ms_ref_scope", "build_ms_end_to_end", "build_gbs_ref_scope", "primary_mrays_per_s",
-                "primary_ms", "peak_pool_bytes", "stage_ms", "roofline")
+        keys = ("build_ms_ref_scope", "build_ms_end_to_end", "build_ms_captured", "captured_identical",
+                "build_gbs_ref_scope", "primary_mrays_per_s", "primary_ms", "peak_pool_bytes", "stage_ms", "roofline")
         line["c3"] = summarise_extra(bench_primary_extra, args, CONFIGS[3], keys + ("config",))
         line["c1"] = summarise_extra(bench_primary_extra, args, CONFIGS[1], keys + ("config",))
         line["batched"] = summarise_extra(bench_batch, args, CONFIGS[4], (

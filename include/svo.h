@@ -179,7 +179,7 @@ int svo_build_batch(svo_ctx *ctx, const svo_fraglist *const *frags, uint32_t n, 
  * src/OctreeBuilder.cpp:142-214).  A plan over n scenes at fixed levels allocates each scene's
  * fragment list and octree at a capacity: frag_caps[i] fragments and word_caps[i] words (either
  * array may be NULL: then one plain voxelize + build of the scenes as they are now sizes it, plus
- * a quarter).  Every count stays on the device, so no pass waits for the host, and the passes are
+ * a quarter, words at most 2^30).  Every count stays on the device, so no pass waits for the host, and the passes are
  * captured once as a HIP graph.  svo_build_plan_run replays them on the context's stream; the
  * scenes' triangles may change between runs (svo_scene_update), their number may not.  A run whose
  * counts pass a capacity writes nothing past it.  svo_build_plan_finish waits for the last run,

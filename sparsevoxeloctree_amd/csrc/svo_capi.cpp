@@ -914,7 +914,7 @@ int svo_build_plan_create(svo_ctx *ctx, const svo_scene *const *scenes, const ui
 		for (uint32_t i = 0; i < n; ++i) {
 			if (!rc) {
 				fc[i] = fl[i]->F + fl[i]->F / 4u + 64u;
-				wc[i] = oc[i]->W + oc[i]->W / 4u + 64u;
+				wc[i] = std::min<uint64_t>(oc[i]->W + oc[i]->W / 4u + 64u, 1ull << 30); // 30-bit child pointers
 			}
 			if (oc[i])
 				svo_octree_destroy(oc[i]);
