@@ -187,8 +187,11 @@ int svo_build_batch(svo_ctx *ctx, const svo_fraglist *const *frags, uint32_t n, 
  * device time of the run), and returns SVO_ERR_OVERFLOW when a count passed its capacity: the
  * lists and octrees then hold no valid data until a run fits.  After a run that fits they are
  * identical to svo_voxelize_batch + svo_build_batch of the same scenes.  They belong to the plan
- * (svo_build_plan_get); march them after svo_build_plan_finish.  A march plan reads an octree's
- * size when it is created.  use_graph(plan, 0) enqueues the same passes as plain launches.
+ * (svo_build_plan_get).  The single-scene marches read an octree's size from the host, so they
+ * follow svo_build_plan_finish; a march plan made once over the plan's octrees stays valid across
+ * runs and needs no finish in between (both are ordered on the context's stream), so a
+ * steady-state loop can replay both graphs: build plan run, march plan run.  use_graph(plan, 0)
+ * enqueues the same passes as plain launches.
  * n <= 1024; the scenes must outlive the plan. */
 typedef struct svo_build_plan svo_build_plan;
 int svo_build_plan_create(svo_ctx *ctx, const svo_scene *const *scenes, const uint32_t *levels, uint32_t n,

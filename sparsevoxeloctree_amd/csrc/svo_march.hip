@@ -1714,7 +1714,10 @@ constexpr uint32_t kRayBlock[3] = {256, 128, 64};
 MarchSeg march_seg(const svo_octree *o, const svo_camera *cam) {
 	MarchSeg g{};
 	g.oct = o->words;
-	g.W = o->W;
+	// node loads are bounds-checked against the allocation: an octree of a build plan keeps its
+	// buffer while its used size changes from run to run, so a march plan made over it once stays
+	// valid (every other octree's allocation is its used size)
+	g.W = o->cap > o->W ? o->cap : o->W;
 	if (cam)
 		g.cam = cam_args(cam);
 	g.orig_sz = ldexpf(1.0f, 1 - (int)o->level); // exp2f(1.0f - level), src/OctreeTracer.cpp:206

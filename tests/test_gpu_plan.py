@@ -199,3 +199,44 @@ def test_plan_octrees_march(ctx, no_device_errors, geometry):
     for i in range(plan.n):
         ok, msg = hits_equal(got[i], ref[i])
         assert ok, "octree %d: %s" % (i, msg)
+
+
+def test_plan_then_march_plan_steady_state(ctx, no_device_errors, geometry):
+    """The whole steady-state loop from two graphs: new geometry, build plan run, then a march plan
+    made once over the build plan's octrees, with no finish in between; every frame equals a
+    fresh batched build + march of the same geometry."""
+    import sparsevoxeloctree_amd as S
+
+    levels = [lv for _, lv in SCENES]
+    variants = [_variant(geometry, k) for k in range(3)]
+    refs = [_reference(ctx, v, levels) for v in variants]
+    fcap = [max(r[0][i].shape[0] for r in refs) for i in range(len(SCENES))]
+    wcap = [max(r[1][i].shape[0] for r in refs) for i in range(len(SCENES))]
+    scenes = [S.Scene.Create(ctx, x, c) for x, c in variants[0]]
+    plan = S.BuildPlan(scenes, levels, fcap, wcap)
+    plan.run()
+    plan.finish()
+    W, H = 48, 32
+    cams = []
+    for i in range(len(scenes)):
+        a = 0.9 * i + 0.3
+        eye = (1.5 + 1.6 * math.cos(a), 1.5 + 0.4 * math.sin(1.7 * a), 1.5 + 1.6 * math.sin(a))
+        cams.append(S.camera_look_at(eye, (1.5, 1.5, 1.5), fov=math.pi / 3, aspect=W / H).uniform())
+    mp = S.MarchPlan([plan.octree(i) for i in range(plan.n)], cams, W, H, beam=True)
+    mp.use_graph(True)
+    hits = mp.alloc_hits()
+    for k in [1, 2, 0, 2]:
+        print("steady state variant %d" % k, flush=True)
+        for sc, (x, c) in zip(scenes, variants[k]):
+            if sc.T:
+                sc.update(x, c)
+        plan.run()
+        mp.run(hits)
+        plan.finish()
+        got = hits.download(S.HIT_DTYPE, plan.n * W * H).reshape(plan.n, H, W)
+        ref_scenes = [S.Scene.Create(ctx, x, c) for x, c in variants[k]]
+        ref_octs = S.build_batch(S.voxelize_batch(ref_scenes, levels))
+        ref = S.march_batch(ref_octs, cams, W, H, beam=True)
+        for i in range(plan.n):
+            ok, msg = hits_equal(got[i], ref[i])
+            assert ok, "variant %d octree %d: %s" % (k, i, msg)
