@@ -162,6 +162,14 @@ def test_c3_full_size(ctx, no_device_errors):
     finally:
         ctx.check(lib.svo_march_mode(ctx.h, -1))
     print("c3: persistent schedule identical", flush=True)
+    # the same voxelize + build as a captured build plan replay: counts and every octree word
+    F = vox.GetVoxelFragmentCount()
+    plan = S.BuildPlan([keep[0]], [level])
+    plan.run()
+    plan.finish()
+    assert plan.F == [F] and plan.W == [octree.W], (plan.F, plan.W, F, octree.W)
+    assert np.array_equal(plan.octree(0).words(), words), "captured plan's octree differs"
+    print("c3: captured build plan identical", flush=True)
 
 
 def test_c4_all_scenes_batched_single_and_oracle(ctx, no_device_errors):
