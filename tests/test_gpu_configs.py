@@ -173,8 +173,9 @@ def test_c3_full_size(ctx, no_device_errors):
 
 
 def test_c4_all_scenes_batched_single_and_oracle(ctx, no_device_errors):
-    # all 256 scenes: batched voxelize / build / march against one scene per launch, every octree
-    # word and hit record; then the smallest scene of every level 6..11 against the oracle
+    # all 256 scenes: batched voxelize / build / march and a captured build plan against one scene
+    # per launch, every octree word and hit record; then the smallest scene of every level 6..11
+    # against the oracle
     import sparsevoxeloctree_amd as S
     from sparsevoxeloctree_amd._capi import lib
 
@@ -198,6 +199,11 @@ def test_c4_all_scenes_batched_single_and_oracle(ctx, no_device_errors):
     octs = S.build_batch(voxs)
     bhits = S.march_batch(octs, cams, W, H, beam=True)
     print("c4: batched done", flush=True)
+    # the same voxelize + build as one captured build plan over all 256 scenes
+    plan = S.BuildPlan(scenes, levels)
+    plan.run()
+    plan.finish()
+    print("c4: captured plan done", flush=True)
     frame = ctx.alloc(W * H * 32)
     for i in range(n):
         vox = S.Voxelizer.Create(scenes[i], levels[i])
@@ -207,6 +213,7 @@ def test_c4_all_scenes_batched_single_and_oracle(ctx, no_device_errors):
         o = b.GetOctree()
         words = o.words()
         assert np.array_equal(words, octs[i].words()), "scene %d: batched octree differs" % i
+        assert np.array_equal(words, plan.octree(i).words()), "scene %d: captured plan's octree differs" % i
         ctx.check(lib.svo_march(ctx.h, o.h, C.byref(cams[i].uniform()), W, H, 1, frame.ptr, None))
         single = frame.download(S.HIT_DTYPE, W * H)
         ok, msg = hits_equal(bhits[i].reshape(-1), single)
