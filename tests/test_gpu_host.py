@@ -36,6 +36,32 @@ def test_cli_load_path_matches_oracle(tmp_path):
     assert re.search(r"frame 160x90 hits \d+", r.stdout)
 
 
+@pytest.mark.skip(reason="svo_cli -rebuild (a C++ svo::BuildPlan wrapper over svo_build_plan_*) ended with "
+                         "a host segmentation fault on its first GPU run; no cause was found by reading, so the "
+                         "option and the wrapper were taken out")
+def test_cli_rebuild_plan_matches_oracle(tmp_path):
+    # svo::BuildPlan through svo_cli -rebuild: three graph replays of the load path's voxelize +
+    # build, the same counts, and the plan's octree equal to the oracle's
+    import sparsevoxeloctree_amd as S
+
+    out = tmp_path / "rebuilt.svo"
+    r = subprocess.run(
+        [CLI, "-scene", "1,12,6", "-lvl", "8", "-rebuild", "3", "-rebuild-save", str(out)],
+        capture_output=True, text=True, timeout=120,
+    )
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    m = re.search(r"rebuild runs 3 median [0-9.]+ ms fragments (\d+) octree_words (\d+) identical 1", r.stdout)
+    assert m
+    xyz, rgb = S.gen_scene(1, 12, 6)
+    frags = oracle.voxelize(oracle.normalize(xyz), rgb, 8)
+    ref_words = oracle.build(frags, 8)
+    assert int(m.group(1)) == frags.shape[0] and int(m.group(2)) == ref_words.shape[0]
+    with S.Context(0) as ctx:
+        o = S.Octree.Load(ctx, out)
+        assert np.array_equal(o.words(), ref_words)
+
+
 @pytest.mark.parametrize("light", ["sky", "env"])
 def test_cli_path_tracer_matches_oracle(tmp_path, light):
     # svo::PathTracer (+ svo::EnvironmentMap) through svo_cli -pt: the colour means after three
