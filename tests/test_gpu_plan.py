@@ -240,3 +240,26 @@ def test_plan_then_march_plan_steady_state(ctx, no_device_errors, geometry):
         for i in range(plan.n):
             ok, msg = hits_equal(got[i], ref[i])
             assert ok, "variant %d octree %d: %s" % (k, i, msg)
+
+
+def test_scene_update_from_device_buffers(ctx, no_device_errors, geometry):
+    """svo_scene_update_device: new triangles from device buffers feed the next run exactly as a
+    scene created from them does."""
+    import sparsevoxeloctree_amd as S
+    from sparsevoxeloctree_amd._capi import lib
+
+    levels = [lv for _, lv in SCENES]
+    turned = _variant(geometry, 2)
+    frags, words = _reference(ctx, turned, levels)
+    scenes = [S.Scene.Create(ctx, x, c) for x, c in geometry]
+    plan = S.BuildPlan(scenes, levels, [f.shape[0] + 64 for f in frags], [w.shape[0] + 64 for w in words])
+    keep = []
+    for sc, (x, c) in zip(scenes, turned):
+        if not sc.T:
+            continue
+        xd, cd = ctx.upload(np.ascontiguousarray(x, np.float32)), ctx.upload(np.ascontiguousarray(c, np.uint32))
+        keep.append((xd, cd))
+        ctx.check(lib.svo_scene_update_device(ctx.h, sc.h, xd.ptr, cd.ptr, 1), "scene update from the device")
+    plan.run()
+    plan.finish()
+    _check(plan, frags, words, "device update")
